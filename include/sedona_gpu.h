@@ -1,0 +1,448 @@
+/*
+ * sedona_gpu.h -- C-ABI of the MI355X implicit-Monte-Carlo transport hot path.
+ *
+ * Plain C: extern "C", plain pointers and sizes, no exceptions across the
+ * boundary, no HIP or torch types in the signatures.  Every compute entry
+ * point is stream-ordered on the context's stream and is called from one host
+ * thread; it replaces a whole reference loop, never a per-packet call.
+ *
+ * Memory: every array pointer handed to an sgpu_* compute call is a DEVICE
+ * pointer (allocate with sgpu_alloc, hipMalloc or a torch tensor).  Tables
+ * are contiguous and zone-major: table[zone * n_nu + bin].
+ *
+ * Status codes follow the reference's conventions:
+ *   GasState::solve_state returns 0 ok / 1 root not bracketed / 2 max iter
+ *   (src/opacity/GasState.cpp:228-231, 281-282, 335); negative values are
+ *   argument or HIP errors of this library; sgpu_last_error() gives the text.
+ *
+ * Reference interfaces replaced (paths relative to the Sedona source tree):
+ *   sgpu_opacity_build      transport::set_opacity            src/transport/transport_opacity.cpp:21-260
+ *                           (GasState::solve_state + computeOpacity +
+ *                            get_planck_mean / get_rosseland_mean)
+ *                                                             src/opacity/GasState.cpp:208-337
+ *                                                             src/opacity/GasState_opacities.cpp:11-136,464-494,564-666
+ *   sgpu_emit_radioactive   transport::emit_radioactive       src/transport/emission.cpp:165-243
+ *   sgpu_propagate          the packet loop of transport::step src/transport/transport.cpp:57-88
+ *                           + propagate / propagate_monte_carlo :202-484
+ *   sgpu_compact            transport::clean_up_particle_vector src/transport/transport.cpp:164-194
+ *   sgpu_wipe_radiation     transport::wipe_radiation          src/transport/transport_mpi.cpp:18-34
+ *   sgpu_normalize_tallies  transport::reduce_radiation (normalisation part)
+ *                                                             src/transport/transport_mpi.cpp:415-441
+ *   sgpu_nlte_solve         AtomicSpecies::solve_nlte (matrix assembly + LU)
+ *                                                             src/opacity/AtomicSpecies_state.cpp:131-196
+ *   sgpu_nlte_batched       (no reference counterpart: all systems of a step in one launch)
+ *   sgpu_homologous_step    hydro_homologous::step + grid expand src/hydro/hydro_homologous.cpp:27-61
+ *   sgpu_adjoint_*          (no reference counterpart: differentiable transport)
+ */
+#ifndef SEDONA_GPU_H
+#define SEDONA_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* status codes                                                        */
+/* ------------------------------------------------------------------ */
+#define SGPU_OK              0
+#define SGPU_NOT_BRACKETED   1   /* GasState::ne_brent_method error 1 */
+#define SGPU_MAX_ITER        2   /* GasState::ne_brent_method error 2 */
+#define SGPU_ERR_ARG        -1
+#define SGPU_ERR_HIP        -2
+#define SGPU_ERR_SIZE       -3
+#define SGPU_ERR_NOT_CONVERGED -4
+
+/* packet type / fate, same numbering as src/transport/particle.h:9-10 */
+#define SGPU_PHOTON   0
+#define SGPU_GAMMARAY 1
+#define SGPU_MOVING   0
+#define SGPU_STOPPED  1
+#define SGPU_ESCAPED  2
+#define SGPU_ABSORBED 3
+
+/* locate_array types (src/utils/locate_array.h:13) */
+#define SGPU_LOC_FLEX 0
+#define SGPU_LOC_LIN  1
+#define SGPU_LOC_LOG  2
+#define SGPU_LOC_NONE 3
+
+/* grid kinds */
+#define SGPU_GRID_1D_SPHERE 1
+#define SGPU_GRID_2D_CYLN   2
+
+/* propagation precision */
+#define SGPU_PREC_FP64  0   /* plain drop-in: FP64 everywhere            */
+#define SGPU_PREC_MIXED 1   /* FP32 geometry/opacity, FP64 tallies        */
+
+/* NLTE precision */
+#define SGPU_NLTE_FP64        0   /* FP64 partial-pivot LU                 */
+#define SGPU_NLTE_FP32_REFINE 1   /* FP32 LU + FP64 iterative refinement   */
+
+/* ------------------------------------------------------------------ */
+/* data descriptors (plain structs of device pointers and sizes)      */
+/* ------------------------------------------------------------------ */
+
+/* A binned axis, the data of locate_array (src/utils/locate_array.h:15-90):
+ * x[i] is the RIGHT edge of bin i, min the left edge of bin 0; bins are
+ * closed on the left, open on the right. */
+typedef struct {
+  int32_t n;          /* number of bins (= number of right edges)        */
+  int32_t type;       /* SGPU_LOC_*                                      */
+  double  min;        /* left edge of bin 0                              */
+  double  del;        /* lin: width; log: fractional step (1+del ratio)  */
+  const double* x;    /* [n] right edges                                 */
+} sgpu_locate;
+
+/* Homologous grid.  1D: r_out shells, zone i = shell i (grid_1D_sphere).
+ * 2D: cylindrical (x = varpi, z), zone index ix*nz + iz (grid_2D_cyln). */
+typedef struct {
+  int32_t kind;        /* SGPU_GRID_*                                    */
+  int32_t n_zones;
+  double  t_now;       /* grid clock: v = x / t_now, dv/ds = 1/t_now      */
+  double  r_core;      /* 1D inner absorbing core radius (0 = none)      */
+  sgpu_locate r_out;   /* 1D shell outer radii                            */
+  sgpu_locate x_out;   /* 2D varpi edges (nx bins)                        */
+  sgpu_locate z_out;   /* 2D z edges (nz bins)                            */
+  const double* vol;   /* [n_zones] zone volumes                          */
+} sgpu_grid;
+
+/* Per-zone gas state, SoA (zone record src/grid/zone.h:9-50). */
+typedef struct {
+  int32_t n_zones;
+  int32_t n_elems;
+  const double* rho;          /* [nz] g/cm^3                                      */
+  const double* T_gas;        /* [nz] K                                           */
+  const double* X;            /* [nz][n_elems] initial (undecayed) mass fractions */
+  const double* e_gas;        /* [nz] or NULL: only read when fleck_alpha != 0    */
+  const double* grey_opacity; /* [nz] total_grey_opacity (cm^2/g); 0 = non-grey   */
+  const int32_t* elem_Z;      /* [n_elems]                                        */
+  const int32_t* elem_A;      /* [n_elems]                                        */
+} sgpu_zone_state;
+
+/* Atomic data for the gas: one atom per element (GasState::atoms), levels
+ * and ions flattened over atoms; the fully ionised stage that the reference
+ * appends (src/opacity/AtomicData.cpp:429-446) is included by the caller as
+ * the last ion / level of each atom.  Fuzz lines (fuzz_line_structure,
+ * src/opacity/AtomicData.h:12-20) sorted by (atom, bin) with a CSR index. */
+typedef struct {
+  int32_t n_atoms;                 /* == n_elems of the zone state        */
+  int32_t max_levels;              /* max levels of any atom              */
+  int32_t n_levels;                /* levels over all atoms (lev_off[n_atoms]) */
+  int32_t n_ions;                  /* ions over all atoms (ion_off[n_atoms])   */
+  const int32_t* lev_off;          /* [n_atoms+1]                         */
+  const int32_t* ion_off;          /* [n_atoms+1]                         */
+  const double*  lev_E;            /* [n_lev] excitation energy (eV)      */
+  const int32_t* lev_g;            /* [n_lev] statistical weight          */
+  const int32_t* lev_ion;          /* [n_lev] ion stage within its atom   */
+  const double*  ion_chi;          /* [n_ion] ionisation energy (eV)      */
+  const int32_t* atom_zero_eps;    /* [n_atoms] 1: epsilon = 0 for atom   */
+  int64_t n_lines;                 /* fuzz lines                          */
+  const double*  line_nu;          /* [n_lines] Hz                        */
+  const double*  line_gf;          /* [n_lines]                           */
+  const double*  line_El;          /* [n_lines] lower level energy (eV)   */
+  const int32_t* line_ion;         /* [n_lines] ion stage within atom     */
+  const int32_t* line_bin;         /* [n_lines] nu-grid bin               */
+  const int64_t* line_bin_off;     /* [n_atoms*(n_nu+1)] CSR: lines of (atom a, bin b)
+                                      are [off[a*(n_nu+1)+b], off[a*(n_nu+1)+b+1]) */
+} sgpu_atoms;
+
+/* Opacity-build parameters (names of sedona_defaults.lua / transport members). */
+typedef struct {
+  double t_now;                    /* transport::t_now_ (decay and expansion time) */
+  double dt;                       /* step length (Fleck factor)                   */
+  double epsilon;                  /* opacity_epsilon                              */
+  double temp_min;                 /* limits_temp_min                              */
+  double temp_max;                 /* limits_temp_max                              */
+  double maximum_opacity;          /* opacity_maximum_opacity                      */
+  double line_velocity_width;      /* line_velocity_width                          */
+  double fleck_alpha;              /* transport_fleck_alpha                        */
+  int32_t use_electron_scattering; /* opacity_electron_scattering                  */
+  int32_t use_fuzz_expansion;      /* opacity_fuzz_expansion                       */
+  int32_t omit_scattering;         /* opacity_no_scattering                        */
+  int32_t omit_composition_decay;  /* dont_decay_composition                       */
+  int32_t radiative_eq;            /* transport_radiative_equilibrium              */
+  int32_t first_step;              /* transport::first_step_: the state solve and
+                                      opacities then use the UNclamped T_gas
+                                      (transport_opacity.cpp:109-120)            */
+} sgpu_opacity_params;
+
+/* Opacity-build outputs.  Tables [nz][n_nu]; scalars [nz]; optional
+ * per-(zone, ion) / per-(zone, level) state arrays may be NULL. */
+typedef struct {
+  double* abs;              /* abs_opacity_ (cm^-1)                              */
+  double* scat;             /* scat_opacity_                                     */
+  double* emis_cdf;         /* emissivity_ CDF, normalised to 1                  */
+  double* L_thermal;        /* zone L_thermal                                    */
+  double* planck_mean;      /* planck_mean_opacity_                              */
+  double* rosseland_mean;   /* rosseland_mean_opacity_                           */
+  double* n_elec;           /* zone n_elec                                       */
+  double* compton_opac;     /* gamma-ray Compton coefficient                     */
+  double* photoion_opac;    /* gamma-ray photo-absorption coefficient            */
+  double* eps_imc;          /* Fleck factor                                      */
+  int32_t* zone_status;     /* solve_state return value per zone                 */
+  double* ion_frac;         /* [nz][n_ion] or NULL                               */
+  double* ion_part;         /* [nz][n_ion] or NULL                               */
+  double* lev_n;            /* [nz][n_lev] or NULL                               */
+} sgpu_opacity_out;
+
+/* Monte Carlo packets, SoA (class particle, src/transport/particle.h:12-48).
+ * id and draw form the Philox-4x32-10 counter of the packet's private
+ * random stream: counter = {draw, 0, id_lo, id_hi}, key = seed. */
+typedef struct {
+  int64_t capacity;
+  double *x, *y, *z;        /* position (cm)                       */
+  double *Dx, *Dy, *Dz;     /* direction                           */
+  double *t;                /* time (s)                            */
+  double *e;                /* energy (erg)                        */
+  double *nu;               /* frequency (Hz; MeV for gamma-rays)  */
+  int32_t *ind;             /* zone                                */
+  int32_t *type;            /* SGPU_PHOTON / SGPU_GAMMARAY         */
+  int32_t *fate;            /* SGPU_MOVING ... SGPU_ABSORBED       */
+  uint32_t *draw;           /* Philox draw counter                 */
+  uint64_t *id;             /* Philox stream id                    */
+} sgpu_packets;
+
+/* Read-only opacity tables consumed by propagation. */
+typedef struct {
+  const double* abs;           /* [nz][n_nu] */
+  const double* scat;          /* [nz][n_nu] (NULL when omit_scattering) */
+  const double* emis_cdf;      /* [nz][n_nu] */
+  const double* eps_imc;       /* [nz] */
+  const double* compton_opac;  /* [nz] */
+  const double* photoion_opac; /* [nz] */
+} sgpu_tables;
+
+/* Escaping-packet histogram (spectrum_array, src/transport/spectrum_array.cpp:36-83). */
+typedef struct {
+  sgpu_locate time_grid;
+  sgpu_locate nu_grid;
+  sgpu_locate mu_grid;
+  sgpu_locate phi_grid;
+  double* flux;             /* [n_t*n_nu*n_mu*n_phi] */
+  double* click;            /* same                  */
+} sgpu_spectrum;
+
+/* Zone tallies (zone.h e_abs/e_rad/f*_rad/L_radio_*, transport::J_nu_). */
+typedef struct {
+  double* e_abs;
+  double* J_nu;             /* [nz][n_nu] (or [nz][1] when store_Jnu = 0) */
+  double* fx_rad;
+  double* fy_rad;
+  double* fz_rad;
+  double* fr_rad;
+  double* L_radio_dep;
+  double* L_radio_emit;
+  double* e_rad;
+  sgpu_spectrum optical;
+  sgpu_spectrum gamma;
+} sgpu_tallies;
+
+/* Transport parameters (transport members, transport.h:150-260). */
+typedef struct {
+  double  t_now;               /* transport::t_now_ at step start           */
+  double  dt;                  /* step (1 in steady state)                  */
+  uint64_t seed;               /* Philox key                                */
+  int32_t steady_state;        /* transport_steady_iterate > 0              */
+  int32_t radiative_eq;        /* transport_radiative_equilibrium          */
+  int32_t boundary_in_reflect; /* transport_boundary_in_reflect             */
+  int32_t boundary_out_reflect;/* transport_boundary_out_reflect            */
+  int32_t store_Jnu;           /* transport_store_Jnu                       */
+  int32_t omit_scattering;     /* opacity_no_scattering                     */
+  int32_t precision;           /* SGPU_PREC_*                               */
+  int32_t max_events;          /* 0 = unbounded; else packets still moving after
+                                  this many events are left 'moving'          */
+} sgpu_transport_params;
+
+/* Device counters of one propagation call (roofline inputs). */
+typedef struct {
+  uint64_t n_packets;
+  uint64_t n_events;        /* loop iterations of propagate_monte_carlo */
+  uint64_t n_binshift;      /* events whose distance was the nu-bin step */
+  uint64_t n_boundary;      /* zone-boundary events (incl. bin steps)    */
+  uint64_t n_scatter;       /* interaction events                        */
+  uint64_t n_redist;        /* emissivity-CDF redistributions            */
+  uint64_t n_escaped;
+  uint64_t n_absorbed;
+  uint64_t n_stopped;
+  uint64_t n_moving;        /* left moving by max_events                 */
+} sgpu_counters;
+
+/* ------------------------------------------------------------------ */
+/* context, memory, errors                                             */
+/* ------------------------------------------------------------------ */
+typedef struct sgpu_ctx sgpu_ctx;
+
+/* Create a context on HIP device `device` with its own (default-priority)
+ * stream.  Returns NULL on failure (then sgpu_last_error explains). */
+sgpu_ctx* sgpu_create(int device);
+void      sgpu_destroy(sgpu_ctx* ctx);
+/* Make the context issue its work on an existing hipStream_t (passed as void*);
+ * NULL restores the context's own stream. */
+int  sgpu_set_stream(sgpu_ctx* ctx, void* hip_stream);
+void* sgpu_get_stream(sgpu_ctx* ctx);
+int  sgpu_synchronize(sgpu_ctx* ctx);
+/* Copy the last error message of this thread into buf (NUL-terminated). */
+int  sgpu_last_error(char* buf, size_t n);
+int  sgpu_alloc(sgpu_ctx* ctx, void** ptr, size_t bytes);
+int  sgpu_free(sgpu_ctx* ctx, void* ptr);
+int  sgpu_memcpy_h2d(sgpu_ctx* ctx, void* dst, const void* src, size_t bytes);
+int  sgpu_memcpy_d2h(sgpu_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Elapsed milliseconds of the most recent kernel of the named phase, timed
+ * with HIP events on the context's stream ("opacity", "emit", "propagate",
+ * "nlte").  Returns -1 when none was timed. */
+double sgpu_last_kernel_ms(sgpu_ctx* ctx, const char* phase);
+/* Enable (1) or disable (0) the per-phase HIP event timing above. */
+int  sgpu_enable_timing(sgpu_ctx* ctx, int on);
+
+/* ------------------------------------------------------------------ */
+/* K1-K3: opacity / emissivity build                                   */
+/* ------------------------------------------------------------------ */
+/* For every zone: temperature clamp, radioactive decay of the composition,
+ * LTE state solve by the reference's Brent iteration for n_e (only where
+ * grey_opacity == 0), computeOpacity (grey, or electron scattering + fuzz
+ * expansion), emissivity CDF, L_thermal, max-extinction clamp, Planck and
+ * Rosseland means, gamma-ray coefficients and eps_imc.
+ * Returns SGPU_OK, or the largest zone_status over zones (1 or 2) when some
+ * zone's n_e solve failed (the tables are still written, as the reference
+ * continues after a warning), or a negative error. */
+int sgpu_opacity_build(sgpu_ctx* ctx,
+                       const sgpu_locate* nu_grid,
+                       const sgpu_zone_state* zones,
+                       const sgpu_atoms* atoms,
+                       const sgpu_opacity_params* params,
+                       sgpu_opacity_out* out);
+
+/* ------------------------------------------------------------------ */
+/* K5: radioactive packet emission                                     */
+/* ------------------------------------------------------------------ */
+/* Emits n_emit packets (56Ni/56Co/48Cr chains) into packets[offset ..
+ * offset+n_emit), stream ids id_base + q.  Adds the positron share to
+ * tallies->L_radio_dep and writes tallies->L_radio_emit.  *n_out receives the
+ * number emitted (0 when the grid's decay luminosity is zero). */
+int sgpu_emit_radioactive(sgpu_ctx* ctx,
+                          const sgpu_grid* grid,
+                          const sgpu_locate* nu_grid,
+                          const sgpu_zone_state* zones,
+                          const sgpu_tables* tables,
+                          const sgpu_transport_params* params,
+                          int64_t n_emit, int32_t force_rproc, uint64_t id_base,
+                          sgpu_packets* packets, int64_t offset,
+                          sgpu_tallies* tallies, int64_t* n_out);
+
+/* ------------------------------------------------------------------ */
+/* K4: propagation                                                     */
+/* ------------------------------------------------------------------ */
+/* Propagates packets [0, n) to the end of the step (or to escape / absorption
+ * in steady state), tallies e_abs, J_nu, f_rad, L_radio_dep, and counts
+ * escaping packets into the optical / gamma spectra at t_obs = t - x.D/c
+ * (then sets t = t_obs, as transport::step does).  `counters` (device
+ * pointer to an sgpu_counters, may be NULL) is accumulated. */
+int sgpu_propagate(sgpu_ctx* ctx,
+                   const sgpu_grid* grid,
+                   const sgpu_locate* nu_grid,
+                   const sgpu_tables* tables,
+                   const sgpu_transport_params* params,
+                   sgpu_packets* packets, int64_t n,
+                   sgpu_tallies* tallies,
+                   sgpu_counters* counters);
+
+/* Removes escaped and absorbed packets, keeping order of the survivors
+ * (src/transport/transport.cpp:164-194 keeps a different order; packets are
+ * independent, so only the set matters).  `scratch` is an (n) packet-sized
+ * staging buffer set (device), may alias nothing else.  *n_alive and
+ * *n_escaped are host outputs. */
+int sgpu_compact(sgpu_ctx* ctx, sgpu_packets* packets, int64_t n,
+                 sgpu_packets* scratch, int64_t* n_alive, int64_t* n_escaped);
+
+/* Zero the zone and spectrum tallies (transport::wipe_radiation; the spectra
+ * are wiped only when wipe_spectra != 0). */
+int sgpu_wipe_radiation(sgpu_ctx* ctx, int32_t n_zones, int32_t n_nu,
+                        sgpu_tallies* tallies, int32_t wipe_spectra);
+
+/* reduce_radiation normalisation (transport_mpi.cpp:415-441). */
+int sgpu_normalize_tallies(sgpu_ctx* ctx, const sgpu_grid* grid,
+                           const sgpu_locate* nu_grid,
+                           const sgpu_transport_params* params,
+                           sgpu_tallies* tallies);
+
+/* ------------------------------------------------------------------ */
+/* K8: NLTE level populations                                          */
+/* ------------------------------------------------------------------ */
+/* One system per call, FP64 partial-pivot LU in device memory: the
+ * drop-in for AtomicSpecies::solve_nlte.  rates[n*n] (row = from-level,
+ * already multiplied by the LTE population of the source level, as
+ * set_rates leaves them) and lte[n] are device pointers; the solver builds
+ * M_ii = -sum_j R_ij, M_ij = R_ji, last row = lte, rhs e_{n-1}, factors and
+ * solves; b[n] receives the departure coefficients and pop[n] (may be NULL)
+ * max(b*lte, 1e-30).  work must hold n*n + n doubles (device). */
+int sgpu_nlte_solve(sgpu_ctx* ctx, int32_t n, const double* rates,
+                    const double* lte, double* b, double* pop, double* work);
+
+/* All systems in one persistent launch.  System s has size n[s]; its rates
+ * start at rates + rate_off[s] (n[s]^2 doubles, row-major) and its lte/b/pop
+ * at vec_off[s].  precision: SGPU_NLTE_FP64 or SGPU_NLTE_FP32_REFINE.
+ * tol: refinement stops when max|dx|/max|x| <= tol (FP32 mode).
+ * iters[s] (device, may be NULL) receives the refinement iterations used, or
+ * -1 when the FP32 solve did not converge and the system was re-solved in FP64.
+ * n[], rate_off[], vec_off[] are device arrays. */
+int sgpu_nlte_batched(sgpu_ctx* ctx, int32_t nsys, const int32_t* n,
+                      const int64_t* rate_off, const int64_t* vec_off,
+                      const double* rates, const double* lte,
+                      double* b, double* pop,
+                      int32_t precision, double tol, int32_t* iters);
+
+/* Choose launch shapes for sgpu_nlte_batched by timing candidates on the
+ * given batch (project-written autotuner).  The choice is kept in the
+ * context and used by later sgpu_nlte_batched calls. */
+int sgpu_nlte_autotune(sgpu_ctx* ctx, int32_t nsys, const int32_t* n,
+                       const int64_t* rate_off, const int64_t* vec_off,
+                       const double* rates, const double* lte,
+                       double* b, double* pop, int32_t* iters,
+                       char* report, size_t report_len);
+
+/* ------------------------------------------------------------------ */
+/* homologous expansion                                                */
+/* ------------------------------------------------------------------ */
+/* rho /= e^3, grid edges and volumes scaled by e (host-side descriptor
+ * grid->t_now is NOT changed: the caller owns it).  rho, edges and vol are
+ * device arrays; e = (t+dt)/t. */
+int sgpu_homologous_step(sgpu_ctx* ctx, sgpu_grid* grid, double* rho,
+                         double* r_out, double* x_out, double* z_out,
+                         double* vol, double e);
+
+/* ------------------------------------------------------------------ */
+/* differentiable transport (continuous-absorption estimator)          */
+/* ------------------------------------------------------------------ */
+/* Forward pass: 1D grid, steady state.  kappa_abs[nz][n_nu] (FP32, cm^2/g)
+ * times rho gives the absorption coefficient; scattering scat[nz][n_nu]
+ * (cm^-1, FP64) is sampled discretely; absorption acts as a weight
+ * w = exp(-sum kappa rho dshift ds).  Escaping weight*energy is binned into
+ * F[n_nu_spec] (FP64).  rec_bin[n] / rec_w[n] (device, may be NULL) keep each
+ * packet's spectrum bin (-1 when not counted) and final weight for the
+ * adjoint replay. */
+int sgpu_adjoint_forward(sgpu_ctx* ctx, const sgpu_grid* grid,
+                         const sgpu_locate* nu_grid, const sgpu_locate* spec_grid,
+                         const float* kappa_abs, const double* rho,
+                         const double* scat, const sgpu_transport_params* params,
+                         const sgpu_packets* packets, int64_t n,
+                         double* F, int32_t* rec_bin, double* rec_w);
+
+/* Adjoint replay: given dL/dF[n_nu_spec] and the forward records, replays
+ * every packet path from its Philox counters and accumulates
+ * dL/dkappa_abs[nz][n_nu] (FP64 accumulation, device). */
+int sgpu_adjoint_backward(sgpu_ctx* ctx, const sgpu_grid* grid,
+                          const sgpu_locate* nu_grid, const sgpu_locate* spec_grid,
+                          const float* kappa_abs, const double* rho,
+                          const double* scat, const sgpu_transport_params* params,
+                          const sgpu_packets* packets, int64_t n,
+                          const double* dL_dF, const int32_t* rec_bin,
+                          const double* rec_w, double* dL_dkappa);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SEDONA_GPU_H */

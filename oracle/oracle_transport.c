@@ -1,0 +1,638 @@
+/*
+ * oracle_transport.c -- TEST INFRASTRUCTURE: FP64 CPU restatement of packet
+ * emission, propagation, interactions, escape spectrum, tally normalisation,
+ * compaction and homologous expansion.  Every packet draws from its own
+ * Philox stream, so the result does not depend on the thread count.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "oracle.h"
+
+typedef struct {
+  double x[3], D[3], t, e, nu;
+  int ind, type;
+  uint64_t id;
+  uint32_t draw;
+  double gamma, dshift, dvds;
+} pkt_t;
+
+typedef struct {
+  const sgpu_grid* g;
+  const sgpu_locate* nu;
+  const sgpu_tables* tab;
+  const sgpu_transport_params* p;
+  sgpu_tallies* ta;
+  uint64_t n_events, n_binshift, n_boundary, n_scatter, n_redist;
+} tctx_t;
+
+static inline double urand(const tctx_t* c, pkt_t* p)
+{
+  return or_uniform(c->p->seed, p->id, p->draw++);
+}
+
+static inline void atomic_add(double* a, double v)
+{
+#pragma omp atomic
+  *a += v;
+}
+
+/* grid_1D_sphere::get_zone (grid_1D_sphere.cpp:494-505),
+ * grid_2D_cyln::get_zone (grid_2D_cyln.cpp:329-354) */
+int or_get_zone(const sgpu_grid* g, const double* x)
+{
+  if (g->kind == SGPU_GRID_1D_SPHERE) {
+    double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    if (r < g->r_out.min) return -1;
+    if (r >= g->r_out.x[g->r_out.n - 1]) return -2;
+    return or_locate_within_bounds(&g->r_out, r);
+  }
+  double p = sqrt(x[0] * x[0] + x[1] * x[1]);
+  int nx = g->x_out.n, nz = g->z_out.n;
+  if (p < g->x_out.min) return -2;
+  if (p > g->x_out.x[nx - 1]) return -2;
+  if (x[2] < g->z_out.min) return -2;
+  if (x[2] > g->z_out.x[nz - 1]) return -2;
+  int i = or_locate_within_bounds(&g->x_out, p);
+  int k = or_locate_within_bounds(&g->z_out, x[2]);
+  return i * nz + k;
+}
+
+/* grid_1D_sphere::get_next_zone, grid_1D_sphere.cpp:510-590 */
+static int next_zone_1d(const sgpu_grid* g, const double* x, const double* D, int i,
+                        double r_core, double* l)
+{
+  double rsq = (x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  double xdotD = (D[0] * x[0] + D[1] * x[1] + D[2] * x[2]);
+  double r_o = g->r_out.x[i];
+  double l_out = -1 * xdotD + sqrt(xdotD * xdotD + r_o * r_o - rsq);
+  double r_in;
+  int ind_in;
+  if (i != 0) { r_in = g->r_out.x[i - 1]; ind_in = i - 1; }
+  else { r_in = g->r_out.min; ind_in = -1; }
+  if (r_core >= r_in) { r_in = r_core; ind_in = -1; }
+  double l_in;
+  if ((i == 0) && (r_in == 0)) l_in = -1;
+  else {
+    double rad = xdotD * xdotD + r_in * r_in - rsq;
+    if (rad < 0) l_in = -1;
+    else l_in = -1 * xdotD - sqrt(rad);
+  }
+  int ind;
+  double tiny_offset = 1 + 1e-6;
+  if ((l_out < l_in) || (l_in < 0)) {
+    ind = i + 1;
+    if (ind == g->n_zones) { ind = -2; *l = l_out / tiny_offset; }
+    else *l = l_out * tiny_offset;
+  } else {
+    ind = ind_in;
+    if (ind_in == -1) *l = l_in / tiny_offset;
+    else *l = l_in * tiny_offset;
+  }
+  return ind;
+}
+
+/* grid_2D_cyln::get_next_zone, grid_2D_cyln.cpp:360-465 */
+static int next_zone_2d(const sgpu_grid* g, const double* x, const double* D, int i, double* l)
+{
+  int nz = g->z_out.n, nx = g->x_out.n;
+  double p = sqrt(x[0] * x[0] + x[1] * x[1]);
+  double z = x[2];
+  double Dz = D[2];
+  int ix = i / nz, iz = i % nz;
+  double lp, lz;
+  int d_ip = 0, d_iz = 0;
+  double tiny = 1e-10;
+  double dz = or_delta(&g->z_out, iz), dx = or_delta(&g->x_out, ix);
+  if (Dz > 0) {
+    double zt = g->z_out.x[iz] + dz * tiny;
+    lz = (zt - z) / Dz;
+    d_iz = 1;
+  } else {
+    double zt = or_left(&g->z_out, iz) - dz * tiny;
+    lz = (zt - z) / Dz;
+    if (Dz == 0) lz = INFINITY;
+    d_iz = -1;
+  }
+  double pt, c, lp_out, lp_in, det;
+  double a = D[0] * D[0] + D[1] * D[1];
+  double b = 2 * (x[0] * D[0] + x[1] * D[1]);
+  if (a == 0) lp = INFINITY;
+  else {
+    pt = g->x_out.x[ix] + dx * tiny;
+    c = p * p - pt * pt;
+    det = b * b - 4 * a * c;
+    if (det < 0) lp_out = INFINITY;
+    else lp_out = (-1.0 * b + sqrt(det)) / (2 * a);
+    if (lp_out < 0) lp_out = INFINITY;
+    pt = or_left(&g->x_out, ix) - dx * tiny;
+    c = p * p - pt * pt;
+    det = b * b - 4 * a * c;
+    if (det < 0) lp_in = INFINITY;
+    else lp_in = (-1.0 * b - sqrt(det)) / (2 * a);
+    if (lp_in < 0) lp_in = INFINITY;
+    if (ix == 0) lp_in = INFINITY;
+    if (lp_in < lp_out) { lp = lp_in; d_ip = -1; }
+    else { lp = lp_out; d_ip = 1; }
+  }
+  int new_iz = iz, new_ip = ix;
+  if (lz < lp) { *l = lz; new_iz += d_iz; }
+  else { *l = lp; new_ip += d_ip; }
+  if ((new_iz < 0) || (new_iz >= nz) || (new_ip >= nx)) return -2;
+  return new_ip * nz + new_iz;
+}
+
+int or_get_next_zone(const sgpu_grid* g, const double* x, const double* D, int i,
+                     double r_core, double* l)
+{
+  if (g->kind == SGPU_GRID_1D_SPHERE) return next_zone_1d(g, x, D, i, r_core, l);
+  return next_zone_2d(g, x, D, i, l);
+}
+
+/* homologous get_velocity (grid_1D_sphere.cpp:707-716, grid_2D_cyln.cpp:546-554)
+ * and transport::do_dshift (relativity.cpp:14-43) */
+static double do_dshift(const tctx_t* c, pkt_t* p, int tolab)
+{
+  double tn = c->g->t_now;
+  double v[3] = {p->x[0] / tn, p->x[1] / tn, p->x[2] / tn};
+  double dvds = 1.0 / tn;
+  if (tolab) { v[0] *= -1; v[1] *= -1; v[2] *= -1; }
+  double beta2 = (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / OR_C / OR_C;
+  double vdd = v[0] * p->D[0] + v[1] * p->D[1] + v[2] * p->D[2];
+  double gamma = 1.0 / sqrt(1 - beta2);
+  double dshift = gamma * (1 - vdd / OR_C);
+  p->gamma = gamma; p->dshift = dshift; p->dvds = dvds;
+  return dshift;
+}
+
+/* transport::lorentz_transform, relativity.cpp:84-127 (D divided by |D|^2) */
+static void lorentz_transform(const tctx_t* c, pkt_t* p, int tolab)
+{
+  double tn = c->g->t_now;
+  double v[3] = {p->x[0] / tn, p->x[1] / tn, p->x[2] / tn};
+  double dvds = 1.0 / tn;
+  if (tolab) { v[0] *= -1; v[1] *= -1; v[2] *= -1; }
+  double beta2 = (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / OR_C / OR_C;
+  double vdd = v[0] * p->D[0] + v[1] * p->D[1] + v[2] * p->D[2];
+  double gamma = 1.0 / sqrt(1 - beta2);
+  double dshift = gamma * (1 - vdd / OR_C);
+  p->gamma = gamma; p->dshift = dshift; p->dvds = dvds;
+  p->e *= dshift;
+  p->nu *= dshift;
+  for (int k = 0; k < 3; k++)
+    p->D[k] = 1.0 / dshift * (p->D[k] - gamma * v[k] / OR_C * (1 - gamma * vdd / OR_C / (gamma + 1)));
+  double norm = p->D[0] * p->D[0] + p->D[1] * p->D[1] + p->D[2] * p->D[2];
+  p->D[0] /= norm; p->D[1] /= norm; p->D[2] /= norm;
+}
+
+/* transport::get_opacity, transport_opacity.cpp:268-299 */
+static int get_opacity(const tctx_t* c, const pkt_t* p, double dshift, double* opac, double* eps)
+{
+  double nu = p->nu * dshift;
+  int i_nu = 0;
+  int nn = c->nu->n;
+  if (p->type == SGPU_PHOTON) {
+    i_nu = or_locate_within_bounds(c->nu, nu);
+    size_t k = (size_t)p->ind * nn + i_nu;
+    double a = c->tab->abs[k];
+    double s = 0;
+    if (!c->p->omit_scattering) s = c->tab->scat[k];
+    *opac = a + s;
+    *eps = (*opac == 0) ? 0 : a / *opac;
+  } else {
+    double co = c->tab->compton_opac[p->ind] * or_klein_nishina(p->nu);
+    double po = c->tab->photoion_opac[p->ind] * pow(p->nu, -3.5);
+    *opac = co + po;
+    *eps = po / (co + po);
+  }
+  return i_nu;
+}
+
+/* transport::sample_photon_frequency, emission.cpp:27-46 */
+static void sample_photon_frequency(tctx_t* c, pkt_t* p)
+{
+  if (p->type == SGPU_PHOTON) {
+    int nn = c->nu->n;
+    int inu = or_cdf_sample(c->tab->emis_cdf + (size_t)p->ind * nn, nn, urand(c, p));
+    p->nu = or_sample(c->nu, inu, urand(c, p));
+    c->n_redist++;
+  } else {
+    p->nu = 1;
+  }
+}
+
+/* transport::isotropic_scatter, scatter.cpp:303-351 (D divided by |D|) */
+static void isotropic_scatter(tctx_t* c, pkt_t* p, int redist)
+{
+  double tn = c->g->t_now;
+  double V[3] = {p->x[0] / tn, p->x[1] / tn, p->x[2] / tn};
+  double vdotD = V[0] * p->D[0] + V[1] * p->D[1] + V[2] * p->D[2];
+  double beta2 = (V[0] * V[0] + V[1] * V[1] + V[2] * V[2]) / OR_C / OR_C;
+  double gamma = 1.0 / sqrt(1 - beta2);
+  double dshift_in = gamma * (1 - vdotD / OR_C);
+  p->e = p->e * dshift_in;
+  p->nu = p->nu * dshift_in;
+  double mu = 1 - 2.0 * urand(c, p);
+  double phi = 2.0 * OR_PI * urand(c, p);
+  double smu = sqrt(1 - mu * mu);
+  double Dn[3] = {smu * cos(phi), smu * sin(phi), mu};
+  if (redist) sample_photon_frequency(c, p);
+  for (int i = 0; i < 3; i++) V[i] = -1 * V[i];
+  double vdp = (Dn[0] * V[0] + Dn[1] * V[1] + Dn[2] * V[2]);
+  double vd_out = gamma * (1 - vdp / OR_C);
+  for (int k = 0; k < 3; k++)
+    p->D[k] = 1.0 / vd_out * (Dn[k] - gamma * V[k] / OR_C * (1 - gamma * vdp / OR_C / (gamma + 1)));
+  double norm = sqrt(p->D[0] * p->D[0] + p->D[1] * p->D[1] + p->D[2] * p->D[2]);
+  p->D[0] = p->D[0] / norm; p->D[1] = p->D[1] / norm; p->D[2] = p->D[2] / norm;
+  p->e = p->e * vd_out;
+  p->nu = p->nu * vd_out;
+}
+
+/* transport::compton_scatter, scatter.cpp:76-133 */
+static void compton_scatter(tctx_t* c, pkt_t* p)
+{
+  lorentz_transform(c, p, 0);
+  double E_ratio, Dn[3];
+  while (1) {
+    double mu = 1 - 2.0 * urand(c, p);
+    double phi = 2.0 * OR_PI * urand(c, p);
+    double smu = sqrt(1 - mu * mu);
+    Dn[0] = smu * cos(phi); Dn[1] = smu * sin(phi); Dn[2] = mu;
+    double cost = p->D[0] * Dn[0] + p->D[1] * Dn[1] + p->D[2] * Dn[2];
+    E_ratio = 1 / (1 + p->nu / OR_M_E_MEV * (1 - cost));
+    double diff_cs = 0.5 * (E_ratio * E_ratio * (1 / E_ratio + E_ratio - 1 + cost * cost));
+    if (urand(c, p) < diff_cs) break;
+  }
+  p->nu = p->nu * E_ratio;
+  if (urand(c, p) > E_ratio) {
+    atomic_add(&c->ta->L_radio_dep[p->ind], p->e);
+    p->type = SGPU_PHOTON;
+    double mu = 1 - 2.0 * urand(c, p);
+    double phi = 2.0 * OR_PI * urand(c, p);
+    double smu = sqrt(1 - mu * mu);
+    Dn[0] = smu * cos(phi); Dn[1] = smu * sin(phi); Dn[2] = mu;
+    sample_photon_frequency(c, p);
+  }
+  p->D[0] = Dn[0]; p->D[1] = Dn[1]; p->D[2] = Dn[2];
+  lorentz_transform(c, p, 1);
+}
+
+/* transport::do_scatter, scatter.cpp:12-70 */
+static int do_scatter(tctx_t* c, pkt_t* p, double eps)
+{
+  int fate = SGPU_MOVING;
+  if (p->type == SGPU_PHOTON) {
+    if (urand(c, p) > eps) isotropic_scatter(c, p, 0);
+    else {
+      double z2 = urand(c, p);
+      if ((z2 > c->tab->eps_imc[p->ind]) || (c->p->radiative_eq)) isotropic_scatter(c, p, 1);
+      else fate = SGPU_ABSORBED;
+    }
+  } else if (p->type == SGPU_GAMMARAY) {
+    if (urand(c, p) > eps) compton_scatter(c, p);
+    else {
+      atomic_add(&c->ta->L_radio_dep[p->ind], p->e);
+      p->type = SGPU_PHOTON;
+      double mu = 1 - 2.0 * urand(c, p);
+      double phi = 2.0 * OR_PI * urand(c, p);
+      double smu = sqrt(1 - mu * mu);
+      p->D[0] = smu * cos(phi); p->D[1] = smu * sin(phi); p->D[2] = mu;
+      sample_photon_frequency(c, p);
+      lorentz_transform(c, p, 1);
+    }
+  }
+  return fate;
+}
+
+/* transport::propagate_monte_carlo, transport.cpp:260-484 (no DDMC) */
+static int propagate_mc(tctx_t* c, pkt_t* p, double tstop, int max_events, int* n_ev)
+{
+  int fate = SGPU_MOVING;
+  int nn = c->nu->n;
+  while (fate == SGPU_MOVING) {
+    if (max_events > 0 && *n_ev >= max_events) break;
+    (*n_ev)++;
+    c->n_events++;
+    double d_bn = 0;
+    int new_ind = or_get_next_zone(c->g, p->x, p->D, p->ind, c->g->r_core, &d_bn);
+    double dshift = do_dshift(c, p, 0);
+    double kappa, eps;
+    int i_nu = get_opacity(c, p, dshift, &kappa, &eps);
+    double d_nu = or_delta(c->nu, i_nu) / p->nu * OR_C / p->dvds;
+    if (p->dvds == 0) d_nu = INFINITY;
+    if (d_nu < 0) d_nu = -1 * d_nu;
+    int binshift = 0;
+    if (d_nu < d_bn) { d_bn = d_nu; new_ind = p->ind; binshift = 1; }
+    double tot_lab = kappa * dshift;
+    double tau_r = -1.0 * log(1 - urand(c, p));
+    double d_sc = tau_r / tot_lab;
+    if (tot_lab == 0) d_sc = INFINITY;
+    double d_tm = (tstop - p->t) * OR_C;
+    if (c->p->steady_state) d_tm = INFINITY;
+    int event;
+    double this_d;
+    if ((d_sc < d_bn) && (d_sc < d_tm)) { event = 0; this_d = d_sc; }
+    else if (d_bn < d_tm) { event = 1; this_d = d_bn; }
+    else { event = 2; this_d = d_tm; }
+    double this_E = p->e * this_d;
+    int z = p->ind;
+    if (p->type == SGPU_PHOTON) {
+      atomic_add(&c->ta->e_abs[z], this_E * dshift * (kappa) * eps * dshift * c->tab->eps_imc[z]);
+      if (c->p->store_Jnu) atomic_add(&c->ta->J_nu[(size_t)z * nn + i_nu], this_E);
+      else atomic_add(&c->ta->J_nu[(size_t)z * (c->p->store_Jnu ? nn : 1)], this_E);
+    }
+    atomic_add(&c->ta->fx_rad[z], this_E * dshift * kappa * p->D[0] * dshift);
+    atomic_add(&c->ta->fy_rad[z], this_E * dshift * kappa * p->D[1] * dshift);
+    atomic_add(&c->ta->fz_rad[z], this_E * dshift * kappa * p->D[2] * dshift);
+    double rr = sqrt(p->x[0] * p->x[0] + p->x[1] * p->x[1] + p->x[2] * p->x[2]);
+    double xdotD = p->x[0] * p->D[0] + p->x[1] * p->D[1] + p->x[2] * p->D[2];
+    atomic_add(&c->ta->fr_rad[z], this_E * dshift * kappa * xdotD / rr * dshift);
+    p->x[0] += this_d * p->D[0];
+    p->x[1] += this_d * p->D[1];
+    p->x[2] += this_d * p->D[2];
+    p->t = p->t + this_d / OR_C;
+    if (event == 1) {
+      c->n_boundary++;
+      if (binshift) c->n_binshift++;
+      if (new_ind == -1) {
+        if (c->p->boundary_in_reflect) { p->D[0] *= -1; p->D[1] *= -1; p->D[2] *= -1; }
+        else { p->ind = new_ind; fate = SGPU_ABSORBED; }
+      } else if (new_ind == -2) {
+        if (c->p->boundary_out_reflect) { p->D[0] *= -1; p->D[1] *= -1; p->D[2] *= -1; }
+        else { p->ind = new_ind; fate = SGPU_ESCAPED; }
+      } else {
+        p->ind = new_ind;
+      }
+    } else if (event == 0) {
+      c->n_scatter++;
+      fate = do_scatter(c, p, eps);
+    } else {
+      fate = SGPU_STOPPED;
+    }
+  }
+  return fate;
+}
+
+/* spectrum_array::count, spectrum_array.cpp:215-246 */
+static void spectrum_count(sgpu_spectrum* s, double t, double w, double E, const double* D)
+{
+  double mu = D[2];
+  double phi = atan2(D[1], D[0]);
+  if (phi < 0) phi += 2. * OR_PI;
+  int t_bin = or_locate(&s->time_grid, t);
+  int l_bin = or_locate(&s->nu_grid, w);
+  int m_bin = or_locate(&s->mu_grid, mu);
+  int p_bin = or_locate(&s->phi_grid, phi);
+  if (l_bin < 0) l_bin = 0;
+  if (l_bin >= s->nu_grid.n) l_bin = 0;
+  if ((t_bin < 0) || (l_bin < 0) || (m_bin < 0) || (p_bin < 0)) return;
+  if (t_bin >= s->time_grid.n) return;
+  if (m_bin >= s->mu_grid.n) return;
+  if (p_bin >= s->phi_grid.n) return;
+  int a3 = s->phi_grid.n, a2 = s->mu_grid.n * a3, a1 = s->nu_grid.n * a2;
+  size_t ind = (size_t)t_bin * a1 + (size_t)l_bin * a2 + (size_t)m_bin * a3 + p_bin;
+  atomic_add(&s->flux[ind], E);
+  atomic_add(&s->click[ind], 1);
+}
+
+static void load_pkt(const sgpu_packets* pk, int64_t i, pkt_t* p)
+{
+  p->x[0] = pk->x[i]; p->x[1] = pk->y[i]; p->x[2] = pk->z[i];
+  p->D[0] = pk->Dx[i]; p->D[1] = pk->Dy[i]; p->D[2] = pk->Dz[i];
+  p->t = pk->t[i]; p->e = pk->e[i]; p->nu = pk->nu[i];
+  p->ind = pk->ind[i]; p->type = pk->type[i];
+  p->id = pk->id[i]; p->draw = pk->draw[i];
+}
+
+static void store_pkt(sgpu_packets* pk, int64_t i, const pkt_t* p, int fate)
+{
+  pk->x[i] = p->x[0]; pk->y[i] = p->x[1]; pk->z[i] = p->x[2];
+  pk->Dx[i] = p->D[0]; pk->Dy[i] = p->D[1]; pk->Dz[i] = p->D[2];
+  pk->t[i] = p->t; pk->e[i] = p->e; pk->nu[i] = p->nu;
+  pk->ind[i] = p->ind; pk->type[i] = p->type; pk->fate[i] = fate;
+  pk->id[i] = p->id; pk->draw[i] = p->draw;
+}
+
+/* packet loop of transport::step (transport.cpp:57-88) with
+ * transport::propagate (:202-254) */
+int or_propagate(const sgpu_grid* grid, const sgpu_locate* nu_grid,
+                 const sgpu_tables* tab, const sgpu_transport_params* prm,
+                 sgpu_packets* pk, int64_t n, sgpu_tallies* ta,
+                 sgpu_counters* cnt, int n_threads)
+{
+#ifdef _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#else
+  (void)n_threads;
+#endif
+  uint64_t tot[10] = {0};
+#pragma omp parallel
+  {
+    tctx_t c = {grid, nu_grid, tab, prm, ta, 0, 0, 0, 0, 0};
+    uint64_t esc = 0, absb = 0, stp = 0, mov = 0;
+#pragma omp for schedule(guided)
+    for (int64_t i = 0; i < n; i++) {
+      pkt_t p;
+      load_pkt(pk, i, &p);
+      int fate;
+      int n_ev = 0;
+      p.ind = or_get_zone(grid, p.x);
+      if (p.ind == -1) fate = SGPU_ABSORBED;
+      else if (p.ind == -2) fate = SGPU_ESCAPED;
+      else {
+        double tstop = prm->t_now + prm->dt;
+        fate = SGPU_MOVING;
+        while (fate == SGPU_MOVING) {
+          fate = propagate_mc(&c, &p, tstop, prm->max_events, &n_ev);
+          if (prm->max_events > 0 && n_ev >= prm->max_events) break;
+        }
+      }
+      if (fate == SGPU_ESCAPED) {
+        double t_obs = p.t - (p.x[0] * p.D[0] + p.x[1] * p.D[1] + p.x[2] * p.D[2]) / OR_C;
+        if (p.type == SGPU_PHOTON && ta->optical.flux) spectrum_count(&ta->optical, t_obs, p.nu, p.e, p.D);
+        if (p.type == SGPU_GAMMARAY && ta->gamma.flux) spectrum_count(&ta->gamma, t_obs, p.nu, p.e, p.D);
+        p.t = t_obs;
+        esc++;
+      } else if (fate == SGPU_ABSORBED) absb++;
+      else if (fate == SGPU_STOPPED) stp++;
+      else mov++;
+      store_pkt(pk, i, &p, fate);
+    }
+#pragma omp critical
+    {
+      tot[0] += c.n_events; tot[1] += c.n_binshift; tot[2] += c.n_boundary;
+      tot[3] += c.n_scatter; tot[4] += c.n_redist; tot[5] += esc; tot[6] += absb;
+      tot[7] += stp; tot[8] += mov;
+    }
+  }
+  if (cnt) {
+    cnt->n_packets += (uint64_t)n;
+    cnt->n_events += tot[0]; cnt->n_binshift += tot[1]; cnt->n_boundary += tot[2];
+    cnt->n_scatter += tot[3]; cnt->n_redist += tot[4]; cnt->n_escaped += tot[5];
+    cnt->n_absorbed += tot[6]; cnt->n_stopped += tot[7]; cnt->n_moving += tot[8];
+  }
+  return 0;
+}
+
+/* sample_in_zone, grid_1D_sphere.cpp:681-700 and grid_2D_cyln.cpp:528-541 */
+static void sample_in_zone(const sgpu_grid* g, int i, const double* ran, double* r)
+{
+  if (g->kind == SGPU_GRID_1D_SPHERE) {
+    double r_0 = (i == 0) ? g->r_out.min : g->r_out.x[i - 1];
+    double ro = g->r_out.x[i];
+    double r_samp = pow(r_0 * r_0 * r_0 + ran[0] * (ro * ro * ro - r_0 * r_0 * r_0), 1.0 / 3.0);
+    double mu = 1 - 2.0 * ran[1];
+    double phi = 2.0 * OR_PI * ran[2];
+    double sin_theta = sqrt(1 - mu * mu);
+    r[0] = r_samp * sin_theta * cos(phi);
+    r[1] = r_samp * sin_theta * sin(phi);
+    r[2] = r_samp * mu;
+  } else {
+    int nz = g->z_out.n;
+    int ix = i / nz, iz = i % nz;
+    double phi = 2 * OR_PI * ran[1];
+    double p_in = or_left(&g->x_out, ix);
+    double p_out = g->x_out.x[ix];
+    double p_samp = sqrt(p_in * p_in + ran[0] * (p_out * p_out - p_in * p_in));
+    r[0] = p_samp * cos(phi);
+    r[1] = p_samp * sin(phi);
+    r[2] = or_left(&g->z_out, iz) + ran[2] * or_delta(&g->z_out, iz);
+  }
+}
+
+/* transport::emit_radioactive (emission.cpp:165-243) with
+ * create_isotropic_particle (:54-105).  Packet q draws, in order: zone,
+ * time, type, 3 position, 2 direction, [2 frequency for photons]. */
+int or_emit_radioactive(const sgpu_grid* grid, const sgpu_locate* nu_grid,
+                        const sgpu_zone_state* zs, const sgpu_tables* tab,
+                        const sgpu_transport_params* prm, int64_t n_emit,
+                        int force_rproc, uint64_t id_base,
+                        sgpu_packets* pk, int64_t offset, sgpu_tallies* ta,
+                        int64_t* n_out)
+{
+  int nz = grid->n_zones;
+  double* cdf = (double*)malloc(sizeof(double) * nz);
+  double* gfr = (double*)malloc(sizeof(double) * nz);
+  double L_tot = 0;
+  *n_out = 0;
+  for (int i = 0; i < nz; i++) {
+    double vol = grid->vol[i];
+    double gfrac;
+    double L = or_decay(zs->n_elems, zs->elem_Z, zs->elem_A, zs->X + (size_t)i * zs->n_elems,
+                        prm->t_now, &gfrac, force_rproc);
+    L = zs->rho[i] * L * vol;
+    ta->L_radio_emit[i] = L;
+    gfr[i] = gfrac;
+    L_tot += L;
+    cdf[i] = i == 0 ? L : cdf[i - 1] + L;
+  }
+  or_cdf_normalize(cdf, nz);
+  if (L_tot == 0 || n_emit <= 0) { free(cdf); free(gfr); return 0; }
+  double E_p = L_tot * prm->dt / (1.0 * n_emit);
+  tctx_t c = {grid, nu_grid, tab, prm, ta, 0, 0, 0, 0, 0};
+  for (int64_t q = 0; q < n_emit; q++) {
+    pkt_t p;
+    memset(&p, 0, sizeof(p));
+    p.id = id_base + (uint64_t)q;
+    p.draw = 0;
+    int i = or_cdf_sample(cdf, nz, urand(&c, &p));
+    double t = prm->t_now + prm->dt * urand(&c, &p);
+    int type;
+    if (urand(&c, &p) < gfr[i]) type = SGPU_GAMMARAY;
+    else { ta->L_radio_dep[i] += E_p; type = SGPU_PHOTON; }
+    p.ind = i;
+    p.type = type;
+    double ran[3];
+    ran[0] = urand(&c, &p); ran[1] = urand(&c, &p); ran[2] = urand(&c, &p);
+    sample_in_zone(grid, i, ran, p.x);
+    double mu = 1 - 2.0 * urand(&c, &p);
+    double phi = 2.0 * OR_PI * urand(&c, &p);
+    double smu = sqrt(1 - mu * mu);
+    p.D[0] = smu * cos(phi); p.D[1] = smu * sin(phi); p.D[2] = mu;
+    sample_photon_frequency(&c, &p);
+    p.e = E_p;
+    lorentz_transform(&c, &p, 1);
+    p.t = t;
+    store_pkt(pk, offset + q, &p, SGPU_MOVING);
+  }
+  *n_out = n_emit;
+  free(cdf); free(gfr);
+  return 0;
+}
+
+/* transport::reduce_radiation normalisation, transport_mpi.cpp:415-441 */
+int or_normalize_tallies(const sgpu_grid* grid, const sgpu_locate* nu,
+                         const sgpu_transport_params* prm, sgpu_tallies* ta)
+{
+  int nn = nu->n;
+  double dt = prm->dt;
+  for (int i = 0; i < grid->n_zones; i++) {
+    double vol = grid->vol[i];
+    ta->e_abs[i] /= vol * dt;
+    ta->L_radio_dep[i] /= vol * dt;
+    ta->L_radio_emit[i] /= vol;
+    ta->fr_rad[i] /= vol * OR_C * dt;
+    if (nn == 1 || !prm->store_Jnu) {
+      ta->e_rad[i] = ta->J_nu[(size_t)i * (prm->store_Jnu ? nn : 1)] / (vol * dt * OR_C);
+    } else {
+      double esum = 0;
+      for (int j = 0; j < nn; j++) {
+        double* J = &ta->J_nu[(size_t)i * nn + j];
+        *J /= vol * dt * 4 * OR_PI * or_delta(nu, j);
+        esum += *J * or_delta(nu, j) * 4 * OR_PI / OR_C;
+      }
+      ta->e_rad[i] = esum;
+    }
+  }
+  return 0;
+}
+
+/* transport::clean_up_particle_vector, transport.cpp:164-194 (order-keeping) */
+int64_t or_compact(sgpu_packets* pk, int64_t n, int64_t* n_escaped)
+{
+  int64_t k = 0, esc = 0;
+  for (int64_t i = 0; i < n; i++) {
+    int f = pk->fate[i];
+    if (f == SGPU_ESCAPED) esc++;
+    if (f == SGPU_ESCAPED || f == SGPU_ABSORBED) continue;
+    if (k != i) {
+      pk->x[k] = pk->x[i]; pk->y[k] = pk->y[i]; pk->z[k] = pk->z[i];
+      pk->Dx[k] = pk->Dx[i]; pk->Dy[k] = pk->Dy[i]; pk->Dz[k] = pk->Dz[i];
+      pk->t[k] = pk->t[i]; pk->e[k] = pk->e[i]; pk->nu[k] = pk->nu[i];
+      pk->ind[k] = pk->ind[i]; pk->type[k] = pk->type[i]; pk->fate[k] = pk->fate[i];
+      pk->draw[k] = pk->draw[i]; pk->id[k] = pk->id[i];
+    }
+    k++;
+  }
+  *n_escaped = esc;
+  return k;
+}
+
+/* hydro_homologous::step (hydro_homologous.cpp:27-61) with
+ * grid_1D_sphere::expand (grid_1D_sphere.cpp:413-426) and
+ * grid_2D_cyln::expand (grid_2D_cyln.cpp:315-324); the caller advances t_now. */
+void or_homologous_step(sgpu_grid* g, double* rho, double* r_out, double* x_out,
+                        double* z_out, double* vol, double e)
+{
+  for (int i = 0; i < g->n_zones; i++) rho[i] = rho[i] / e / e / e;
+  if (g->kind == SGPU_GRID_1D_SPHERE) {
+    for (int i = 0; i < g->r_out.n; i++) r_out[i] *= e;
+    if (g->r_out.type == SGPU_LOC_LIN) g->r_out.del *= e;
+    g->r_out.min *= e;
+    for (int i = 0; i < g->n_zones; i++) {
+      double r0 = (i == 0) ? g->r_out.min : r_out[i - 1];
+      vol[i] = 4.0 * OR_PI / 3.0 * (r_out[i] * r_out[i] * r_out[i] - r0 * r0 * r0);
+    }
+  } else {
+    for (int i = 0; i < g->x_out.n; i++) x_out[i] *= e;
+    for (int i = 0; i < g->z_out.n; i++) z_out[i] *= e;
+    if (g->x_out.type == SGPU_LOC_LIN) g->x_out.del *= e;
+    if (g->z_out.type == SGPU_LOC_LIN) g->z_out.del *= e;
+    g->x_out.min *= e;
+    g->z_out.min *= e;
+    for (int i = 0; i < g->n_zones; i++) vol[i] = vol[i] * e * e * e;
+  }
+}

@@ -1,0 +1,94 @@
+"""pubsed_amd -- MI355X-native implicit Monte Carlo transport hot path of
+Sedona (packet propagation, opacity/emissivity build, NLTE solves).
+
+The compute path is libsedona_gpu.so (hand-written HIP for gfx950, C-ABI in
+include/sedona_gpu.h).  This package is the host-side binding: ctypes
+descriptors (abi), seeded workloads (workloads), device plumbing (runtime)
+and the transport / GasState driver (transport).  There is no CPU fallback:
+if the library is missing, loading fails loudly.
+"""
+import ctypes
+import os
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsedona_gpu.so")
+
+_lib = None
+
+
+class SgpuError(RuntimeError):
+    pass
+
+
+def _declare(lib):
+    V, I, I32, I64, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    P = ctypes.POINTER
+    sig = {
+        "sgpu_create": (V, [I]),
+        "sgpu_destroy": (None, [V]),
+        "sgpu_set_stream": (I, [V, V]),
+        "sgpu_get_stream": (V, [V]),
+        "sgpu_synchronize": (I, [V]),
+        "sgpu_last_error": (I, [ctypes.c_char_p, ctypes.c_size_t]),
+        "sgpu_alloc": (I, [V, P(V), ctypes.c_size_t]),
+        "sgpu_free": (I, [V, V]),
+        "sgpu_memcpy_h2d": (I, [V, V, V, ctypes.c_size_t]),
+        "sgpu_memcpy_d2h": (I, [V, V, V, ctypes.c_size_t]),
+        "sgpu_last_kernel_ms": (D, [V, ctypes.c_char_p]),
+        "sgpu_enable_timing": (I, [V, I]),
+        "sgpu_opacity_build": (I, [V, P(abi.Locate), P(abi.ZoneState), P(abi.Atoms),
+                                   P(abi.OpacityParams), P(abi.OpacityOut)]),
+        "sgpu_emit_radioactive": (I, [V, P(abi.Grid), P(abi.Locate), P(abi.ZoneState),
+                                      P(abi.Tables), P(abi.TransportParams), I64, I32,
+                                      ctypes.c_uint64, P(abi.Packets), I64, P(abi.Tallies),
+                                      P(I64)]),
+        "sgpu_propagate": (I, [V, P(abi.Grid), P(abi.Locate), P(abi.Tables),
+                               P(abi.TransportParams), P(abi.Packets), I64, P(abi.Tallies), V]),
+        "sgpu_compact": (I, [V, P(abi.Packets), I64, P(abi.Packets), P(I64), P(I64)]),
+        "sgpu_wipe_radiation": (I, [V, I32, I32, P(abi.Tallies), I32]),
+        "sgpu_normalize_tallies": (I, [V, P(abi.Grid), P(abi.Locate), P(abi.TransportParams),
+                                       P(abi.Tallies)]),
+        "sgpu_nlte_solve": (I, [V, I32, V, V, V, V, V]),
+        "sgpu_nlte_batched": (I, [V, I32, V, V, V, V, V, V, V, I32, D, V]),
+        "sgpu_nlte_autotune": (I, [V, I32, V, V, V, V, V, V, V, V, ctypes.c_char_p,
+                                   ctypes.c_size_t]),
+        "sgpu_homologous_step": (I, [V, P(abi.Grid), V, V, V, V, V, D]),
+        "sgpu_adjoint_forward": (I, [V, P(abi.Grid), P(abi.Locate), P(abi.Locate), V, V, V,
+                                     P(abi.TransportParams), P(abi.Packets), I64, V, V, V]),
+        "sgpu_adjoint_backward": (I, [V, P(abi.Grid), P(abi.Locate), P(abi.Locate), V, V, V,
+                                      P(abi.TransportParams), P(abi.Packets), I64, V, V, V, V]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+def load_library(path=None):
+    """Load libsedona_gpu.so (built by pubsed_amd.build / __graft_entry__.build)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise SgpuError("libsedona_gpu.so not built (%s): run python -m pubsed_amd.build" % p)
+    lib = _declare(ctypes.CDLL(p))
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(1024)
+    load_library().sgpu_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc, what=""):
+    """Raise SgpuError for negative status codes; return non-negative ones."""
+    if rc < 0:
+        raise SgpuError("%s failed (%d): %s" % (what, rc, last_error()))
+    return rc

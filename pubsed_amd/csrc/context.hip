@@ -1,0 +1,166 @@
+// context.hip -- sgpu_ctx lifetime, device memory helpers, error text and
+// per-phase HIP event timing.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include "sgpu_internal.h"
+
+namespace {
+thread_local char g_err[1024] = "";
+}
+
+namespace sgpu {
+
+void set_error(const char* fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line)
+{
+  set_error("HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+  return SGPU_ERR_HIP;
+}
+
+void* scratch(sgpu_ctx* ctx, const char* name, size_t bytes, int* status)
+{
+  sgpu_scratch& s = ctx->scratch[name];
+  if (s.bytes >= bytes && s.ptr) { *status = SGPU_OK; return s.ptr; }
+  if (s.ptr) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) { *status = hip_fail(e, "sync before scratch regrow", __FILE__, __LINE__); return nullptr; }
+    (void)hipFree(s.ptr);
+    s.ptr = nullptr; s.bytes = 0;
+  }
+  size_t b = bytes < 256 ? 256 : bytes;
+  hipError_t e = hipMalloc(&s.ptr, b);
+  if (e != hipSuccess) { *status = hip_fail(e, name, __FILE__, __LINE__); s.ptr = nullptr; return nullptr; }
+  s.bytes = b;
+  *status = SGPU_OK;
+  return s.ptr;
+}
+
+void timer_begin(sgpu_ctx* ctx, const char* phase)
+{
+  if (!ctx->timing) return;
+  sgpu_phase_timer& t = ctx->timers[phase];
+  if (!t.start) { (void)hipEventCreate(&t.start); (void)hipEventCreate(&t.stop); }
+  (void)hipEventRecord(t.start, ctx->stream);
+}
+
+void timer_end(sgpu_ctx* ctx, const char* phase)
+{
+  if (!ctx->timing) return;
+  sgpu_phase_timer& t = ctx->timers[phase];
+  (void)hipEventRecord(t.stop, ctx->stream);
+  t.pending = true;
+}
+
+}  // namespace sgpu
+
+extern "C" {
+
+sgpu_ctx* sgpu_create(int device)
+{
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) { sgpu::hip_fail(e, "hipSetDevice", __FILE__, __LINE__); return nullptr; }
+  sgpu_ctx* ctx = new sgpu_ctx();
+  ctx->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
+  e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { sgpu::hip_fail(e, "hipStreamCreate", __FILE__, __LINE__); delete ctx; return nullptr; }
+  ctx->stream = ctx->own_stream;
+  return ctx;
+}
+
+void sgpu_destroy(sgpu_ctx* ctx)
+{
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->scratch) if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+  for (auto& kv : ctx->timers) {
+    if (kv.second.start) (void)hipEventDestroy(kv.second.start);
+    if (kv.second.stop) (void)hipEventDestroy(kv.second.stop);
+  }
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+  delete ctx;
+}
+
+int sgpu_set_stream(sgpu_ctx* ctx, void* s)
+{
+  SGPU_REQUIRE(ctx, "null context");
+  ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+  return SGPU_OK;
+}
+
+void* sgpu_get_stream(sgpu_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int sgpu_synchronize(sgpu_ctx* ctx)
+{
+  SGPU_REQUIRE(ctx, "null context");
+  SGPU_HIP(hipStreamSynchronize(ctx->stream));
+  return SGPU_OK;
+}
+
+int sgpu_last_error(char* buf, size_t n)
+{
+  if (!buf || n == 0) return SGPU_ERR_ARG;
+  strncpy(buf, g_err, n - 1);
+  buf[n - 1] = 0;
+  return SGPU_OK;
+}
+
+int sgpu_alloc(sgpu_ctx* ctx, void** ptr, size_t bytes)
+{
+  SGPU_REQUIRE(ctx && ptr, "null argument");
+  SGPU_HIP(hipMalloc(ptr, bytes < 1 ? 1 : bytes));
+  return SGPU_OK;
+}
+
+int sgpu_free(sgpu_ctx* ctx, void* ptr)
+{
+  SGPU_REQUIRE(ctx, "null context");
+  if (ptr) SGPU_HIP(hipFree(ptr));
+  return SGPU_OK;
+}
+
+int sgpu_memcpy_h2d(sgpu_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+  SGPU_REQUIRE(ctx, "null context");
+  SGPU_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  SGPU_HIP(hipStreamSynchronize(ctx->stream));
+  return SGPU_OK;
+}
+
+int sgpu_memcpy_d2h(sgpu_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+  SGPU_REQUIRE(ctx, "null context");
+  SGPU_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  SGPU_HIP(hipStreamSynchronize(ctx->stream));
+  return SGPU_OK;
+}
+
+int sgpu_enable_timing(sgpu_ctx* ctx, int on)
+{
+  SGPU_REQUIRE(ctx, "null context");
+  ctx->timing = on;
+  return SGPU_OK;
+}
+
+double sgpu_last_kernel_ms(sgpu_ctx* ctx, const char* phase)
+{
+  if (!ctx || !phase) return -1;
+  auto it = ctx->timers.find(phase);
+  if (it == ctx->timers.end() || !it->second.pending) return -1;
+  if (hipEventSynchronize(it->second.stop) != hipSuccess) return -1;
+  float ms = -1;
+  if (hipEventElapsedTime(&ms, it->second.start, it->second.stop) != hipSuccess) return -1;
+  return ms;
+}
+
+}  // extern "C"

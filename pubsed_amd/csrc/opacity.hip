@@ -1,0 +1,516 @@
+// opacity.hip -- K1-K3: the per-zone opacity / emissivity build.
+//
+//   K1 state_kernel     one wave64 per zone: temperature clamp, composition
+//                       decay, mass fractions, LTE Saha-Boltzmann state with
+//                       the reference's Brent iteration for n_e; levels are
+//                       spread over the lanes, partition functions and the
+//                       net ion fraction are wave reductions.
+//   K2 bins_kernel      one thread per (zone, frequency bin): electron
+//                       scattering, fuzz-line expansion opacity summed over
+//                       the bin's lines (lines sorted by bin, CSR index),
+//                       Planck emissivity, max-extinction clamp; writes abs,
+//                       scat and emis*dnu.
+//   K3 finalize_kernel  one block per zone: emissivity CDF (block scan),
+//                       normalisation, L_thermal, Planck / Rosseland means,
+//                       gamma-ray coefficients, eps_imc.
+//
+// Reference: transport::set_opacity (src/transport/transport_opacity.cpp:21-260),
+// GasState::solve_state / ne_brent_method (src/opacity/GasState.cpp:208-337),
+// AtomicSpecies::solve_lte (src/opacity/AtomicSpecies_state.cpp:56-123),
+// GasState::computeOpacity (src/opacity/GasState_opacities.cpp:11-136),
+// fuzz_expansion_opacity (:464-494) and fuzzline_expansion_opacity
+// (src/opacity/AtomicSpecies_opacities.cpp:292-324), get_planck_mean /
+// get_rosseland_mean (GasState_opacities.cpp:564-666).
+#include "sgpu_internal.h"
+#include "sgpu_physics.h"
+
+namespace sgpu {
+
+constexpr int MAX_ELEMS = 64;        // elements per zone
+constexpr int MAX_LEV = 2048;        // levels over all atoms (LDS per zone)
+constexpr int MAX_ION = 256;         // ions over all atoms
+constexpr int FIN_THREADS = 256;
+
+struct StateArgs {
+  sgpu_zone_state zs;
+  sgpu_atoms at;
+  sgpu_opacity_params p;
+  int n_lev, n_ion;
+  double* temp_used;   // [nz]
+  double* nion;        // [nz][n_ion]  n_dens * ion_frac
+  double* upart;       // [nz][n_ion]  partition function
+  sgpu_opacity_out out;
+};
+
+struct ZoneLds {
+  double X[MAX_ELEMS];
+  double mf[MAX_ELEMS];
+  double bz[MAX_LEV];     // exp(-E/kT) per level
+  double U[MAX_ION];      // partition function per ion
+  double frac[MAX_ION];   // ion fractions of the current evaluation
+  double mu_I;
+};
+
+// Saha chain of one atom (AtomicSpecies_state.cpp:69-97) into L.frac,
+// evaluated by lane 0 (sequential over the atom's ions).
+__device__ inline void saha_chain(ZoneLds& L, const sgpu_atoms& at, int a, double ne, double T)
+{
+  int i0 = at.ion_off[a], ni = at.ion_off[a + 1] - i0;
+  double lt = H_PLANCK * H_PLANCK / (2.0 * PI * M_ELECTRON * K_BOLTZ * T);
+  double fac = 2 / ne / pow(lt, 1.5);
+  double* f = L.frac + i0;
+  const double* U = L.U + i0;
+  f[0] = 1.;
+  for (int i = 1; i < ni; ++i) {
+    double chi = at.ion_chi[i0 + i - 1];
+    double saha = exp(-1.0 * chi / K_EV / T);
+    saha = saha * (U[i] / U[i - 1]) * fac;
+    f[i] = saha * f[i - 1];
+    if (ne < 1e-50) f[i] = 0;
+    double norm = 0;
+    for (int j = 0; j <= i; j++) norm += f[j];
+    for (int j = 0; j <= i; j++) f[j] = f[j] / norm;
+  }
+}
+
+// GasState::charge_conservation (GasState.cpp:242-263) for one zone,
+// executed by the whole wave (one wave per block).
+__device__ double charge_conservation(ZoneLds& L, const StateArgs& A, double ne, double dens,
+                                      double T, int lane)
+{
+  const sgpu_atoms& at = A.at;
+  double f = 0;
+  for (int a = 0; a < at.n_atoms; ++a) {
+    if (lane == 0) saha_chain(L, at, a, ne, T);
+    __syncthreads();
+    int l0 = at.lev_off[a], l1 = at.lev_off[a + 1];
+    int i0 = at.ion_off[a];
+    double acc = 0;
+    for (int l = l0 + lane; l < l1; l += WAVE) {
+      int ion = at.lev_ion[l];
+      double fr = L.frac[i0 + ion];
+      double v = fr * (double)at.lev_g[l] * L.bz[l] / L.U[i0 + ion];
+      if (v < 1e-30) v = 1e-30;
+      acc += v * ion;
+    }
+    double x = wave_sum(acc);
+    f += dens * L.mf[a] / (A.zs.elem_A[a] * M_PROTON) * x;
+    __syncthreads();
+  }
+  return f - ne;
+}
+
+__global__ void __launch_bounds__(WAVE) state_kernel(StateArgs A)
+{
+  __shared__ ZoneLds L;
+  const int z = blockIdx.x;
+  const int lane = threadIdx.x;
+  const sgpu_zone_state& zs = A.zs;
+  const sgpu_atoms& at = A.at;
+  const int ne_ = zs.n_elems;
+  double dens = zs.rho[z];
+  double T = zs.T_gas[z];
+  if (T < A.p.temp_min) T = A.p.temp_min;
+  if (T > A.p.temp_max) T = A.p.temp_max;
+  if (A.p.first_step) T = zs.T_gas[z];   // transport_opacity.cpp:111-113
+  if (lane == 0) {
+    for (int j = 0; j < ne_; j++) L.X[j] = zs.X[(size_t)z * ne_ + j];
+    if (!A.p.omit_composition_decay) decay_composition(ne_, zs.elem_Z, zs.elem_A, L.X, A.p.t_now);
+    // GasState::set_mass_fractions, GasState.cpp:137-156
+    double norm = 0.0;
+    for (int j = 0; j < ne_; ++j) { L.mf[j] = L.X[j]; norm += L.X[j]; }
+    for (int j = 0; j < ne_; ++j) L.mf[j] /= norm;
+    double inv = 0.;
+    for (int j = 0; j < ne_; ++j) inv += L.mf[j] / zs.elem_A[j];
+    L.mu_I = 1. / inv;
+    A.temp_used[z] = T;
+  }
+  __syncthreads();
+  double grey = zs.grey_opacity ? zs.grey_opacity[z] : 0.0;
+  if (grey != 0 || at.n_atoms == 0) {
+    for (int k = lane; k < A.n_ion; k += WAVE) {
+      A.nion[(size_t)z * A.n_ion + k] = 0;
+      A.upart[(size_t)z * A.n_ion + k] = 1;
+      if (A.out.ion_frac) A.out.ion_frac[(size_t)z * A.n_ion + k] = 0;
+      if (A.out.ion_part) A.out.ion_part[(size_t)z * A.n_ion + k] = 0;
+    }
+    if (A.out.lev_n)
+      for (int k = lane; k < A.n_lev; k += WAVE) A.out.lev_n[(size_t)z * A.n_lev + k] = 0;
+    if (lane == 0) {
+      A.out.n_elec[z] = 0;
+      if (A.out.zone_status) A.out.zone_status[z] = 0;
+    }
+    return;
+  }
+  // Boltzmann factors and partition functions depend on T only, so they are
+  // computed once for all Brent evaluations (AtomicSpecies_state.cpp:59-67)
+  for (int l = lane; l < A.n_lev; l += WAVE) L.bz[l] = exp(-at.lev_E[l] / K_EV / T);
+  __syncthreads();
+  for (int a = 0; a < at.n_atoms; ++a) {
+    int l0 = at.lev_off[a], l1 = at.lev_off[a + 1];
+    int i0 = at.ion_off[a], ni = at.ion_off[a + 1] - i0;
+    for (int s = 0; s < ni; ++s) {
+      double acc = 0;
+      for (int l = l0 + lane; l < l1; l += WAVE)
+        if (at.lev_ion[l] == s) acc += (double)at.lev_g[l] * L.bz[l];
+      double u = wave_sum(acc);
+      if (lane == 0) L.U[i0 + s] = u;
+    }
+  }
+  __syncthreads();
+
+  // GasState::ne_brent_method (GasState.cpp:271-337), wave-uniform
+  const double x1 = 1e-10 * dens / (L.mu_I * M_PROTON);
+  const double x2 = 100 * dens / (L.mu_I * M_PROTON);
+  const double tol = 1e-3, EPS = 3.0e-8;
+  int err = 0;
+  double a = x1, b = x2, c = x2, d = 0, e = 0, min1, min2;
+  double fa = charge_conservation(L, A, a, dens, T, lane);
+  double fb = charge_conservation(L, A, b, dens, T, lane);
+  double fc, pp, q, r, s, tol1, xm;
+  double root = 0.0;
+  bool done = false;
+  if ((fa > 0.0 && fb > 0.0) || (fa < 0.0 && fb < 0.0)) err = 1;
+  fc = fb;
+  for (int iter = 1; iter <= 100; iter++) {
+    if ((fb > 0.0 && fc > 0.0) || (fb < 0.0 && fc < 0.0)) { c = a; fc = fa; e = d = b - a; }
+    if (fabs(fc) < fabs(fb)) { a = b; b = c; c = a; fa = fb; fb = fc; fc = fa; }
+    tol1 = 2.0 * EPS * fabs(b) + 0.5 * tol;
+    xm = 0.5 * (c - b);
+    if (fabs(xm) <= tol1 || fb == 0.0) { root = b; done = true; break; }
+    if (fabs(e) >= tol1 && fabs(fa) > fabs(fb)) {
+      s = fb / fa;
+      if (a == c) { pp = 2.0 * xm * s; q = 1.0 - s; }
+      else {
+        q = fa / fc; r = fb / fc;
+        pp = s * (2.0 * xm * q * (q - r) - (b - a) * (r - 1.0));
+        q = (q - 1.0) * (r - 1.0) * (s - 1.0);
+      }
+      if (pp > 0.0) q = -q;
+      pp = fabs(pp);
+      min1 = 3.0 * xm * q - fabs(tol1 * q);
+      min2 = fabs(e * q);
+      if (2.0 * pp < (min1 < min2 ? min1 : min2)) { e = d; d = pp / q; }
+      else { d = xm; e = d; }
+    } else { d = xm; e = d; }
+    a = b; fa = fb;
+    if (fabs(d) > tol1) b += d;
+    else b += (xm >= 0.0 ? fabs(tol1) : -fabs(tol1));
+    fb = charge_conservation(L, A, b, dens, T, lane);
+  }
+  if (!done) { err = 2; root = 0.0; }
+
+  // outputs: the atoms keep the state of the LAST evaluation (L.frac)
+  for (int at_i = 0; at_i < at.n_atoms; ++at_i) {
+    double n_dens = dens * L.mf[at_i] / (zs.elem_A[at_i] * M_PROTON);
+    for (int k = at.ion_off[at_i] + lane; k < at.ion_off[at_i + 1]; k += WAVE) {
+      A.nion[(size_t)z * A.n_ion + k] = n_dens * L.frac[k];
+      A.upart[(size_t)z * A.n_ion + k] = L.U[k];
+      if (A.out.ion_frac) A.out.ion_frac[(size_t)z * A.n_ion + k] = L.frac[k];
+      if (A.out.ion_part) A.out.ion_part[(size_t)z * A.n_ion + k] = L.U[k];
+    }
+  }
+  if (A.out.lev_n) {
+    for (int at_i = 0; at_i < at.n_atoms; ++at_i) {
+      int i0 = at.ion_off[at_i];
+      for (int l = at.lev_off[at_i] + lane; l < at.lev_off[at_i + 1]; l += WAVE) {
+        int ion = at.lev_ion[l];
+        double v = L.frac[i0 + ion] * (double)at.lev_g[l] * L.bz[l] / L.U[i0 + ion];
+        A.out.lev_n[(size_t)z * A.n_lev + l] = v < 1e-30 ? 1e-30 : v;
+      }
+    }
+  }
+  if (lane == 0) {
+    A.out.n_elec[z] = root;
+    if (A.out.zone_status) A.out.zone_status[z] = err;
+  }
+}
+
+struct BinArgs {
+  sgpu_locate nu;
+  sgpu_zone_state zs;
+  sgpu_atoms at;
+  sgpu_opacity_params p;
+  int n_ion;
+  const double* temp_used;
+  const double* nion;
+  const double* upart;
+  double* abs;
+  double* scat;
+  double* ednu;    // emissivity * dnu, scanned in place by K3
+  const double* n_elec;
+};
+
+__global__ void __launch_bounds__(256) bins_kernel(BinArgs A)
+{
+  const int z = blockIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nn = A.nu.n;
+  if (j >= nn) return;
+  const double T = A.temp_used[z];
+  const double dens = A.zs.rho[z];
+  const double grey = A.zs.grey_opacity ? A.zs.grey_opacity[z] : 0.0;
+  const double eps = A.p.epsilon;
+  const double nuc = loc_center(A.nu, j);
+  const double dnu = loc_delta(A.nu, j);
+  double ab = 0, sc = 0, em = 0;
+  if (grey != 0) {
+    double gopac = dens * grey;
+    ab = gopac * eps;
+    sc = gopac * (1 - eps);
+    double ezeta = exp(1.0 * H_PLANCK * nuc / K_BOLTZ / T);
+    double bb = 2.0 * nuc * nuc * nuc * H_PLANCK / C_LIGHT / C_LIGHT / (ezeta - 1);
+    em += bb * ab;
+  } else {
+    if (A.p.use_electron_scattering) {
+      double es = THOMSON * A.n_elec[z];
+      sc += es;
+      ab += 1e-20 * eps * es;
+    }
+    if (A.p.use_fuzz_expansion) {
+      const double time = A.p.t_now;
+      double opac = 0, aopac = 0;
+      for (int a = 0; a < A.at.n_atoms; ++a) {
+        const double this_eps = (A.at.atom_zero_eps && A.at.atom_zero_eps[a]) ? 0 : eps;
+        const int64_t* off = A.at.line_bin_off + (size_t)a * (nn + 1);
+        const int64_t l0 = off[j], l1 = off[j + 1];
+        const int i0 = A.at.ion_off[a];
+        const double* nion = A.nion + (size_t)z * A.n_ion + i0;
+        const double* U = A.upart + (size_t)z * A.n_ion + i0;
+        double atom_opac = 0;
+        for (int64_t l = l0; l < l1; ++l) {
+          const double lnu = A.at.line_nu[l];
+          const int ion = A.at.line_ion[l];
+          double nl = nion[ion] * exp(-1.0 * A.at.line_El[l] / K_EV / T) / U[ion];
+          double lam = C_LIGHT / lnu;
+          double stim_cor = (1 - exp(-H_PLANCK * lnu / K_BOLTZ / T));
+          double tau = SIGMA_TOT * lam * nl * A.at.line_gf[l] * stim_cor * time;
+          atom_opac += (1 - exp(-tau));
+        }
+        atom_opac = atom_opac * nuc / dnu / C_LIGHT / time;
+        opac += atom_opac;
+        aopac += atom_opac * this_eps;
+      }
+      ab += aopac;
+      sc += opac - aopac;
+      double ezeta = exp(1.0 * H_PLANCK * nuc / K_BOLTZ / T);
+      double bb = 2.0 * nuc * nuc * nuc * H_PLANCK / C_LIGHT / C_LIGHT / (ezeta - 1);
+      em += bb * aopac;
+    }
+  }
+  double ed = em * dnu;
+  if (nn > 1) {
+    const double max_ext = A.p.maximum_opacity * dens;
+    if (sc > max_ext) sc = max_ext;
+    if (ab > max_ext) ab = max_ext;
+  }
+  const size_t k = (size_t)z * nn + j;
+  A.abs[k] = ab;
+  A.scat[k] = sc;
+  A.ednu[k] = ed;
+}
+
+struct FinArgs {
+  sgpu_locate nu;
+  sgpu_zone_state zs;
+  sgpu_opacity_params p;
+  const double* temp_used;
+  sgpu_opacity_out out;
+};
+
+template <int NT>
+__device__ inline double block_sum(double v, double* red)
+{
+  v = wave_sum(v);
+  const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double t = 0;
+  for (int i = 0; i < NT / WAVE; i++) t += red[i];
+  return t;
+}
+
+__global__ void __launch_bounds__(FIN_THREADS) finalize_kernel(FinArgs A)
+{
+  __shared__ double red[FIN_THREADS / WAVE];
+  __shared__ double wsum[FIN_THREADS / WAVE];
+  const int z = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int nn = A.nu.n;
+  const double T = A.temp_used[z];
+  const double dens = A.zs.rho[z];
+  double* cdf = A.out.emis_cdf + (size_t)z * nn;
+  const double* ab = A.out.abs + (size_t)z * nn;
+  const double* sc = A.out.scat + (size_t)z * nn;
+
+  if (nn == 1) {
+    if (tid == 0) {
+      double Tg = A.zs.T_gas[z];
+      double bb_int = SB * pow(Tg, 4) / PI;
+      A.out.L_thermal[z] = 4 * PI * ab[0] * bb_int;
+      cdf[0] = 1;
+      A.out.planck_mean[z] = ab[0];
+      A.out.rosseland_mean[z] = ab[0] + sc[0];
+    }
+  } else {
+    // contiguous chunk of bins per thread; sequential inside the chunk
+    const int per = (nn + FIN_THREADS - 1) / FIN_THREADS;
+    const int j0 = tid * per, j1 = min(nn, j0 + per);
+    double local = 0, lth = 0, pm = 0, pn = 0, rm = 0, rn = 0;
+    for (int j = j0; j < j1; ++j) {
+      double ed = cdf[j];
+      local += ed;
+      lth += 4 * PI * ed;
+      double nuc = loc_center(A.nu, j), dnu = loc_delta(A.nu, j);
+      double zeta = H_PLANCK * nuc / K_BOLTZ / T;
+      double ezeta = exp(zeta);
+      double Bnu = 2.0 * nuc * nuc * nuc * H_PLANCK / C_LIGHT / C_LIGHT / (ezeta - 1);
+      if (isnan(Bnu)) Bnu = 0;
+      double W = Bnu * dnu;
+      pm += W * ab[j];
+      pn += W;
+      double dBdT = pow(nuc, 4) * ezeta / (ezeta - 1) / (ezeta - 1);
+      if (isnan(dBdT)) dBdT = 0;
+      double Wr = dBdT * dnu;
+      rm += Wr / (ab[j] + sc[j]);
+      rn += Wr;
+    }
+    // exclusive scan of chunk totals across the block (wave scan + LDS)
+    const int lane = tid % WAVE, w = tid / WAVE;
+    double incl = local;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+      double t = __shfl_up(incl, o, WAVE);
+      if (lane >= o) incl += t;
+    }
+    if (lane == WAVE - 1) wsum[w] = incl;
+    __syncthreads();
+    double base = 0;
+    for (int i = 0; i < w; i++) base += wsum[i];
+    double run = base + incl - local;
+    for (int j = j0; j < j1; ++j) { run += cdf[j]; cdf[j] = run; }
+    double L_th = block_sum<FIN_THREADS>(lth, red);
+    double P_m = block_sum<FIN_THREADS>(pm, red);
+    double P_n = block_sum<FIN_THREADS>(pn, red);
+    double R_m = block_sum<FIN_THREADS>(rm, red);
+    double R_n = block_sum<FIN_THREADS>(rn, red);
+    __syncthreads();
+    // cdf_array::normalize (cdf_array.h:62-75): NaNs propagate to the last
+    // cumulative entry, and a NaN or zero total makes the CDF all ones
+    const double last = cdf[nn - 1];
+    const bool flat = isnan(last) || last == 0;
+    __syncthreads();   // every thread has read the total before any rewrites it
+    for (int j = j0; j < j1; ++j) cdf[j] = flat ? 1.0 : cdf[j] / last;
+    if (tid == 0) {
+      A.out.L_thermal[z] = L_th;
+      A.out.planck_mean[z] = P_m / P_n;
+      A.out.rosseland_mean[z] = R_n / R_m;
+    }
+  }
+  if (tid == 0) {
+    // gamma-ray opacities with the undecayed composition (transport_opacity.cpp:190-203)
+    const int ne_ = A.zs.n_elems;
+    double compton = 0, photoion = 0;
+    for (int k = 0; k < ne_; k++) {
+      double d = A.zs.X[(size_t)z * ne_ + k] * dens;
+      double ndens = d / (M_PROTON * A.zs.elem_A[k]);
+      compton += ndens * THOMSON * A.zs.elem_Z[k];
+      double photo = pow(ALPHA_FS, 4.0) * 4.0 * sqrt(2.0);
+      photo *= pow(1.0 * A.zs.elem_Z[k], 5.0);
+      photo *= pow(M_E_MEV, 3.5);
+      photoion += ndens * 2.0 * THOMSON * photo;
+    }
+    A.out.compton_opac[z] = compton;
+    A.out.photoion_opac[z] = photoion;
+    // Fleck factor (transport_opacity.cpp:231-252)
+    double eps_imc;
+    if (A.p.radiative_eq) eps_imc = 1.;
+    else {
+      double f_imc = 0.;
+      if (A.p.fleck_alpha != 0) {
+        double beta = 4.0 * A_RAD * pow(A.zs.T_gas[z], 4) / (A.zs.e_gas[z] * dens);
+        double tfac = C_LIGHT * A.out.planck_mean[z] * A.p.dt;
+        f_imc = A.p.fleck_alpha * beta * tfac;
+      }
+      eps_imc = 1.0 / (1.0 + f_imc);
+    }
+    A.out.eps_imc[z] = eps_imc;
+  }
+}
+
+__global__ void max_status_kernel(const int32_t* st, int n, int32_t* worst)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && st[i] > 0) atomicMax(worst, st[i]);
+}
+
+}  // namespace sgpu
+
+using namespace sgpu;
+
+extern "C" int sgpu_opacity_build(sgpu_ctx* ctx, const sgpu_locate* nu_grid,
+                                  const sgpu_zone_state* zones, const sgpu_atoms* atoms,
+                                  const sgpu_opacity_params* params, sgpu_opacity_out* out)
+{
+  SGPU_REQUIRE(ctx && nu_grid && zones && params && out, "null argument");
+  SGPU_REQUIRE(out->abs && out->scat && out->emis_cdf && out->L_thermal && out->planck_mean &&
+               out->rosseland_mean && out->n_elec && out->compton_opac && out->photoion_opac &&
+               out->eps_imc, "missing opacity output array");
+  SGPU_REQUIRE(nu_grid->n >= 1 && nu_grid->x, "bad frequency grid");
+  const int nz = zones->n_zones;
+  SGPU_REQUIRE(nz >= 0, "negative zone count");
+  if (nz == 0) return SGPU_OK;
+  SGPU_REQUIRE(zones->n_elems >= 1 && zones->n_elems <= MAX_ELEMS, "n_elems %d outside [1, %d]",
+               zones->n_elems, MAX_ELEMS);
+  SGPU_REQUIRE(zones->rho && zones->T_gas && zones->X && zones->elem_Z && zones->elem_A,
+               "missing zone arrays");
+  SGPU_REQUIRE(params->fleck_alpha == 0 || params->radiative_eq || zones->e_gas,
+               "fleck_alpha != 0 needs e_gas");
+  sgpu_atoms at{};
+  if (atoms) at = *atoms;
+  if (at.n_atoms > 0) {
+    SGPU_REQUIRE(at.n_atoms == zones->n_elems, "one atom per element required (%d atoms, %d elements)",
+                 at.n_atoms, zones->n_elems);
+    SGPU_REQUIRE(at.max_levels >= 1, "max_levels must be set");
+  }
+  int n_lev = 0, n_ion = 0;
+  if (at.n_atoms > 0) {
+    n_lev = at.n_levels; n_ion = at.n_ions;
+    SGPU_REQUIRE(n_lev >= at.n_atoms && n_ion >= at.n_atoms, "n_levels / n_ions not set");
+    SGPU_REQUIRE(n_lev <= MAX_LEV && n_ion <= MAX_ION, "atomic data too large: %d levels (max %d), %d ions (max %d)",
+                 n_lev, MAX_LEV, n_ion, MAX_ION);
+  }
+  int st = 0;
+  size_t nion_sz = (size_t)nz * (n_ion > 0 ? n_ion : 1);
+  double* temp_used = (double*)scratch(ctx, "op_temp", sizeof(double) * nz, &st);
+  if (st) return st;
+  double* nion = (double*)scratch(ctx, "op_nion", sizeof(double) * nion_sz, &st);
+  if (st) return st;
+  double* upart = (double*)scratch(ctx, "op_upart", sizeof(double) * nion_sz, &st);
+  if (st) return st;
+  int32_t* worst = (int32_t*)scratch(ctx, "op_worst", sizeof(int32_t), &st);
+  if (st) return st;
+
+  timer_begin(ctx, "opacity");
+  StateArgs sa{*zones, at, *params, n_lev, n_ion, temp_used, nion, upart, *out};
+  hipLaunchKernelGGL(state_kernel, dim3(nz), dim3(WAVE), 0, ctx->stream, sa);
+  SGPU_LAUNCH_CHECK();
+  BinArgs ba{*nu_grid, *zones, at, *params, n_ion, temp_used, nion, upart,
+             out->abs, out->scat, out->emis_cdf, out->n_elec};
+  hipLaunchKernelGGL(bins_kernel, dim3((nu_grid->n + 255) / 256, nz), dim3(256), 0, ctx->stream, ba);
+  SGPU_LAUNCH_CHECK();
+  FinArgs fa{*nu_grid, *zones, *params, temp_used, *out};
+  hipLaunchKernelGGL(finalize_kernel, dim3(nz), dim3(FIN_THREADS), 0, ctx->stream, fa);
+  SGPU_LAUNCH_CHECK();
+  timer_end(ctx, "opacity");
+  if (!out->zone_status) return SGPU_OK;
+  SGPU_HIP(hipMemsetAsync(worst, 0, sizeof(int32_t), ctx->stream));
+  hipLaunchKernelGGL(max_status_kernel, dim3((nz + 255) / 256), dim3(256), 0, ctx->stream,
+                     out->zone_status, nz, worst);
+  SGPU_LAUNCH_CHECK();
+  int32_t hw = 0;
+  SGPU_HIP(hipMemcpyAsync(&hw, worst, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  SGPU_HIP(hipStreamSynchronize(ctx->stream));
+  return hw;
+}

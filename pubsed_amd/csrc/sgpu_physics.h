@@ -1,0 +1,127 @@
+// sgpu_physics.h -- device restatements of the reference's small physics
+// formulae used by the opacity build and the packet sources.
+#pragma once
+#include "sgpu_device.h"
+
+namespace sgpu {
+
+// radioactive::decay_composition, src/transport/radioactive.cpp:58-113;
+// X has n_el entries (mass fractions), modified in place.
+__device__ inline void decay_composition(int n_el, const int32_t* Z, const int32_t* A,
+                                         double* X, double t)
+{
+  double X_ni = 0, X_co = 0;
+  for (int i = 0; i < n_el; i++) {
+    if (Z[i] == 28 && A[i] == 56) X_ni = X[i];
+    if (Z[i] == 27 && A[i] == 56) X_co = X[i];
+  }
+  double ni_f = exp(-t / TAU_56NI);
+  double co_f = TAU_56CO / (TAU_56NI - TAU_56CO) * (exp(-t / TAU_56NI) - exp(-t / TAU_56CO));
+  double fe_f = 1 - ni_f - co_f;
+  double eco = exp(-t / TAU_56CO);
+  for (int i = 0; i < n_el; i++) {
+    if (Z[i] == 28 && A[i] == 56) X[i] = X_ni * ni_f;
+    if (Z[i] == 27 && A[i] == 56) X[i] = X_ni * co_f + X_co * eco;
+    if (Z[i] == 26 && A[i] == 56) X[i] += X_ni * fe_f + X_co * (1 - eco);
+  }
+  double X_cr = 0, X_vn = 0;
+  for (int i = 0; i < n_el; i++) {
+    if (Z[i] == 24 && A[i] == 48) X_cr = X[i];
+    if (Z[i] == 23 && A[i] == 48) X_vn = X[i];
+  }
+  double cr_f = exp(-t / TAU_48CR);
+  double vn_f = TAU_48V / (TAU_48CR - TAU_48V) * (exp(-t / TAU_48CR) - exp(-t / TAU_48V));
+  double ti_f = 1 - cr_f - vn_f;
+  double evn = exp(-t / TAU_48V);
+  for (int i = 0; i < n_el; i++) {
+    if (Z[i] == 24 && A[i] == 48) X[i] = X_cr * cr_f;
+    if (Z[i] == 23 && A[i] == 48) X[i] = X_cr * vn_f + X_vn * evn;
+    if (Z[i] == 22 && A[i] == 48) X[i] += X_cr * ti_f + X_vn * (1 - evn);
+  }
+}
+
+// radioactive::rprocess_heating_rate, radioactive.cpp:116-142 (Lippuner &
+// Roberts 2015 heating fit, Barnes et al. 2016 thermalisation); all of it is
+// deposited locally, so the gamma-ray fraction is zero.
+__device__ inline double rprocess_heating_rate(double t, double* gfrac)
+{
+  double td = t / 3600.0 / 24.0;
+  double eps = 8.4939E+09 * pow(td, -1.0 * 1.3642E+00) + 8.3425E+09 * exp(-1.0 * td / 3.6280E+00) +
+               8.8616E+08 * exp(-1.0 * td / 1.0847E+01);
+  double x = 2 * 0.17 * pow(td, 0.74);
+  double f = 0.36 * (exp(-0.56 * td) + log(1 + x) / x);
+  *gfrac = 0;
+  return eps * f;
+}
+
+// radioactive::decay_energy_rate for the 56Ni, 56Co and 48Cr chains,
+// radioactive.cpp:189-242 (elements with Z >= 58 route through the
+// r-process fit instead, as radioactive::decay does)
+__device__ inline double decay_energy_rate(int Z, int A, double t, double* gfrac)
+{
+  double total = 0, gtotal = 0;
+  if (Z == 28 && A == 56) {
+    double e_ni = exp(-t / TAU_56NI);
+    double e_co = exp(-t / TAU_56CO);
+    double ni56 = (e_ni / TAU_56NI);
+    double co56 = 1.0 / (TAU_56NI - TAU_56CO) * (e_ni - e_co);
+    double ni_E = ni56 * (E_56NI * MEV_TO_ERGS);
+    double co_E = co56 * (E_56CO * MEV_TO_ERGS);
+    gtotal = ni_E + (1 - CO_POS) * co_E;
+    total = ni_E + co_E;
+  }
+  if (Z == 27 && A == 56) {
+    double e_co = exp(-t / TAU_56CO);
+    double co56 = 1.0 / (TAU_56CO) * e_co;
+    double co_E = co56 * (E_56CO * MEV_TO_ERGS);
+    gtotal = (1 - CO_POS) * co_E;
+    total = co_E;
+  }
+  if (Z == 24 && A == 48) {
+    double e_cr = exp(-t / TAU_48CR);
+    double e_vn = exp(-t / TAU_48V);
+    double cr48 = (e_cr / TAU_48CR);
+    double vn48 = 1.0 / (TAU_48CR - TAU_48V) * (e_cr - e_vn);
+    double cr_E = cr48 * (E_48CR * MEV_TO_ERGS);
+    double vn_E = vn48 * (E_48V * MEV_TO_ERGS);
+    gtotal += cr_E + vn_E;
+    total += cr_E + vn_E;
+  }
+  *gfrac = gtotal / total;
+  if (total == 0) *gfrac = 0;
+  return total;
+}
+
+// radioactive::decay, radioactive.cpp:151-184
+__device__ inline double decay(int n_el, const int32_t* Z, const int32_t* A, const double* X,
+                               double t, double* gfrac, int force_rproc)
+{
+  int rproc = 0;
+  for (int k = 0; k < n_el; k++)
+    if (((Z[k] > 57) && (A[k] > 0)) || force_rproc) rproc = 1;
+  if (rproc) return rprocess_heating_rate(t, gfrac);
+  double total = 0, gtotal = 0, gf;
+  for (int k = 0; k < n_el; k++) {
+    double val = decay_energy_rate(Z[k], A[k], t, &gf);
+    val = val * X[k] / (A[k] * M_PROTON);
+    total += val;
+    gtotal += val * gf;
+  }
+  *gfrac = gtotal / total;
+  if (total == 0) *gfrac = 0;
+  return total;
+}
+
+// transport::klein_nishina, transport_opacity.cpp:306-316 (x in MeV)
+template <class R>
+__device__ __forceinline__ R klein_nishina(R x)
+{
+  x = x / (R)M_E_MEV;
+  R logfac = log(1 + 2 * x);
+  R term1 = (1 + x) / x / x / x * (2 * x * (1 + x) / (1 + 2 * x) - logfac);
+  R term2 = (R)1.0 / (R)2.0 / x * logfac;
+  R term3 = (R)-1.0 * (1 + 3 * x) / (1 + 2 * x) / (1 + 2 * x);
+  return (R).75 * (term1 + term2 + term3);
+}
+
+}  // namespace sgpu

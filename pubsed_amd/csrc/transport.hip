@@ -1,0 +1,1034 @@
+// transport.hip -- K4/K5/K6/K7: packet emission, propagation, compaction,
+// tally wipe / normalisation and the homologous expansion step.
+//
+// Propagation (K4) runs one packet per lane.  Two forms share the per-event
+// code (sgpu_geometry.h):
+//   SGPU_PREC_FP64   the plain drop-in: one thread per packet, FP64 geometry,
+//                    every tally a global FP64 atomic, as the reference's
+//                    omp-atomic loop (src/transport/transport.cpp:57-88,
+//                    260-484).
+//   SGPU_PREC_MIXED  persistent lanes that refill themselves with the next
+//                    packet when theirs finishes (no lane idles on a
+//                    long-lived neighbour), FP32 geometry and Doppler shifts,
+//                    packed FP32 (kappa, eps) opacity table (one 8-byte
+//                    gather per event), FP32 emissivity CDF, zone tallies
+//                    accumulated in LDS per workgroup and flushed once,
+//                    J_nu and spectra as FP64 global atomics.
+#include <hip/hip_runtime.h>
+#include "sgpu_internal.h"
+#include "sgpu_geometry.h"
+
+namespace sgpu {
+
+constexpr int PROP_THREADS = 256;
+constexpr int LDS_ZONE_MAX = 1024;   // zone tallies in LDS up to this many zones
+
+struct Counts {
+  uint32_t events, binshift, boundary, scatter, redist, escaped, absorbed, stopped, moving;
+};
+
+template <class R>
+struct Tab;
+
+template <>
+struct Tab<double> {
+  const double* abs;
+  const double* scat;
+  const double* cdf;
+};
+
+template <>
+struct Tab<float> {
+  const float2* ke;     // (kappa = abs + scat, eps = abs / kappa)
+  const float* cdf;
+};
+
+template <class R>
+struct PropArgs {
+  GridView<R> g;
+  Axis<R> nu;
+  Tab<R> tab;
+  const double* eps_imc;
+  const double* compton;
+  const double* photoion;
+  sgpu_transport_params p;
+  sgpu_packets pk;
+  int64_t n;
+  sgpu_tallies ta;
+  sgpu_counters* cnt;
+  int lds_tallies;   // zone scalars accumulated in LDS
+};
+
+// dynamic LDS of the propagation and emission kernels
+extern __shared__ double dyn_lds[];
+
+// zone-scalar tally sink: LDS (per workgroup, ds_add_f64 on dyn_lds) or
+// global FP64 atomics.  Fields are addressed by (field, zone).
+enum { T_EABS = 0, T_FX, T_FY, T_FZ, T_FR, T_LDEP, T_NFIELDS };
+
+struct ZoneSink {
+  double* g[T_NFIELDS];
+  int nz;
+  bool lds;
+  __device__ __forceinline__ void add(int field, int zone, double v) const
+  {
+    if (lds) atomicAdd(&dyn_lds[field * nz + zone], v);
+    else atomic_add_f64(&g[field][zone], v);
+  }
+};
+
+// transport::get_opacity (transport_opacity.cpp:268-299)
+__device__ __forceinline__ int get_opacity(const PropArgs<double>& A, const Pkt<double>& p,
+                                           double dshift, double* opac, double* eps)
+{
+  double nu = p.nu * dshift;
+  int i_nu = 0;
+  if (p.type == SGPU_PHOTON) {
+    i_nu = ax_within(A.nu, nu);
+    size_t k = (size_t)p.ind * A.nu.n + i_nu;
+    double a = A.tab.abs[k];
+    double s = 0;
+    if (!A.p.omit_scattering) s = A.tab.scat[k];
+    *opac = a + s;
+    *eps = (*opac == 0) ? 0 : a / *opac;
+  } else {
+    double co = A.compton[p.ind] * klein_nishina(p.nu);
+    double po = A.photoion[p.ind] * pow(p.nu, -3.5);
+    *opac = co + po;
+    *eps = po / (co + po);
+  }
+  return i_nu;
+}
+
+__device__ __forceinline__ int get_opacity(const PropArgs<float>& A, const Pkt<float>& p,
+                                           float dshift, float* opac, float* eps)
+{
+  float nu = p.nu * dshift;
+  int i_nu = 0;
+  if (p.type == SGPU_PHOTON) {
+    i_nu = ax_within(A.nu, nu);
+    float2 ke = A.tab.ke[(size_t)p.ind * A.nu.n + i_nu];
+    *opac = ke.x;
+    *eps = ke.y;
+  } else {
+    float co = (float)A.compton[p.ind] * klein_nishina(p.nu);
+    float po = (float)A.photoion[p.ind] * __powf(p.nu, -3.5f);
+    *opac = co + po;
+    *eps = po / (co + po);
+  }
+  return i_nu;
+}
+
+// transport::sample_photon_frequency (emission.cpp:27-46)
+template <class R>
+__device__ __forceinline__ void sample_photon_frequency(const PropArgs<R>& A, Pkt<R>& p, Counts& c)
+{
+  if (p.type == SGPU_PHOTON) {
+    const int nn = A.nu.n;
+    int inu = cdf_sample(A.tab.cdf + (size_t)p.ind * nn, nn, p.rng.template u<R>());
+    p.nu = ax_sample(A.nu, inu, p.rng.template u<R>());
+    c.redist++;
+  } else {
+    p.nu = 1;
+  }
+}
+
+// transport::isotropic_scatter (scatter.cpp:303-351); D divided by |D|
+template <class R>
+__device__ __forceinline__ void isotropic_scatter(const PropArgs<R>& A, Pkt<R>& p, int redist, Counts& c)
+{
+  const GridView<R>& g = A.g;
+  const R cl = (R)C_LIGHT;
+  R V[3] = {p.x[0] / g.t_now, p.x[1] / g.t_now, p.x[2] / g.t_now};
+  R vdotD = V[0] * p.D[0] + V[1] * p.D[1] + V[2] * p.D[2];
+  R beta2 = (V[0] * V[0] + V[1] * V[1] + V[2] * V[2]) / cl / cl;
+  R gamma = 1 / rsqrt_(1 - beta2);
+  R dshift_in = gamma * (1 - vdotD / cl);
+  p.e = p.e * (double)dshift_in;
+  p.nu = p.nu * dshift_in;
+  R Dn[3];
+  isotropic_dir(p, Dn);
+  if (redist) sample_photon_frequency(A, p, c);
+  for (int i = 0; i < 3; i++) V[i] = -1 * V[i];
+  R vdp = (Dn[0] * V[0] + Dn[1] * V[1] + Dn[2] * V[2]);
+  R vd_out = gamma * (1 - vdp / cl);
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+    p.D[k] = 1 / vd_out * (Dn[k] - gamma * V[k] / cl * (1 - gamma * vdp / cl / (gamma + 1)));
+  R norm = rsqrt_(p.D[0] * p.D[0] + p.D[1] * p.D[1] + p.D[2] * p.D[2]);
+  p.D[0] = p.D[0] / norm; p.D[1] = p.D[1] / norm; p.D[2] = p.D[2] / norm;
+  p.e = p.e * (double)vd_out;
+  p.nu = p.nu * vd_out;
+}
+
+// transport::compton_scatter (scatter.cpp:76-133)
+template <class R>
+__device__ __forceinline__ void compton_scatter(const PropArgs<R>& A, Pkt<R>& p, const ZoneSink& zs,
+                                                Counts& c)
+{
+  lorentz_transform(A.g, p, 0);
+  R E_ratio, Dn[3];
+  while (true) {
+    isotropic_dir(p, Dn);
+    R cost = p.D[0] * Dn[0] + p.D[1] * Dn[1] + p.D[2] * Dn[2];
+    E_ratio = 1 / (1 + p.nu / (R)M_E_MEV * (1 - cost));
+    R diff_cs = (R)0.5 * (E_ratio * E_ratio * (1 / E_ratio + E_ratio - 1 + cost * cost));
+    if (p.rng.template u<R>() < diff_cs) break;
+  }
+  p.nu = p.nu * E_ratio;
+  if (p.rng.template u<R>() > E_ratio) {
+    zs.add(T_LDEP, p.ind, p.e);
+    p.type = SGPU_PHOTON;
+    isotropic_dir(p, Dn);
+    sample_photon_frequency(A, p, c);
+  }
+  p.D[0] = Dn[0]; p.D[1] = Dn[1]; p.D[2] = Dn[2];
+  lorentz_transform(A.g, p, 1);
+}
+
+// transport::do_scatter (scatter.cpp:12-70)
+template <class R>
+__device__ __forceinline__ int do_scatter(const PropArgs<R>& A, Pkt<R>& p, R eps, const ZoneSink& zs,
+                                          Counts& c)
+{
+  int fate = SGPU_MOVING;
+  if (p.type == SGPU_PHOTON) {
+    if (p.rng.template u<R>() > eps) isotropic_scatter(A, p, 0, c);
+    else {
+      R z2 = p.rng.template u<R>();
+      if ((z2 > (R)A.eps_imc[p.ind]) || (A.p.radiative_eq)) isotropic_scatter(A, p, 1, c);
+      else fate = SGPU_ABSORBED;
+    }
+  } else if (p.type == SGPU_GAMMARAY) {
+    if (p.rng.template u<R>() > eps) compton_scatter(A, p, zs, c);
+    else {
+      zs.add(T_LDEP, p.ind, p.e);
+      p.type = SGPU_PHOTON;
+      isotropic_dir(p, p.D);
+      sample_photon_frequency(A, p, c);
+      lorentz_transform(A.g, p, 1);
+    }
+  }
+  return fate;
+}
+
+// one iteration of transport::propagate_monte_carlo (transport.cpp:266-481)
+template <class R>
+__device__ __forceinline__ int mc_event(const PropArgs<R>& A, Pkt<R>& p, double tstop,
+                                        const ZoneSink& zs, Counts& c)
+{
+  const int nn = A.nu.n;
+  const R cl = (R)C_LIGHT;
+  c.events++;
+  R d_bn = 0;
+  int new_ind = get_next_zone(A.g, p.x, p.D, p.ind, &d_bn);
+  R dshift = dshift_lab_to_comoving(A.g, p);
+  R kappa, eps;
+  int i_nu = get_opacity(A, p, dshift, &kappa, &eps);
+  R d_nu = ax_delta(A.nu, i_nu) / p.nu * cl / p.dvds;
+  if (p.dvds == 0) d_nu = rinf<R>();
+  if (d_nu < 0) d_nu = -1 * d_nu;
+  bool binshift = false;
+  if (d_nu < d_bn) { d_bn = d_nu; new_ind = p.ind; binshift = true; }
+  R tot_lab = kappa * dshift;
+  R tau_r = -1 * rlog<R>(1 - p.rng.template u<R>());
+  R d_sc = tau_r / tot_lab;
+  if (tot_lab == 0) d_sc = rinf<R>();
+  double d_tm = (tstop - p.t) * C_LIGHT;
+  if (A.p.steady_state) d_tm = INFINITY;
+  int event;
+  R this_d;
+  if ((d_sc < d_bn) && ((double)d_sc < d_tm)) { event = 0; this_d = d_sc; }
+  else if ((double)d_bn < d_tm) { event = 1; this_d = d_bn; }
+  else { event = 2; this_d = (R)d_tm; }
+  const double this_E = p.e * (double)this_d;
+  const int z = p.ind;
+  const double ds = (double)dshift, kap = (double)kappa;
+  if (p.type == SGPU_PHOTON) {
+    zs.add(T_EABS, z, this_E * ds * kap * (double)eps * ds * A.eps_imc[z]);
+    atomic_add_f64(&A.ta.J_nu[(size_t)z * (A.p.store_Jnu ? nn : 1) + (A.p.store_Jnu ? i_nu : 0)], this_E);
+  }
+  zs.add(T_FX, z, this_E * ds * kap * (double)p.D[0] * ds);
+  zs.add(T_FY, z, this_E * ds * kap * (double)p.D[1] * ds);
+  zs.add(T_FZ, z, this_E * ds * kap * (double)p.D[2] * ds);
+  R rr = rsqrt_(p.x[0] * p.x[0] + p.x[1] * p.x[1] + p.x[2] * p.x[2]);
+  R xdotD = p.x[0] * p.D[0] + p.x[1] * p.D[1] + p.x[2] * p.D[2];
+  zs.add(T_FR, z, this_E * ds * kap * (double)xdotD / (double)rr * ds);
+  p.x[0] += this_d * p.D[0];
+  p.x[1] += this_d * p.D[1];
+  p.x[2] += this_d * p.D[2];
+  p.t = p.t + (double)this_d / C_LIGHT;
+  int fate = SGPU_MOVING;
+  if (event == 1) {
+    c.boundary++;
+    if (binshift) c.binshift++;
+    if (new_ind == -1) {
+      if (A.p.boundary_in_reflect) { p.D[0] *= -1; p.D[1] *= -1; p.D[2] *= -1; }
+      else { p.ind = new_ind; fate = SGPU_ABSORBED; }
+    } else if (new_ind == -2) {
+      if (A.p.boundary_out_reflect) { p.D[0] *= -1; p.D[1] *= -1; p.D[2] *= -1; }
+      else { p.ind = new_ind; fate = SGPU_ESCAPED; }
+    } else {
+      p.ind = new_ind;
+    }
+  } else if (event == 0) {
+    c.scatter++;
+    fate = do_scatter(A, p, eps, zs, c);
+  } else {
+    fate = SGPU_STOPPED;
+  }
+  return fate;
+}
+
+// spectrum_array::count (spectrum_array.cpp:215-246)
+__device__ __forceinline__ void spectrum_count(const sgpu_spectrum& s, double t, double w, double E,
+                                               double D0, double D1, double D2)
+{
+  if (!s.flux) return;
+  double mu = D2;
+  double phi = atan2(D1, D0);
+  if (phi < 0) phi += 2. * PI;
+  int t_bin = loc_locate(s.time_grid, t);
+  int l_bin = loc_locate(s.nu_grid, w);
+  int m_bin = loc_locate(s.mu_grid, mu);
+  int p_bin = loc_locate(s.phi_grid, phi);
+  if (l_bin < 0) l_bin = 0;
+  if (l_bin >= s.nu_grid.n) l_bin = 0;
+  if ((t_bin < 0) || (l_bin < 0) || (m_bin < 0) || (p_bin < 0)) return;
+  if (t_bin >= s.time_grid.n) return;
+  if (m_bin >= s.mu_grid.n) return;
+  if (p_bin >= s.phi_grid.n) return;
+  int a3 = s.phi_grid.n, a2 = s.mu_grid.n * a3, a1 = s.nu_grid.n * a2;
+  size_t ind = (size_t)t_bin * a1 + (size_t)l_bin * a2 + (size_t)m_bin * a3 + p_bin;
+  atomic_add_f64(&s.flux[ind], E);
+  atomic_add_f64(&s.click[ind], 1.0);
+}
+
+template <class R>
+__device__ __forceinline__ void load_packet(const PropArgs<R>& A, int64_t i, Pkt<R>& p)
+{
+  const sgpu_packets& k = A.pk;
+  p.x[0] = (R)k.x[i]; p.x[1] = (R)k.y[i]; p.x[2] = (R)k.z[i];
+  p.D[0] = (R)k.Dx[i]; p.D[1] = (R)k.Dy[i]; p.D[2] = (R)k.Dz[i];
+  p.t = k.t[i]; p.e = k.e[i]; p.nu = (R)k.nu[i];
+  p.ind = k.ind[i]; p.type = k.type[i];
+  p.rng = make_rng(A.p.seed, k.id[i], k.draw[i]);
+  p.dvds = 0;
+}
+
+template <class R>
+__device__ __forceinline__ void finish_packet(const PropArgs<R>& A, int64_t i, Pkt<R>& p, int fate, Counts& c)
+{
+  const sgpu_packets& k = A.pk;
+  if (fate == SGPU_ESCAPED) {
+    double x0 = p.x[0], x1 = p.x[1], x2 = p.x[2], D0 = p.D[0], D1 = p.D[1], D2 = p.D[2];
+    double t_obs = p.t - (x0 * D0 + x1 * D1 + x2 * D2) / C_LIGHT;
+    if (p.type == SGPU_PHOTON) spectrum_count(A.ta.optical, t_obs, (double)p.nu, p.e, D0, D1, D2);
+    if (p.type == SGPU_GAMMARAY) spectrum_count(A.ta.gamma, t_obs, (double)p.nu, p.e, D0, D1, D2);
+    p.t = t_obs;
+    c.escaped++;
+  } else if (fate == SGPU_ABSORBED) c.absorbed++;
+  else if (fate == SGPU_STOPPED) c.stopped++;
+  else c.moving++;
+  k.x[i] = p.x[0]; k.y[i] = p.x[1]; k.z[i] = p.x[2];
+  k.Dx[i] = p.D[0]; k.Dy[i] = p.D[1]; k.Dz[i] = p.D[2];
+  k.t[i] = p.t; k.e[i] = p.e; k.nu[i] = p.nu;
+  k.ind[i] = p.ind; k.type[i] = p.type; k.fate[i] = fate;
+  k.draw[i] = p.rng.draw;
+}
+
+__device__ __forceinline__ void flush_counts(sgpu_counters* cnt, const Counts& c, uint64_t npk)
+{
+  if (!cnt) return;
+  unsigned long long v[10] = {npk, c.events, c.binshift, c.boundary, c.scatter, c.redist,
+                              c.escaped, c.absorbed, c.stopped, c.moving};
+  unsigned long long* dst = (unsigned long long*)cnt;
+  const int lane = threadIdx.x % WAVE;
+#pragma unroll
+  for (int f = 0; f < 10; f++) {
+    unsigned long long s = wave_sum(v[f]);
+    if (lane == 0 && s) atomicAdd(&dst[f], s);
+  }
+}
+
+// PERSIST = false: one thread per packet (the plain drop-in).
+// PERSIST = true : lanes stride through the packets and refill on finish.
+template <class R, bool PERSIST>
+__global__ void __launch_bounds__(PROP_THREADS) propagate_kernel(PropArgs<R> A)
+{
+  const int nz = A.g.n_zones;
+  const bool use_lds = A.lds_tallies;
+  ZoneSink zs = {{A.ta.e_abs, A.ta.fx_rad, A.ta.fy_rad, A.ta.fz_rad, A.ta.fr_rad, A.ta.L_radio_dep},
+                 nz, use_lds};
+  if (use_lds) {
+    for (int k = threadIdx.x; k < T_NFIELDS * nz; k += blockDim.x) dyn_lds[k] = 0;
+    __syncthreads();
+  }
+  Counts c = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const double tstop = A.p.t_now + A.p.dt;
+  const int64_t stride = PERSIST ? (int64_t)gridDim.x * blockDim.x : A.n;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t mine = 0;
+  bool have = false;
+  int n_ev = 0;
+  Pkt<R> p;
+  while (true) {
+    if (!have) {
+      if (i >= A.n) break;
+      load_packet(A, i, p);
+      mine++;
+      p.ind = get_zone(A.g, p.x);
+      n_ev = 0;
+      if (p.ind < 0) {
+        finish_packet(A, i, p, p.ind == -1 ? SGPU_ABSORBED : SGPU_ESCAPED, c);
+        i += stride;
+        continue;
+      }
+      have = true;
+    }
+    int fate = mc_event(A, p, tstop, zs, c);
+    n_ev++;
+    if (fate != SGPU_MOVING || (A.p.max_events > 0 && n_ev >= A.p.max_events)) {
+      finish_packet(A, i, p, fate, c);
+      have = false;
+      i += stride;
+    }
+  }
+  flush_counts(A.cnt, c, mine);
+  if (use_lds) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < T_NFIELDS * nz; k += blockDim.x) {
+      double v = dyn_lds[k];
+      if (v != 0) atomic_add_f64(&zs.g[k / nz][k % nz], v);
+    }
+  }
+}
+
+// FP32 (kappa, eps) table and CDF for the mixed form
+__global__ void pack_tables_kernel(const double* abs, const double* scat, const double* cdf,
+                                   int omit_scat, size_t n, float2* ke, float* cdf32)
+{
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double a = abs[k];
+  double s = omit_scat ? 0.0 : scat[k];
+  double op = a + s;
+  ke[k] = make_float2((float)op, op == 0 ? 0.f : (float)(a / op));
+  cdf32[k] = (float)cdf[k];
+}
+
+__global__ void to_f32_kernel(const double* src, float* dst, int n)
+{
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst[k] = (float)src[k];
+}
+
+// ------------------------------------------------------------------
+// K5: radioactive emission (emission.cpp:165-243, :54-105)
+// ------------------------------------------------------------------
+struct EmitPrep {
+  sgpu_grid g;
+  sgpu_zone_state zs;
+  double t_now;
+  int force_rproc;
+  double* cdf;       // [nz] zone emission CDF
+  double* gfrac;     // [nz]
+  double* L_emit;    // [nz] tallies.L_radio_emit
+  double* L_tot;     // [1]
+};
+
+// single workgroup: per-zone decay luminosity, CDF (block scan), L_tot
+__global__ void __launch_bounds__(1024) emit_prep_kernel(EmitPrep P)
+{
+  __shared__ double wsum[1024 / WAVE];
+  const int nz = P.g.n_zones, tid = threadIdx.x, NT = blockDim.x;
+  const int per = (nz + NT - 1) / NT;
+  const int j0 = tid * per, j1 = min(nz, j0 + per);
+  double local = 0;
+  for (int i = j0; i < j1; ++i) {
+    double gf;
+    double L = decay(P.zs.n_elems, P.zs.elem_Z, P.zs.elem_A, P.zs.X + (size_t)i * P.zs.n_elems,
+                     P.t_now, &gf, P.force_rproc);
+    L = P.zs.rho[i] * L * P.g.vol[i];
+    P.L_emit[i] = L;
+    P.gfrac[i] = gf;
+    P.cdf[i] = L;
+    local += L;
+  }
+  const int lane = tid % WAVE, w = tid / WAVE;
+  double incl = local;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    double t = __shfl_up(incl, o, WAVE);
+    if (lane >= o) incl += t;
+  }
+  if (lane == WAVE - 1) wsum[w] = incl;
+  __syncthreads();
+  double base = 0;
+  for (int i = 0; i < w; i++) base += wsum[i];
+  double run = base + incl - local;
+  for (int i = j0; i < j1; ++i) { run += P.cdf[i]; P.cdf[i] = run; }
+  __syncthreads();
+  const double last = P.cdf[nz - 1];
+  const bool flat = isnan(last) || last == 0;
+  __syncthreads();
+  for (int i = j0; i < j1; ++i) P.cdf[i] = flat ? 1.0 : P.cdf[i] / last;
+  if (tid == 0) {
+    double tot = 0;
+    for (int i = 0; i < NT / WAVE; i++) tot += wsum[i];
+    *P.L_tot = tot;
+  }
+}
+
+struct EmitArgs {
+  GridView<double> g;
+  Axis<double> nu;
+  const double* emis_cdf;
+  const double* zcdf;
+  const double* gfrac;
+  const double* L_tot;
+  double t_now, dt;
+  uint64_t seed, id_base;
+  int64_t n_emit, offset;
+  sgpu_packets pk;
+  double* L_dep;
+  int lds_dep;
+};
+
+__device__ __forceinline__ void sample_in_zone(const GridView<double>& g, int i, const double* ran, double* r)
+{
+  if (g.kind == SGPU_GRID_1D_SPHERE) {
+    double r_0 = (i == 0) ? g.r_out.min : g.r_out.x[i - 1];
+    double ro = g.r_out.x[i];
+    double r_samp = pow(r_0 * r_0 * r_0 + ran[0] * (ro * ro * ro - r_0 * r_0 * r_0), 1.0 / 3.0);
+    double mu = 1 - 2.0 * ran[1];
+    double phi = 2.0 * PI * ran[2];
+    double sin_theta = sqrt(1 - mu * mu);
+    r[0] = r_samp * sin_theta * cos(phi);
+    r[1] = r_samp * sin_theta * sin(phi);
+    r[2] = r_samp * mu;
+  } else {
+    const int nz = g.z_out.n;
+    const int ix = i / nz, iz = i - ix * nz;
+    double phi = 2 * PI * ran[1];
+    double p_in = ax_left(g.x_out, ix);
+    double p_out = g.x_out.x[ix];
+    double p_samp = sqrt(p_in * p_in + ran[0] * (p_out * p_out - p_in * p_in));
+    r[0] = p_samp * cos(phi);
+    r[1] = p_samp * sin(phi);
+    r[2] = ax_left(g.z_out, iz) + ran[2] * ax_delta(g.z_out, iz);
+  }
+}
+
+__global__ void __launch_bounds__(256) emit_kernel(EmitArgs A)
+{
+  const int nz = A.g.n_zones;
+  if (A.lds_dep) {
+    for (int k = threadIdx.x; k < nz; k += blockDim.x) dyn_lds[k] = 0;
+    __syncthreads();
+  }
+  const double L_tot = *A.L_tot;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < A.n_emit && L_tot != 0) {
+    const double E_p = L_tot * A.dt / (1.0 * A.n_emit);
+    Pkt<double> p;
+    p.rng = make_rng(A.seed, A.id_base + (uint64_t)q, 0);
+    int i = cdf_sample(A.zcdf, nz, p.rng.uniform());
+    double t = A.t_now + A.dt * p.rng.uniform();
+    if (p.rng.uniform() < A.gfrac[i]) p.type = SGPU_GAMMARAY;
+    else {
+      p.type = SGPU_PHOTON;
+      if (A.lds_dep) atomicAdd(&dyn_lds[i], E_p);
+      else atomic_add_f64(&A.L_dep[i], E_p);
+    }
+    p.ind = i;
+    double ran[3];
+    ran[0] = p.rng.uniform(); ran[1] = p.rng.uniform(); ran[2] = p.rng.uniform();
+    sample_in_zone(A.g, i, ran, p.x);
+    isotropic_dir(p, p.D);
+    if (p.type == SGPU_PHOTON) {
+      const int nn = A.nu.n;
+      int inu = cdf_sample(A.emis_cdf + (size_t)i * nn, nn, p.rng.uniform());
+      p.nu = ax_sample(A.nu, inu, p.rng.uniform());
+    } else {
+      p.nu = 1;
+    }
+    p.e = E_p;
+    lorentz_transform(A.g, p, 1);
+    p.t = t;
+    const int64_t o = A.offset + q;
+    const sgpu_packets& k = A.pk;
+    k.x[o] = p.x[0]; k.y[o] = p.x[1]; k.z[o] = p.x[2];
+    k.Dx[o] = p.D[0]; k.Dy[o] = p.D[1]; k.Dz[o] = p.D[2];
+    k.t[o] = p.t; k.e[o] = p.e; k.nu[o] = p.nu;
+    k.ind[o] = p.ind; k.type[o] = p.type; k.fate[o] = SGPU_MOVING;
+    k.draw[o] = p.rng.draw; k.id[o] = A.id_base + (uint64_t)q;
+  }
+  if (A.lds_dep) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < nz; k += blockDim.x)
+      if (dyn_lds[k] != 0) atomic_add_f64(&A.L_dep[k], dyn_lds[k]);
+  }
+}
+
+// ------------------------------------------------------------------
+// K6: stream compaction (clean_up_particle_vector), order-keeping
+// ------------------------------------------------------------------
+constexpr int CMP_THREADS = 256;
+
+__global__ void keep_count_kernel(const int32_t* fate, int64_t n, int32_t* blk_keep, int32_t* blk_esc)
+{
+  __shared__ int ks[CMP_THREADS / WAVE], es[CMP_THREADS / WAVE];
+  int64_t i = (int64_t)blockIdx.x * CMP_THREADS + threadIdx.x;
+  int f = i < n ? fate[i] : SGPU_ESCAPED + 100;
+  int keep = (i < n) && f != SGPU_ESCAPED && f != SGPU_ABSORBED;
+  int esc = (i < n) && f == SGPU_ESCAPED;
+  int k = wave_sum(keep), e = wave_sum(esc);
+  if (threadIdx.x % WAVE == 0) { ks[threadIdx.x / WAVE] = k; es[threadIdx.x / WAVE] = e; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tk = 0, te = 0;
+    for (int w = 0; w < CMP_THREADS / WAVE; w++) { tk += ks[w]; te += es[w]; }
+    blk_keep[blockIdx.x] = tk;
+    blk_esc[blockIdx.x] = te;
+  }
+}
+
+// single workgroup exclusive scan of block counts; totals to out[0..1]
+__global__ void __launch_bounds__(1024) scan_blocks_kernel(int32_t* blk_keep, const int32_t* blk_esc,
+                                                           int nb, long long* totals)
+{
+  __shared__ long long wsum[1024 / WAVE];
+  __shared__ long long esum[1024 / WAVE];
+  const int tid = threadIdx.x, NT = blockDim.x;
+  const int per = (nb + NT - 1) / NT;
+  const int j0 = tid * per, j1 = min(nb, j0 + per);
+  long long local = 0, le = 0;
+  for (int j = j0; j < j1; ++j) { local += blk_keep[j]; le += blk_esc[j]; }
+  const int lane = tid % WAVE, w = tid / WAVE;
+  long long incl = local;
+  for (int o = 1; o < WAVE; o <<= 1) {
+    long long t = __shfl_up(incl, o, WAVE);
+    if (lane >= o) incl += t;
+  }
+  long long es = le;
+  for (int o = 32; o > 0; o >>= 1) es += __shfl_xor(es, o, WAVE);
+  if (lane == WAVE - 1) wsum[w] = incl;
+  if (lane == 0) esum[w] = es;
+  __syncthreads();
+  long long base = 0;
+  for (int i = 0; i < w; i++) base += wsum[i];
+  long long run = base + incl - local;
+  for (int j = j0; j < j1; ++j) { int v = blk_keep[j]; blk_keep[j] = (int32_t)run; run += v; }
+  if (tid == 0) {
+    long long tk = 0, te = 0;
+    for (int i = 0; i < NT / WAVE; i++) { tk += wsum[i]; te += esum[i]; }
+    totals[0] = tk;
+    totals[1] = te;
+  }
+}
+
+__global__ void scatter_keep_kernel(sgpu_packets src, sgpu_packets dst, int64_t n, const int32_t* blk_off)
+{
+  __shared__ int woff[CMP_THREADS / WAVE];
+  int64_t i = (int64_t)blockIdx.x * CMP_THREADS + threadIdx.x;
+  int f = i < n ? src.fate[i] : SGPU_ESCAPED;
+  int keep = (i < n) && f != SGPU_ESCAPED && f != SGPU_ABSORBED;
+  const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+  unsigned long long mask = __ballot(keep);
+  int before = __popcll(mask & ((1ull << lane) - 1ull));
+  if (lane == 0) woff[w] = __popcll(mask);
+  __syncthreads();
+  int base = blk_off[blockIdx.x];
+  for (int k = 0; k < w; k++) base += woff[k];
+  if (keep) {
+    int64_t o = base + before;
+    dst.x[o] = src.x[i]; dst.y[o] = src.y[i]; dst.z[o] = src.z[i];
+    dst.Dx[o] = src.Dx[i]; dst.Dy[o] = src.Dy[i]; dst.Dz[o] = src.Dz[i];
+    dst.t[o] = src.t[i]; dst.e[o] = src.e[i]; dst.nu[o] = src.nu[i];
+    dst.ind[o] = src.ind[i]; dst.type[o] = src.type[i]; dst.fate[o] = src.fate[i];
+    dst.draw[o] = src.draw[i]; dst.id[o] = src.id[i];
+  }
+}
+
+template <class T>
+__global__ void copy_kernel(const T* __restrict__ s, T* __restrict__ d, int64_t n)
+{
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = s[i];
+}
+
+__global__ void fill_kernel(double* p, int64_t n, double v)
+{
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------
+// K7: tally normalisation (transport_mpi.cpp:415-441)
+// ------------------------------------------------------------------
+__global__ void __launch_bounds__(256) normalize_kernel(sgpu_locate nu, const double* vol, double dt,
+                                                        int store_Jnu, sgpu_tallies ta)
+{
+  __shared__ double red[256 / WAVE];
+  const int i = blockIdx.x, tid = threadIdx.x, nn = nu.n;
+  const double v = vol[i];
+  if (tid == 0) {
+    ta.e_abs[i] /= v * dt;
+    ta.L_radio_dep[i] /= v * dt;
+    ta.L_radio_emit[i] /= v;
+    ta.fr_rad[i] /= v * C_LIGHT * dt;
+  }
+  if (nn == 1 || !store_Jnu) {
+    if (tid == 0) ta.e_rad[i] = ta.J_nu[(size_t)i * (store_Jnu ? nn : 1)] / (v * dt * C_LIGHT);
+    return;
+  }
+  double es = 0;
+  for (int j = tid; j < nn; j += blockDim.x) {
+    double* J = &ta.J_nu[(size_t)i * nn + j];
+    double dnu = loc_delta(nu, j);
+    *J /= v * dt * 4 * PI * dnu;
+    es += *J * dnu * 4 * PI / C_LIGHT;
+  }
+  es = wave_sum(es);
+  if (tid % WAVE == 0) red[tid / WAVE] = es;
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0;
+    for (int w = 0; w < 256 / WAVE; w++) t += red[w];
+    ta.e_rad[i] = t;
+  }
+}
+
+// ------------------------------------------------------------------
+// homologous step (hydro_homologous.cpp:27-61, grid expand)
+// ------------------------------------------------------------------
+__global__ void homologous_kernel(int kind, int nz, double* rho, double* r_out, int nr, double* x_out,
+                                  int nx, double* z_out, int nzz, double* vol, double r_min_new, double e)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nz) rho[i] = rho[i] / e / e / e;
+  if (kind == SGPU_GRID_1D_SPHERE) {
+    if (i < nr) r_out[i] *= e;
+  } else {
+    if (i < nx) x_out[i] *= e;
+    if (i < nzz) z_out[i] *= e;
+    if (i < nz) vol[i] = vol[i] * e * e * e;
+  }
+}
+
+__global__ void shell_volume_kernel(int nz, const double* r_out, double r_min, double* vol)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nz) return;
+  double r0 = (i == 0) ? r_min : r_out[i - 1];
+  vol[i] = 4.0 * PI / 3.0 * (r_out[i] * r_out[i] * r_out[i] - r0 * r0 * r0);
+}
+
+}  // namespace sgpu
+
+using namespace sgpu;
+
+namespace {
+
+template <class R>
+Axis<R> make_axis(const sgpu_locate& l, const R* x)
+{
+  Axis<R> a;
+  a.n = l.n; a.type = l.type; a.min = (R)l.min; a.del = (R)l.del;
+  a.inv_log1p = (R)(1.0 / log1p(l.del));
+  a.x = x;
+  return a;
+}
+
+template <class R>
+GridView<R> make_grid(const sgpu_grid& g, const R* r, const R* xo, const R* zo)
+{
+  GridView<R> v;
+  v.kind = g.kind; v.n_zones = g.n_zones;
+  v.t_now = (R)g.t_now; v.inv_t = (R)(1.0 / g.t_now); v.r_core = (R)g.r_core;
+  v.r_out = make_axis<R>(g.r_out, r);
+  v.x_out = make_axis<R>(g.x_out, xo);
+  v.z_out = make_axis<R>(g.z_out, zo);
+  return v;
+}
+
+int check_grid(const sgpu_grid* g)
+{
+  SGPU_REQUIRE(g, "null grid");
+  SGPU_REQUIRE(g->kind == SGPU_GRID_1D_SPHERE || g->kind == SGPU_GRID_2D_CYLN, "unknown grid kind %d", g->kind);
+  SGPU_REQUIRE(g->t_now > 0, "grid t_now must be > 0");
+  SGPU_REQUIRE(g->vol, "grid volumes missing");
+  if (g->kind == SGPU_GRID_1D_SPHERE)
+    SGPU_REQUIRE(g->r_out.x && g->r_out.n == g->n_zones && g->n_zones > 0, "1D grid needs r_out with n_zones bins");
+  else
+    SGPU_REQUIRE(g->x_out.x && g->z_out.x && (int64_t)g->x_out.n * g->z_out.n == g->n_zones,
+                 "2D grid needs x_out, z_out with nx*nz = n_zones");
+  return SGPU_OK;
+}
+
+int check_packets(const sgpu_packets* p)
+{
+  SGPU_REQUIRE(p && p->x && p->y && p->z && p->Dx && p->Dy && p->Dz && p->t && p->e && p->nu &&
+               p->ind && p->type && p->fate && p->draw && p->id, "packet arrays missing");
+  return SGPU_OK;
+}
+
+unsigned grid_blocks(int64_t n, int threads)
+{
+  return (unsigned)((n + threads - 1) / threads);
+}
+
+}  // namespace
+
+extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_locate* nu_grid,
+                              const sgpu_tables* tables, const sgpu_transport_params* params,
+                              sgpu_packets* packets, int64_t n, sgpu_tallies* tallies,
+                              sgpu_counters* counters)
+{
+  SGPU_REQUIRE(ctx && nu_grid && tables && params && tallies, "null argument");
+  int st = check_grid(grid);
+  if (st) return st;
+  st = check_packets(packets);
+  if (st) return st;
+  SGPU_REQUIRE(n >= 0 && n <= packets->capacity, "n = %lld outside [0, capacity = %lld]", (long long)n,
+               (long long)packets->capacity);
+  SGPU_REQUIRE(nu_grid->n >= 1 && nu_grid->x, "bad frequency grid");
+  SGPU_REQUIRE(tables->abs && tables->emis_cdf && tables->eps_imc && tables->compton_opac &&
+               tables->photoion_opac && (tables->scat || params->omit_scattering), "opacity tables missing");
+  SGPU_REQUIRE(tallies->e_abs && tallies->J_nu && tallies->fx_rad && tallies->fy_rad && tallies->fz_rad &&
+               tallies->fr_rad && tallies->L_radio_dep, "tally arrays missing");
+  SGPU_REQUIRE(params->precision == SGPU_PREC_FP64 || params->precision == SGPU_PREC_MIXED,
+               "unknown precision %d", params->precision);
+  if (n == 0) return SGPU_OK;
+  const int nz = grid->n_zones, nn = nu_grid->n;
+  timer_begin(ctx, "propagate");
+  if (params->precision == SGPU_PREC_FP64) {
+    PropArgs<double> A;
+    A.g = make_grid<double>(*grid, grid->r_out.x, grid->x_out.x, grid->z_out.x);
+    A.nu = make_axis<double>(*nu_grid, nu_grid->x);
+    A.tab = {tables->abs, tables->scat, tables->emis_cdf};
+    A.eps_imc = tables->eps_imc; A.compton = tables->compton_opac; A.photoion = tables->photoion_opac;
+    A.p = *params; A.pk = *packets; A.n = n; A.ta = *tallies; A.cnt = counters; A.lds_tallies = 0;
+    hipLaunchKernelGGL((propagate_kernel<double, false>), dim3(grid_blocks(n, PROP_THREADS)),
+                       dim3(PROP_THREADS), 0, ctx->stream, A);
+    SGPU_LAUNCH_CHECK();
+  } else {
+    const size_t ntab = (size_t)nz * nn;
+    float2* ke = (float2*)scratch(ctx, "prop_ke", sizeof(float2) * ntab, &st);
+    if (st) return st;
+    float* cdf32 = (float*)scratch(ctx, "prop_cdf32", sizeof(float) * ntab, &st);
+    if (st) return st;
+    const int n_edges = nn + grid->r_out.n + grid->x_out.n + grid->z_out.n;
+    float* e32 = (float*)scratch(ctx, "prop_edges32", sizeof(float) * n_edges, &st);
+    if (st) return st;
+    hipLaunchKernelGGL(pack_tables_kernel, dim3(grid_blocks((int64_t)ntab, 256)), dim3(256), 0, ctx->stream,
+                       tables->abs, tables->scat, tables->emis_cdf, params->omit_scattering, ntab, ke, cdf32);
+    SGPU_LAUNCH_CHECK();
+    float* nu32 = e32;
+    float* r32 = nu32 + nn;
+    float* x32 = r32 + grid->r_out.n;
+    float* z32 = x32 + grid->x_out.n;
+    hipLaunchKernelGGL(to_f32_kernel, dim3(grid_blocks(nn, 256)), dim3(256), 0, ctx->stream, nu_grid->x, nu32, nn);
+    if (grid->r_out.n > 0 && grid->r_out.x)
+      hipLaunchKernelGGL(to_f32_kernel, dim3(grid_blocks(grid->r_out.n, 256)), dim3(256), 0, ctx->stream,
+                         grid->r_out.x, r32, grid->r_out.n);
+    if (grid->x_out.n > 0 && grid->x_out.x)
+      hipLaunchKernelGGL(to_f32_kernel, dim3(grid_blocks(grid->x_out.n, 256)), dim3(256), 0, ctx->stream,
+                         grid->x_out.x, x32, grid->x_out.n);
+    if (grid->z_out.n > 0 && grid->z_out.x)
+      hipLaunchKernelGGL(to_f32_kernel, dim3(grid_blocks(grid->z_out.n, 256)), dim3(256), 0, ctx->stream,
+                         grid->z_out.x, z32, grid->z_out.n);
+    SGPU_LAUNCH_CHECK();
+    PropArgs<float> A;
+    A.g = make_grid<float>(*grid, r32, x32, z32);
+    A.nu = make_axis<float>(*nu_grid, nu32);
+    A.tab = {ke, cdf32};
+    A.eps_imc = tables->eps_imc; A.compton = tables->compton_opac; A.photoion = tables->photoion_opac;
+    A.p = *params; A.pk = *packets; A.n = n; A.ta = *tallies; A.cnt = counters;
+    A.lds_tallies = nz <= LDS_ZONE_MAX;
+    const size_t lds = A.lds_tallies ? sizeof(double) * 6 * nz : 0;
+    // persistent lanes: enough workgroups to fill every CU several times over
+    int64_t want = (int64_t)ctx->n_cu * 8;
+    int64_t need = grid_blocks(n, PROP_THREADS);
+    unsigned blocks = (unsigned)(want < need ? want : need);
+    hipLaunchKernelGGL((propagate_kernel<float, true>), dim3(blocks), dim3(PROP_THREADS), lds, ctx->stream, A);
+    SGPU_LAUNCH_CHECK();
+  }
+  timer_end(ctx, "propagate");
+  return SGPU_OK;
+}
+
+extern "C" int sgpu_emit_radioactive(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_locate* nu_grid,
+                                     const sgpu_zone_state* zones, const sgpu_tables* tables,
+                                     const sgpu_transport_params* params, int64_t n_emit,
+                                     int32_t force_rproc, uint64_t id_base, sgpu_packets* packets,
+                                     int64_t offset, sgpu_tallies* tallies, int64_t* n_out)
+{
+  SGPU_REQUIRE(ctx && nu_grid && zones && tables && params && tallies && n_out, "null argument");
+  int st = check_grid(grid);
+  if (st) return st;
+  st = check_packets(packets);
+  if (st) return st;
+  SGPU_REQUIRE(zones->n_zones == grid->n_zones, "zone count mismatch");
+  SGPU_REQUIRE(n_emit >= 0 && offset >= 0 && offset + n_emit <= packets->capacity,
+               "emission range [%lld, %lld) exceeds capacity %lld", (long long)offset,
+               (long long)(offset + n_emit), (long long)packets->capacity);
+  SGPU_REQUIRE(tables->emis_cdf && tallies->L_radio_dep && tallies->L_radio_emit, "missing arrays");
+  *n_out = 0;
+  const int nz = grid->n_zones;
+  double* zcdf = (double*)scratch(ctx, "emit_zcdf", sizeof(double) * (2 * (size_t)nz + 1), &st);
+  if (st) return st;
+  double* gfr = zcdf + nz;
+  double* ltot = gfr + nz;
+  timer_begin(ctx, "emit");
+  EmitPrep P{*grid, *zones, params->t_now, force_rproc, zcdf, gfr, tallies->L_radio_emit, ltot};
+  hipLaunchKernelGGL(emit_prep_kernel, dim3(1), dim3(1024), 0, ctx->stream, P);
+  SGPU_LAUNCH_CHECK();
+  if (n_emit > 0) {
+    EmitArgs A;
+    A.g = make_grid<double>(*grid, grid->r_out.x, grid->x_out.x, grid->z_out.x);
+    A.nu = make_axis<double>(*nu_grid, nu_grid->x);
+    A.emis_cdf = tables->emis_cdf; A.zcdf = zcdf; A.gfrac = gfr; A.L_tot = ltot;
+    A.t_now = params->t_now; A.dt = params->dt; A.seed = params->seed; A.id_base = id_base;
+    A.n_emit = n_emit; A.offset = offset; A.pk = *packets; A.L_dep = tallies->L_radio_dep;
+    A.lds_dep = nz <= 8192;
+    hipLaunchKernelGGL(emit_kernel, dim3(grid_blocks(n_emit, 256)), dim3(256),
+                       A.lds_dep ? sizeof(double) * nz : 0, ctx->stream, A);
+    SGPU_LAUNCH_CHECK();
+  }
+  timer_end(ctx, "emit");
+  double hl = 0;
+  SGPU_HIP(hipMemcpyAsync(&hl, ltot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  SGPU_HIP(hipStreamSynchronize(ctx->stream));
+  *n_out = (hl == 0) ? 0 : n_emit;
+  return SGPU_OK;
+}
+
+extern "C" int sgpu_compact(sgpu_ctx* ctx, sgpu_packets* packets, int64_t n, sgpu_packets* scr,
+                            int64_t* n_alive, int64_t* n_escaped)
+{
+  SGPU_REQUIRE(ctx && n_alive && n_escaped, "null argument");
+  int st = check_packets(packets);
+  if (st) return st;
+  st = check_packets(scr);
+  if (st) return st;
+  SGPU_REQUIRE(n >= 0 && n <= packets->capacity && n <= scr->capacity, "n exceeds capacity");
+  *n_alive = 0; *n_escaped = 0;
+  if (n == 0) return SGPU_OK;
+  const unsigned nb = grid_blocks(n, CMP_THREADS);
+  int32_t* blk = (int32_t*)scratch(ctx, "cmp_blk", sizeof(int32_t) * 2 * (size_t)nb + 64, &st);
+  if (st) return st;
+  long long* tot = (long long*)scratch(ctx, "cmp_tot", sizeof(long long) * 2, &st);
+  if (st) return st;
+  hipLaunchKernelGGL(keep_count_kernel, dim3(nb), dim3(CMP_THREADS), 0, ctx->stream, packets->fate, n, blk,
+                     blk + nb);
+  hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(1024), 0, ctx->stream, blk, blk + nb, (int)nb, tot);
+  hipLaunchKernelGGL(scatter_keep_kernel, dim3(nb), dim3(CMP_THREADS), 0, ctx->stream, *packets, *scr, n, blk);
+  SGPU_LAUNCH_CHECK();
+  long long ht[2];
+  SGPU_HIP(hipMemcpyAsync(ht, tot, sizeof(ht), hipMemcpyDeviceToHost, ctx->stream));
+  SGPU_HIP(hipStreamSynchronize(ctx->stream));
+  const int64_t k = ht[0];
+  if (k > 0) {
+    const unsigned b = grid_blocks(k, 256);
+    const sgpu_packets& s = *scr;
+    const sgpu_packets& d = *packets;
+    hipLaunchKernelGGL(copy_kernel<double>, dim3(b), dim3(256), 0, ctx->stream, s.x, d.x, k);
+    hipLaunchKernelGGL(copy_kernel<double>, dim3(b), dim3(256), 0, ctx->stream, s.y, d.y, k);
+    hipLaunchKernelGGL(copy_kernel<double>, dim3(b), dim3(256), 0, ctx->stream, s.z, d.z, k);
+    hipLaunchKernelGGL(copy_kernel<double>, dim3(b), dim3(256), 0, ctx->stream, s.Dx, d.Dx, k);
+    hipLaunchKernelGGL(copy_kernel<double>, dim3(b), dim3(256), 0, ctx->stream, s.Dy, d.Dy, k);
+    hipLaunchKernelGGL(copy_kernel<double>, dim3(b), dim3(256), 0, ctx->stream, s.Dz, d.Dz, k);
+    hipLaunchKernelGGL(copy_kernel<double>, dim3(b), dim3(256), 0, ctx->stream, s.t, d.t, k);
+    hipLaunchKernelGGL(copy_kernel<double>, dim3(b), dim3(256), 0, ctx->stream, s.e, d.e, k);
+    hipLaunchKernelGGL(copy_kernel<double>, dim3(b), dim3(256), 0, ctx->stream, s.nu, d.nu, k);
+    hipLaunchKernelGGL(copy_kernel<int32_t>, dim3(b), dim3(256), 0, ctx->stream, s.ind, d.ind, k);
+    hipLaunchKernelGGL(copy_kernel<int32_t>, dim3(b), dim3(256), 0, ctx->stream, s.type, d.type, k);
+    hipLaunchKernelGGL(copy_kernel<int32_t>, dim3(b), dim3(256), 0, ctx->stream, s.fate, d.fate, k);
+    hipLaunchKernelGGL(copy_kernel<uint32_t>, dim3(b), dim3(256), 0, ctx->stream, s.draw, d.draw, k);
+    hipLaunchKernelGGL(copy_kernel<uint64_t>, dim3(b), dim3(256), 0, ctx->stream, s.id, d.id, k);
+    SGPU_LAUNCH_CHECK();
+  }
+  *n_alive = k;
+  *n_escaped = ht[1];
+  return SGPU_OK;
+}
+
+static int fill(sgpu_ctx* ctx, double* p, int64_t n)
+{
+  if (!p || n <= 0) return SGPU_OK;
+  hipLaunchKernelGGL(fill_kernel, dim3(grid_blocks(n, 256)), dim3(256), 0, ctx->stream, p, n, 0.0);
+  SGPU_LAUNCH_CHECK();
+  return SGPU_OK;
+}
+
+static int64_t spectrum_size(const sgpu_spectrum& s)
+{
+  return (int64_t)s.time_grid.n * s.nu_grid.n * s.mu_grid.n * s.phi_grid.n;
+}
+
+extern "C" int sgpu_wipe_radiation(sgpu_ctx* ctx, int32_t nz, int32_t nn, sgpu_tallies* ta, int32_t wipe_spectra)
+{
+  SGPU_REQUIRE(ctx && ta && nz >= 0 && nn >= 1, "bad argument");
+  int st;
+  double* zs[] = {ta->e_abs, ta->fx_rad, ta->fy_rad, ta->fz_rad, ta->fr_rad, ta->L_radio_dep,
+                  ta->L_radio_emit, ta->e_rad};
+  for (double* p : zs) if ((st = fill(ctx, p, nz))) return st;
+  if ((st = fill(ctx, ta->J_nu, (int64_t)nz * nn))) return st;
+  if (wipe_spectra) {
+    if ((st = fill(ctx, ta->optical.flux, spectrum_size(ta->optical)))) return st;
+    if ((st = fill(ctx, ta->optical.click, spectrum_size(ta->optical)))) return st;
+    if ((st = fill(ctx, ta->gamma.flux, spectrum_size(ta->gamma)))) return st;
+    if ((st = fill(ctx, ta->gamma.click, spectrum_size(ta->gamma)))) return st;
+  }
+  return SGPU_OK;
+}
+
+extern "C" int sgpu_normalize_tallies(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_locate* nu_grid,
+                                      const sgpu_transport_params* params, sgpu_tallies* ta)
+{
+  SGPU_REQUIRE(ctx && nu_grid && params && ta, "null argument");
+  int st = check_grid(grid);
+  if (st) return st;
+  SGPU_REQUIRE(ta->e_abs && ta->J_nu && ta->fr_rad && ta->L_radio_dep && ta->L_radio_emit && ta->e_rad,
+               "tally arrays missing");
+  hipLaunchKernelGGL(normalize_kernel, dim3(grid->n_zones), dim3(256), 0, ctx->stream, *nu_grid, grid->vol,
+                     params->dt, params->store_Jnu, *ta);
+  SGPU_LAUNCH_CHECK();
+  return SGPU_OK;
+}
+
+extern "C" int sgpu_homologous_step(sgpu_ctx* ctx, sgpu_grid* grid, double* rho, double* r_out,
+                                    double* x_out, double* z_out, double* vol, double e)
+{
+  SGPU_REQUIRE(ctx && grid && rho && vol && e > 0, "bad argument");
+  const int nz = grid->n_zones;
+  int nr = 0, nx = 0, nzz = 0;
+  if (grid->kind == SGPU_GRID_1D_SPHERE) {
+    SGPU_REQUIRE(r_out, "1D grid needs r_out");
+    nr = grid->r_out.n;
+    grid->r_out.min *= e;
+    if (grid->r_out.type == SGPU_LOC_LIN) grid->r_out.del *= e;
+  } else {
+    SGPU_REQUIRE(x_out && z_out, "2D grid needs x_out and z_out");
+    nx = grid->x_out.n; nzz = grid->z_out.n;
+    grid->x_out.min *= e;
+    grid->z_out.min *= e;
+    if (grid->x_out.type == SGPU_LOC_LIN) grid->x_out.del *= e;
+    if (grid->z_out.type == SGPU_LOC_LIN) grid->z_out.del *= e;
+  }
+  int m = nz;
+  if (nr > m) m = nr;
+  if (nx > m) m = nx;
+  if (nzz > m) m = nzz;
+  hipLaunchKernelGGL(homologous_kernel, dim3(grid_blocks(m, 256)), dim3(256), 0, ctx->stream, grid->kind, nz,
+                     rho, r_out, nr, x_out, nx, z_out, nzz, vol, 0.0, e);
+  SGPU_LAUNCH_CHECK();
+  if (grid->kind == SGPU_GRID_1D_SPHERE) {
+    hipLaunchKernelGGL(shell_volume_kernel, dim3(grid_blocks(nz, 256)), dim3(256), 0, ctx->stream, nz, r_out,
+                       grid->r_out.min, vol);
+    SGPU_LAUNCH_CHECK();
+  }
+  return SGPU_OK;
+}
