@@ -107,6 +107,9 @@ int      or_normalize_tallies(const sgpu_grid* grid, const sgpu_locate* nu_grid,
 int64_t  or_compact(sgpu_packets* pk, int64_t n, int64_t* n_escaped);
 void     or_homologous_step(sgpu_grid* grid, double* rho, double* r_out, double* x_out,
                             double* z_out, double* vol, double e);
+/* test hooks on single packets */
+void     or_lorentz(const sgpu_grid* grid, double* x, double* D, double* e, double* nu, int tolab);
+void     or_compton_sample(double nu, int64_t n, uint64_t seed, double* cost_out, double* eratio_out);
 
 /* ---- NLTE (src/opacity/AtomicSpecies_state.cpp:131-196) ---- */
 int      or_nlte_solve(int n, const double* rates, const double* lte, double* b, double* pop);

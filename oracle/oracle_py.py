@@ -79,6 +79,8 @@ def load():
         lib.or_decay.restype = D
         lib.or_decay.argtypes = [I, V, V, V, D, P(D), I]
         lib.or_decay_composition.argtypes = [I, V, V, V, D]
+        lib.or_lorentz.argtypes = [P(abi.Grid), V, V, P(D), P(D), I]
+        lib.or_compton_sample.argtypes = [D, I64, ctypes.c_uint64, V, V]
         _lib = lib
     return _lib
 

@@ -636,3 +636,47 @@ void or_homologous_step(sgpu_grid* g, double* rho, double* r_out, double* x_out,
     for (int i = 0; i < g->n_zones; i++) vol[i] = vol[i] * e * e * e;
   }
 }
+
+/* test hooks: the static per-event functions above, callable on one packet */
+void or_lorentz(const sgpu_grid* grid, double* x, double* D, double* e, double* nu, int tolab)
+{
+  tctx_t c = {grid, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  pkt_t p;
+  memset(&p, 0, sizeof(p));
+  for (int k = 0; k < 3; k++) { p.x[k] = x[k]; p.D[k] = D[k]; }
+  p.e = *e; p.nu = *nu;
+  lorentz_transform(&c, &p, tolab);
+  for (int k = 0; k < 3; k++) D[k] = p.D[k];
+  *e = p.e; *nu = p.nu;
+}
+
+/* Klein-Nishina rejection sampling of compton_scatter (scatter.cpp:82-103)
+ * in the comoving frame for a photon moving along +z with energy nu (MeV):
+ * returns n samples of cos(theta) and E_ratio. */
+void or_compton_sample(double nu, int64_t n, uint64_t seed, double* cost_out, double* eratio_out)
+{
+  sgpu_transport_params prm;
+  memset(&prm, 0, sizeof(prm));
+  prm.seed = seed;
+  tctx_t c = {0, 0, 0, &prm, 0, 0, 0, 0, 0, 0};
+  for (int64_t i = 0; i < n; i++) {
+    pkt_t p;
+    memset(&p, 0, sizeof(p));
+    p.id = (uint64_t)i;
+    p.D[2] = 1;
+    p.nu = nu;
+    double E_ratio, cost;
+    while (1) {
+      double mu = 1 - 2.0 * urand(&c, &p);
+      double phi = 2.0 * OR_PI * urand(&c, &p);
+      double smu = sqrt(1 - mu * mu);
+      double Dn[3] = {smu * cos(phi), smu * sin(phi), mu};
+      cost = p.D[0] * Dn[0] + p.D[1] * Dn[1] + p.D[2] * Dn[2];
+      E_ratio = 1 / (1 + p.nu / OR_M_E_MEV * (1 - cost));
+      double diff_cs = 0.5 * (E_ratio * E_ratio * (1 / E_ratio + E_ratio - 1 + cost * cost));
+      if (urand(&c, &p) < diff_cs) break;
+    }
+    cost_out[i] = cost;
+    eratio_out[i] = E_ratio;
+  }
+}

@@ -179,6 +179,22 @@ __global__ void __launch_bounds__(NLTE_THREADS) nlte_single_kernel(int n, const 
 // ------------------------------------------------------------------
 // batched: device queue of systems, largest first
 // ------------------------------------------------------------------
+// smallest and largest system size (single workgroup): mm[0] = min, mm[1] = max
+__global__ void __launch_bounds__(1024) size_range_kernel(const int32_t* n, int nsys, int32_t* mm)
+{
+  __shared__ int lo[1024 / WAVE], hi[1024 / WAVE];
+  int a = 1 << 30, b = -(1 << 30);
+  for (int s = threadIdx.x; s < nsys; s += blockDim.x) { a = min(a, n[s]); b = max(b, n[s]); }
+  for (int o = 32; o > 0; o >>= 1) { a = min(a, __shfl_xor(a, o, WAVE)); b = max(b, __shfl_xor(b, o, WAVE)); }
+  if (threadIdx.x % WAVE == 0) { lo[threadIdx.x / WAVE] = a; hi[threadIdx.x / WAVE] = b; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x / WAVE); w++) { a = min(a, lo[w]); b = max(b, hi[w]); }
+    a = min(a, lo[0]); b = max(b, hi[0]);
+    mm[0] = a; mm[1] = b;
+  }
+}
+
 // counting sort of system indices by size, descending (single workgroup)
 __global__ void __launch_bounds__(1024) order_kernel(const int32_t* n, int nsys, int32_t* order, int32_t* counter)
 {
@@ -539,7 +555,14 @@ int batched_impl(sgpu_ctx* ctx, int32_t nsys, const int32_t* n, const int64_t* r
   int32_t* order = (int32_t*)scratch(ctx, "nlte_order", sizeof(int32_t) * ((size_t)nsys + 64), &st);
   if (st) return st;
   int32_t* counter = order + nsys;
-  const int max_n = 1024;
+  int32_t* mm = order + nsys + 8;
+  hipLaunchKernelGGL(size_range_kernel, dim3(1), dim3(1024), 0, ctx->stream, n, nsys, mm);
+  SGPU_LAUNCH_CHECK();
+  int32_t hmm[2] = {0, 0};
+  SGPU_HIP(hipMemcpyAsync(hmm, mm, sizeof(hmm), hipMemcpyDeviceToHost, ctx->stream));
+  SGPU_HIP(hipStreamSynchronize(ctx->stream));
+  SGPU_REQUIRE(hmm[0] >= 1 && hmm[1] <= 1024, "system sizes must lie in [1, 1024] (got %d..%d)", hmm[0], hmm[1]);
+  const int max_n = hmm[1];
   const unsigned nwg = (unsigned)std::min<int64_t>((int64_t)ctx->n_cu * wg_per_cu, nsys);
   // slot: 7 vectors + perm + an FP64 matrix for the fallback / FP64 mode
   const size_t slot = (size_t)max_n * max_n + 8 * (size_t)max_n + 64;

@@ -125,7 +125,7 @@ __device__ __forceinline__ void sample_photon_frequency(const PropArgs<R>& A, Pk
 {
   if (p.type == SGPU_PHOTON) {
     const int nn = A.nu.n;
-    int inu = cdf_sample(A.tab.cdf + (size_t)p.ind * nn, nn, p.rng.template u<R>());
+    int inu = min(cdf_sample(A.tab.cdf + (size_t)p.ind * nn, nn, p.rng.template u<R>()), nn - 1);
     p.nu = ax_sample(A.nu, inu, p.rng.template u<R>());
     c.redist++;
   } else {
@@ -533,7 +533,7 @@ __global__ void __launch_bounds__(256) emit_kernel(EmitArgs A)
     const double E_p = L_tot * A.dt / (1.0 * A.n_emit);
     Pkt<double> p;
     p.rng = make_rng(A.seed, A.id_base + (uint64_t)q, 0);
-    int i = cdf_sample(A.zcdf, nz, p.rng.uniform());
+    int i = min(cdf_sample(A.zcdf, nz, p.rng.uniform()), nz - 1);
     double t = A.t_now + A.dt * p.rng.uniform();
     if (p.rng.uniform() < A.gfrac[i]) p.type = SGPU_GAMMARAY;
     else {
@@ -548,7 +548,7 @@ __global__ void __launch_bounds__(256) emit_kernel(EmitArgs A)
     isotropic_dir(p, p.D);
     if (p.type == SGPU_PHOTON) {
       const int nn = A.nu.n;
-      int inu = cdf_sample(A.emis_cdf + (size_t)i * nn, nn, p.rng.uniform());
+      int inu = min(cdf_sample(A.emis_cdf + (size_t)i * nn, nn, p.rng.uniform()), nn - 1);
       p.nu = ax_sample(A.nu, inu, p.rng.uniform());
     } else {
       p.nu = 1;
@@ -892,7 +892,7 @@ extern "C" int sgpu_emit_radioactive(sgpu_ctx* ctx, const sgpu_grid* grid, const
     A.emis_cdf = tables->emis_cdf; A.zcdf = zcdf; A.gfrac = gfr; A.L_tot = ltot;
     A.t_now = params->t_now; A.dt = params->dt; A.seed = params->seed; A.id_base = id_base;
     A.n_emit = n_emit; A.offset = offset; A.pk = *packets; A.L_dep = tallies->L_radio_dep;
-    A.lds_dep = nz <= 8192;
+    A.lds_dep = nz <= 4096;
     hipLaunchKernelGGL(emit_kernel, dim3(grid_blocks(n_emit, 256)), dim3(256),
                        A.lds_dep ? sizeof(double) * nz : 0, ctx->stream, A);
     SGPU_LAUNCH_CHECK();
