@@ -1,0 +1,255 @@
+"""Benchmark of the implicit Monte Carlo transport hot path on one MI355X.
+
+Workload (the line-expansion configuration): 1D homologous ejecta, 128
+zones to v_max = 2e9 cm/s at t = 20 d, rho ~ exp(-v/3e8) normalised to
+1.4 Msun with 56Ni inside 0.6 Msun; 1e5 synthetic fuzz lines, LTE
+Saha-Boltzmann for 4 ion stages per element; 4096 log-frequency bins; 1e7
+radioactive packets per step, steady state.  A step is one pass of the hot
+path with inputs resident in HBM: opacity / emissivity build, emission,
+propagation to escape, tally normalisation.
+
+Reported: packets/s of the batched mixed-precision form (FP32 propagation,
+FP64 tallies), beside the plain FP64 drop-in at equal packet count, the NLTE
+zone-solves/s of the batched FP32+refinement solve versus the per-system
+FP64 drop-in, the roofline of the propagation kernel and the CPU oracle timed
+on this host's cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--packets N]
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--packets", type=int, default=10_000_000)
+    ap.add_argument("--config", default="c2_line")
+    ap.add_argument("--plain-packets", type=int, default=0,
+                    help="packets of the plain FP64 comparison step (0: same as --packets)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-nlte", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def packet_bytes(c, nn, precision):
+    """Algorithmic HBM bytes per packet of the propagation kernel:
+    2 S_state + n_step (w_kappa + 8) + n_redist w_cdf ceil(log2 n_nu) + f_esc 16,
+    S_state = 96 B (9 FP64 + 3 int32 + uint32 + uint64 per packet), w_kappa the
+    opacity gather per event (16 B FP64 abs+scat, 8 B packed FP32 kappa,eps),
+    8 B the J_nu accumulate per event, w_cdf 8 (FP64) / 4 (FP32) bytes per
+    binary-search probe, 16 B of spectrum per escape."""
+    n = max(c["n_packets"], 1)
+    n_step = c["n_events"] / n
+    n_redist = c["n_redist"] / n
+    f_esc = c["n_escaped"] / n
+    wk, wc = (16, 8) if precision == 0 else (8, 4)
+    return 2 * 96 + n_step * (wk + 8) + n_redist * wc * math.ceil(math.log2(nn)) + f_esc * 16
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X and has no "
+              "multi-GPU path (--gpus %d requested)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from pubsed_amd import runtime, abi, workloads as W
+    from pubsed_amd.runtime import GpuBackend
+
+    dev = "cuda:0"
+    gpu = GpuBackend(0)
+    gpu.use_torch_stream()
+    gpu.enable_timing(True)
+    N = a.packets
+    P = runtime.Problem(runtime.Space(dev), a.config)
+    P.alloc_packets(N)
+    cnt = torch.zeros(10, dtype=torch.int64, device=dev)
+
+    def step(precision, n, counters=None):
+        P.tparams.precision = precision
+        gpu.wipe(P)
+        gpu.opacity_build(P)
+        gpu.emit(P, n)
+        gpu.propagate(P, n, counters)
+        gpu.normalize(P)
+
+    def timed(precision, n, steps, warm):
+        for _ in range(warm):
+            step(precision, n)
+        torch.cuda.synchronize()
+        cnt.zero_()
+        prop_ms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(precision, n, cnt)
+            prop_ms.append(gpu.last_ms("propagate"))
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        c = dict(zip(abi.COUNTER_FIELDS, cnt.cpu().tolist()))
+        return dt, prop_ms, c
+
+    print("# bench: config %s, %d packets/step, mixed precision" % (a.config, N), file=sys.stderr, flush=True)
+    dt, prop_ms, c = timed(abi.PREC_MIXED, N, a.steps, a.warmup)
+    value = a.steps * N / dt
+    ms_step = 1e3 * dt / a.steps
+    prop_avg = float(np.mean(prop_ms))
+    bpk = packet_bytes(c, P.n_nu, 1)
+    achieved = bpk * N / (prop_avg * 1e-3) / 1e9
+    per_pkt = {k: c[k] / max(c["n_packets"], 1) for k in ("n_events", "n_binshift", "n_redist",
+                                                             "n_escaped", "n_scatter")}
+    print("# mixed: %.3f s/step, propagate %.1f ms, events/packet %.1f" %
+          (dt / a.steps, prop_avg, per_pkt["n_events"]), file=sys.stderr, flush=True)
+
+    # plain FP64 drop-in at equal packet count
+    Np = a.plain_packets or N
+    print("# plain FP64: %d packets" % Np, file=sys.stderr, flush=True)
+    dtp, prop_p, cp = timed(abi.PREC_FP64, Np, 1, 0)
+    plain = Np / dtp
+    esc_m = c["n_escaped"] / max(c["n_packets"], 1)
+    esc_p = cp["n_escaped"] / max(cp["n_packets"], 1)
+
+    nlte = None
+    if not a.no_nlte:
+        nlte = bench_nlte(gpu, torch, W, abi, np)
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = bench_cpu(a.config, a.cpu_seconds)
+
+    out = {
+        "metric": "packets/s & NLTE solves/s: batched/mixed vs plain GPU path at matched MC noise",
+        "value": value,
+        "unit": "packets/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 geometry / f64 tallies",
+        "data": "synthetic (seeded ejecta, line list and rates)",
+        "config": {"workload": "line-expansion 1D ejecta: 128 zones, 1e5 fuzz lines, 4 ion stages, "
+                               "4096 log bins, steady state",
+                   "packets_per_step": N, "zones": P.nz, "nu_bins": P.n_nu},
+        "plain_fp64_packets_per_s": plain,
+        "mixed_over_plain": value / plain,
+        "escape_fraction": {"mixed": esc_m, "plain": esc_p},
+        "per_packet": per_pkt,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "propagate_kernel<float,true>", "kernel_ms": prop_avg,
+                     "bytes_per_packet": bpk},
+        "nlte": nlte,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def bench_nlte(gpu, torch, W, abi, np):
+    """Batched NLTE (256 zones x 4 ions = 1024 systems, n ~ U{20..200}):
+    FP32 LU + FP64 refinement in one persistent launch vs the per-system
+    FP64 drop-in (one sgpu_nlte_solve per system)."""
+    sy = W.nlte_systems(n_sys=1024, seed=20261203)
+    d = {k: torch.from_numpy(np.ascontiguousarray(sy[k])).cuda() for k in
+         ("n", "rate_off", "vec_off", "rates", "lte")}
+    nv = len(sy["lte"])
+    b = torch.zeros(nv, dtype=torch.float64, device="cuda:0")
+    b64 = torch.zeros(nv, dtype=torch.float64, device="cuda:0")
+    it = torch.zeros(sy["n_sys"], dtype=torch.int32, device="cuda:0")
+    rep = gpu.nlte_autotune(sy["n_sys"], d["n"], d["rate_off"], d["vec_off"], d["rates"], d["lte"], b,
+                            None, it)
+    reps = 5
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        gpu.nlte_batched(sy["n_sys"], d["n"], d["rate_off"], d["vec_off"], d["rates"], d["lte"], b, None,
+                         abi.NLTE_FP32_REFINE, 1e-13, it)
+    torch.cuda.synchronize()
+    t_mixed = (time.perf_counter() - t0) / reps
+    k_ms = gpu.last_ms("nlte")
+    # batched FP64 (same launch, FP64 LU in device memory)
+    t0 = time.perf_counter()
+    gpu.nlte_batched(sy["n_sys"], d["n"], d["rate_off"], d["vec_off"], d["rates"], d["lte"], b64, None,
+                     abi.NLTE_FP64, 0.0, None)
+    torch.cuda.synchronize()
+    t_b64 = time.perf_counter() - t0
+    # plain drop-in: one call per system
+    work = torch.zeros(200 * 200 + 200, dtype=torch.float64, device="cuda:0")
+    bp = torch.zeros(nv, dtype=torch.float64, device="cuda:0")
+    ro, vo, ns = sy["rate_off"], sy["vec_off"], sy["n"]
+    t0 = time.perf_counter()
+    for s in range(sy["n_sys"]):
+        gpu.nlte_solve(int(ns[s]), d["rates"][ro[s]:], d["lte"][vo[s]:], bp[vo[s]:], None, work)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    x, r = b.cpu().numpy(), bp.cpu().numpy()
+    err = max(np.max(np.abs(x[vo[s]:vo[s + 1]] - r[vo[s]:vo[s + 1]])) / np.max(np.abs(r[vo[s]:vo[s + 1]]))
+              for s in range(sy["n_sys"]))
+    n2 = (ns.astype(np.float64))
+    flops = float(np.sum(2.0 / 3.0 * n2 ** 3))
+    itv = it.cpu().numpy()
+    return {"systems": int(sy["n_sys"]), "n_range": [int(ns.min()), int(ns.max())],
+            "batched_mixed_solves_per_s": sy["n_sys"] / t_mixed,
+            "batched_fp64_solves_per_s": sy["n_sys"] / t_b64,
+            "plain_fp64_solves_per_s": sy["n_sys"] / t_plain,
+            "mixed_over_plain": t_plain / t_mixed,
+            "max_rel_err_vs_fp64": float(err),
+            "refine_iters": {"min": int(itv.min()), "max": int(itv.max()),
+                             "fp64_fallbacks": int((itv < 0).sum())},
+            "kernel_ms": k_ms,
+            "lu_tflops": flops / (k_ms * 1e-3) / 1e12 if k_ms > 0 else None,
+            "autotune": rep.strip().splitlines()[-1] if rep else None}
+
+
+def bench_cpu(config, seconds):
+    """The CPU oracle (FP64 restatement with the reference loop structure:
+    OpenMP over packets, atomic tallies) on this host's cores, on a bounded
+    packet sample scaled to about `seconds` of work."""
+    from pubsed_amd import runtime
+    from oracle.oracle_py import OracleBackend
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    ob = OracleBackend(n_threads=cores)
+    P = runtime.Problem(runtime.Space(None), config)
+    ob.opacity_build(P)
+    n = 256
+    rate = None
+    while True:
+        P.alloc_packets(n)
+        ob.wipe(P)
+        t0 = time.perf_counter()
+        ob.opacity_build(P)
+        ob.emit(P, n)
+        ob.propagate(P, n)
+        ob.normalize(P)
+        dt = time.perf_counter() - t0
+        rate = n / dt
+        if dt > seconds / 4 or n >= 1 << 22:
+            break
+        n = int(min(n * max(2.0, seconds / max(dt, 1e-3) / 2), 1 << 22))
+    return {"value": rate, "unit": "packets/s", "cores": cores, "kind": "port",
+            "sample": "%d packets of the same workload, one full step (opacity, emission, "
+                      "propagation, normalisation), FP64 oracle, OpenMP over packets" % n}
+
+
+if __name__ == "__main__":
+    main()

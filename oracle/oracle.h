@@ -123,6 +123,12 @@ int      or_adjoint_forward(const sgpu_grid* grid, const sgpu_locate* nu_grid,
                             const sgpu_transport_params* p, const sgpu_packets* pk,
                             int64_t n, double* F, int32_t* rec_bin, double* rec_w,
                             double* dF_dkappa_dir, const double* dir);
+int      or_adjoint_backward(const sgpu_grid* grid, const sgpu_locate* nu_grid,
+                             const sgpu_locate* spec_grid, const float* kappa_abs,
+                             const double* rho, const double* scat,
+                             const sgpu_transport_params* p, const sgpu_packets* pk,
+                             int64_t n, const double* dL_dF, const int32_t* rec_bin,
+                             const double* rec_w, double* grad);
 
 #ifdef __cplusplus
 }

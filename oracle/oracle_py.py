@@ -81,6 +81,12 @@ def load():
         lib.or_decay_composition.argtypes = [I, V, V, V, D]
         lib.or_lorentz.argtypes = [P(abi.Grid), V, V, P(D), P(D), I]
         lib.or_compton_sample.argtypes = [D, I64, ctypes.c_uint64, V, V]
+        lib.or_adjoint_forward.restype = I
+        lib.or_adjoint_forward.argtypes = [P(abi.Grid), P(abi.Locate), P(abi.Locate), V, V, V,
+                                           P(abi.TransportParams), P(abi.Packets), I64, V, V, V, V, V]
+        lib.or_adjoint_backward.restype = I
+        lib.or_adjoint_backward.argtypes = [P(abi.Grid), P(abi.Locate), P(abi.Locate), V, V, V,
+                                            P(abi.TransportParams), P(abi.Packets), I64, V, V, V, V]
         _lib = lib
     return _lib
 
