@@ -59,7 +59,26 @@ def build(jobs=8, verbose=True):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_host(verbose)
     return LIB
+
+
+HOST = os.path.join(HERE, "host")
+DRIVER = os.path.join(HOST, "drive_grey")
+
+
+def build_host(verbose=True):
+    """The host C++ drop-in (transport class) and its example driver, linked
+    against the HIP library with an rpath to the package directory."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-o", DRIVER + ".tmp",
+           os.path.join(HOST, "drive_grey.cpp"), os.path.join(HOST, "sedona_transport.cpp"),
+           "-L" + HERE, "-lsedona_gpu", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host build failed:\n%s\n%s" % (r.stdout, r.stderr))
+    os.replace(DRIVER + ".tmp", DRIVER)
+    if verbose:
+        print("built", DRIVER)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,255 @@
+// sedona_transport.cpp -- host C++ drop-in for Sedona's transport step (see
+// sedona_transport.hpp).  Only C-ABI calls; no HIP or torch types.
+#include "sedona_transport.hpp"
+
+#include <cmath>
+#include <cstring>
+
+namespace sedona_gpu {
+
+namespace {
+constexpr double PI_REF = 3.14159;   // src/utils/physical_constants.h:9
+
+// locate_array::log_init with a spacing nudged until it yields exactly n bins
+std::vector<double> log_edges(double lo, double hi, int n, double* del_out)
+{
+  double d0 = std::pow(hi / lo, 1.0 / n) - 1.0;
+  for (int k = 1; k <= 40; ++k) {
+    double d = d0 * (1.0 + 1e-12 * k);
+    std::vector<double> x;
+    double v = lo + lo * d;
+    while (v < hi && (int)x.size() <= n) { x.push_back(v); v = v + v * d; }
+    x.push_back(hi);
+    if ((int)x.size() == n) { *del_out = d; return x; }
+  }
+  *del_out = 0;
+  return {};
+}
+}  // namespace
+
+transport::~transport()
+{
+  for (void* p : owned_) sgpu_free(ctx_, p);
+  if (ctx_) sgpu_destroy(ctx_);
+}
+
+int transport::fail(int rc, const char* what)
+{
+  char buf[1024];
+  sgpu_last_error(buf, sizeof(buf));
+  err_ = std::string(what) + ": " + buf;
+  return rc;
+}
+
+void* transport::dalloc(size_t bytes)
+{
+  void* p = nullptr;
+  if (sgpu_alloc(ctx_, &p, bytes) != SGPU_OK) return nullptr;
+  owned_.push_back(p);
+  return p;
+}
+
+template <class T>
+T* transport::upload(const std::vector<T>& v)
+{
+  T* d = (T*)dalloc(sizeof(T) * (v.empty() ? 1 : v.size()));
+  if (d && !v.empty()) sgpu_memcpy_h2d(ctx_, d, v.data(), sizeof(T) * v.size());
+  return d;
+}
+
+std::vector<double> transport::download(const double* d, size_t n) const
+{
+  std::vector<double> h(n);
+  if (n) sgpu_memcpy_d2h(ctx_, h.data(), d, sizeof(double) * n);
+  return h;
+}
+
+int transport::init(const Params& p, const Grid1D& g, const AtomicData* atoms, int device)
+{
+  p_ = p;
+  ctx_ = sgpu_create(device);
+  if (!ctx_) return fail(SGPU_ERR_HIP, "sgpu_create");
+  nz_ = g.n_zones();
+  ne_ = (int)g.elems_Z.size();
+  t_now_ = g.t_now;
+  double del = 0;
+  nu_edges_ = log_edges(p.transport_nu_grid[0], p.transport_nu_grid[1], p.n_nu, &del);
+  if ((int)nu_edges_.size() != p.n_nu) return fail(SGPU_ERR_ARG, "frequency grid");
+  nn_ = p.n_nu;
+  nu_ = {nn_, SGPU_LOC_LOG, p.transport_nu_grid[0], del, upload(nu_edges_)};
+
+  // grid: shell volumes 4/3 pi (r^3 - r0^3) with the reference's pi
+  std::vector<double> vol(nz_);
+  for (int i = 0; i < nz_; ++i) {
+    double r0 = i == 0 ? g.r_min : g.r_out[i - 1];
+    vol[i] = 4.0 * PI_REF / 3.0 * (g.r_out[i] * g.r_out[i] * g.r_out[i] - r0 * r0 * r0);
+  }
+  d_r_out_ = upload(g.r_out);
+  d_vol_ = upload(vol);
+  grid_ = {};
+  grid_.kind = SGPU_GRID_1D_SPHERE;
+  grid_.n_zones = nz_;
+  grid_.t_now = g.t_now;
+  grid_.r_out = {nz_, SGPU_LOC_FLEX, g.r_min, 0.0, d_r_out_};
+  grid_.vol = d_vol_;
+
+  d_rho_ = upload(g.rho);
+  std::vector<double> grey(nz_, p.opacity_grey_opacity);
+  zs_ = {nz_, ne_, d_rho_, upload(g.T_gas), upload(g.X), nullptr, upload(grey),
+         upload(g.elems_Z), upload(g.elems_A)};
+  have_atoms_ = atoms != nullptr;
+  if (have_atoms_) {
+    at_.n_atoms = (int)atoms->lev_off.size() - 1;
+    at_.max_levels = atoms->max_levels;
+    at_.n_levels = atoms->lev_off.back();
+    at_.n_ions = atoms->ion_off.back();
+    at_.lev_off = upload(atoms->lev_off);
+    at_.ion_off = upload(atoms->ion_off);
+    at_.lev_E = upload(atoms->lev_E);
+    at_.lev_g = upload(atoms->lev_g);
+    at_.lev_ion = upload(atoms->lev_ion);
+    at_.ion_chi = upload(atoms->ion_chi);
+    at_.atom_zero_eps = upload(atoms->atom_zero_eps);
+    at_.n_lines = (int64_t)atoms->line_nu.size();
+    at_.line_nu = upload(atoms->line_nu);
+    at_.line_gf = upload(atoms->line_gf);
+    at_.line_El = upload(atoms->line_El);
+    at_.line_ion = upload(atoms->line_ion);
+    at_.line_bin = upload(atoms->line_bin);
+    at_.line_bin_off = upload(atoms->line_bin_off);
+  }
+  const size_t T = (size_t)nz_ * nn_;
+  auto z = [&](size_t n) { return (double*)dalloc(sizeof(double) * n); };
+  out_ = {};
+  out_.abs = z(T); out_.scat = z(T); out_.emis_cdf = z(T);
+  out_.L_thermal = z(nz_); out_.planck_mean = z(nz_); out_.rosseland_mean = z(nz_);
+  out_.n_elec = z(nz_); out_.compton_opac = z(nz_); out_.photoion_opac = z(nz_); out_.eps_imc = z(nz_);
+  out_.zone_status = (int32_t*)dalloc(sizeof(int32_t) * nz_);
+  tab_ = {out_.abs, out_.scat, out_.emis_cdf, out_.eps_imc, out_.compton_opac, out_.photoion_opac};
+
+  // spectrum grids: spectrum_time_grid {0,1,1}, the transport nu grid, one mu / phi bin
+  static const double one[1] = {1.0};
+  static const double two_pi[1] = {2 * PI_REF};
+  double* t_x = upload(std::vector<double>(one, one + 1));
+  double* mu_x = upload(std::vector<double>(one, one + 1));
+  double* ph_x = upload(std::vector<double>(two_pi, two_pi + 1));
+  sgpu_locate tg = {1, SGPU_LOC_LIN, 0.0, 1.0, t_x};
+  sgpu_locate mg = {1, SGPU_LOC_LIN, -1.0, 2.0, mu_x};
+  sgpu_locate pg = {1, SGPU_LOC_LIN, 0.0, 2 * PI_REF, ph_x};
+  ta_ = {};
+  ta_.e_abs = z(nz_); ta_.J_nu = z(T); ta_.fx_rad = z(nz_); ta_.fy_rad = z(nz_); ta_.fz_rad = z(nz_);
+  ta_.fr_rad = z(nz_); ta_.L_radio_dep = z(nz_); ta_.L_radio_emit = z(nz_); ta_.e_rad = z(nz_);
+  ta_.optical = {tg, nu_, mg, pg, z(nn_), z(nn_)};
+  ta_.gamma = {tg, nu_, mg, pg, z(nn_), z(nn_)};
+
+  const int64_t cap = p.particles_max_total;
+  for (sgpu_packets* q : {&pk_, &scr_}) {
+    q->capacity = cap;
+    q->x = z(cap); q->y = z(cap); q->z = z(cap);
+    q->Dx = z(cap); q->Dy = z(cap); q->Dz = z(cap);
+    q->t = z(cap); q->e = z(cap); q->nu = z(cap);
+    q->ind = (int32_t*)dalloc(sizeof(int32_t) * cap);
+    q->type = (int32_t*)dalloc(sizeof(int32_t) * cap);
+    q->fate = (int32_t*)dalloc(sizeof(int32_t) * cap);
+    q->draw = (uint32_t*)dalloc(sizeof(uint32_t) * cap);
+    q->id = (uint64_t*)dalloc(sizeof(uint64_t) * cap);
+    if (!q->id) return fail(SGPU_ERR_HIP, "packet buffers");
+  }
+  if (sgpu_wipe_radiation(ctx_, nz_, nn_, &ta_, 1) != SGPU_OK) return fail(SGPU_ERR_HIP, "wipe");
+  return SGPU_OK;
+}
+
+int transport::set_opacity(double dt)
+{
+  op_ = {};
+  op_.t_now = t_now_;
+  op_.dt = dt;
+  op_.epsilon = p_.opacity_epsilon;
+  op_.temp_min = p_.limits_temp_min;
+  op_.temp_max = p_.limits_temp_max;
+  op_.maximum_opacity = p_.opacity_maximum_opacity;
+  op_.fleck_alpha = p_.transport_fleck_alpha;
+  op_.use_electron_scattering = p_.opacity_electron_scattering;
+  op_.use_fuzz_expansion = p_.opacity_fuzz_expansion;
+  op_.radiative_eq = p_.transport_radiative_equilibrium;
+  op_.first_step = first_step_;
+  int rc = sgpu_opacity_build(ctx_, &nu_, &zs_, have_atoms_ ? &at_ : nullptr, &op_, &out_);
+  if (rc < 0) return fail(rc, "set_opacity");
+  return SGPU_OK;   // positive codes are the reference's solve warnings
+}
+
+int transport::wipe_radiation()
+{
+  int rc = sgpu_wipe_radiation(ctx_, nz_, nn_, &ta_, p_.transport_steady_iterate > 0);
+  return rc < 0 ? fail(rc, "wipe_radiation") : SGPU_OK;
+}
+
+int transport::emit_particles(double dt)
+{
+  int64_t n = p_.particles_n_emit_radioactive;
+  if (n_particles_ + n > p_.particles_max_total) n = 0;   // "Out of particle space"
+  int64_t got = 0;
+  int rc = sgpu_emit_radioactive(ctx_, &grid_, &nu_, &zs_, &tab_, &tp_, n, 0, step_index_ << 32, &pk_,
+                                 n_particles_, &ta_, &got);
+  if (rc < 0) return fail(rc, "emit_particles");
+  n_particles_ += got;
+  return SGPU_OK;
+}
+
+int transport::propagate_all(double dt)
+{
+  int rc = sgpu_propagate(ctx_, &grid_, &nu_, &tab_, &tp_, &pk_, n_particles_, &ta_, nullptr);
+  return rc < 0 ? fail(rc, "propagate") : SGPU_OK;
+}
+
+int transport::clean_up_particle_vector()
+{
+  int64_t alive = 0, esc = 0;
+  int rc = sgpu_compact(ctx_, &pk_, n_particles_, &scr_, &alive, &esc);
+  if (rc < 0) return fail(rc, "clean_up_particle_vector");
+  n_particles_ = alive;
+  n_escaped_ = esc;
+  return SGPU_OK;
+}
+
+int transport::reduce_radiation(double dt)
+{
+  int rc = sgpu_normalize_tallies(ctx_, &grid_, &nu_, &tp_, &ta_);
+  return rc < 0 ? fail(rc, "reduce_radiation") : SGPU_OK;
+}
+
+int transport::step(double dt)
+{
+  if (p_.transport_steady_iterate > 0) dt = 1;
+  tp_ = {};
+  tp_.t_now = t_now_;
+  tp_.dt = dt;
+  tp_.seed = p_.transport_rng_seed;
+  tp_.steady_state = p_.transport_steady_iterate > 0;
+  tp_.radiative_eq = p_.transport_radiative_equilibrium;
+  tp_.boundary_in_reflect = p_.transport_boundary_in_reflect;
+  tp_.boundary_out_reflect = p_.transport_boundary_out_reflect;
+  tp_.store_Jnu = p_.transport_store_Jnu;
+  tp_.precision = p_.precision;
+  int rc;
+  if ((rc = set_opacity(dt))) return rc;
+  if ((rc = wipe_radiation())) return rc;
+  if ((rc = emit_particles(dt))) return rc;
+  if ((rc = propagate_all(dt))) return rc;
+  if ((rc = clean_up_particle_vector())) return rc;
+  if ((rc = reduce_radiation(dt))) return rc;
+  if (sgpu_synchronize(ctx_) != SGPU_OK) return fail(SGPU_ERR_HIP, "synchronize");
+  if (!tp_.steady_state) t_now_ += dt;
+  first_step_ = 0;
+  ++step_index_;
+  return SGPU_OK;
+}
+
+std::vector<double> transport::J_nu() const { return download(ta_.J_nu, (size_t)nz_ * nn_); }
+std::vector<double> transport::e_rad() const { return download(ta_.e_rad, nz_); }
+std::vector<double> transport::e_abs() const { return download(ta_.e_abs, nz_); }
+std::vector<double> transport::L_radio_dep() const { return download(ta_.L_radio_dep, nz_); }
+std::vector<double> transport::optical_spectrum() const { return download(ta_.optical.flux, nn_); }
+std::vector<double> transport::abs_opacity() const { return download(out_.abs, (size_t)nz_ * nn_); }
+
+}  // namespace sedona_gpu

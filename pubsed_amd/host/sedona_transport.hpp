@@ -1,0 +1,122 @@
+// sedona_transport.hpp -- host C++ drop-in for Sedona's transport step on
+// one MI355X.  The class keeps the reference's member names and step order
+// (transport::step, src/transport/transport.cpp:26-140): set_opacity,
+// wipe_radiation, emit_particles, the propagation loop, clean_up of escaped
+// and absorbed packets, reduce_radiation; each is one C-ABI call
+// (include/sedona_gpu.h) over all zones or all packets.  Device buffers are
+// owned by the object; the caller describes the grid and gas on the host.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/sedona_gpu.h"
+
+namespace sedona_gpu {
+
+// parameters, named after sedona_defaults.lua
+struct Params {
+  double transport_nu_grid[3] = {1e14, 1e16, 0};   // {nu_1, nu_2, del} log grid
+  int n_nu = 1024;                                  // bins (del chosen to give exactly n_nu)
+  int transport_steady_iterate = 1;
+  int transport_radiative_equilibrium = 0;
+  int transport_store_Jnu = 1;
+  int transport_boundary_in_reflect = 0;
+  int transport_boundary_out_reflect = 0;
+  double transport_fleck_alpha = 0;
+  double opacity_grey_opacity = 0;
+  double opacity_epsilon = 1.0;
+  int opacity_electron_scattering = 0;
+  int opacity_fuzz_expansion = 0;
+  double opacity_maximum_opacity = 1e40;
+  double limits_temp_min = 1000;
+  double limits_temp_max = 1e8;
+  int64_t particles_n_emit_radioactive = 0;
+  int64_t particles_max_total = 10000000;
+  uint64_t transport_rng_seed = 20261203;
+  int precision = SGPU_PREC_MIXED;
+};
+
+// 1D spherical homologous grid, host side (grid_1D_sphere's fields)
+struct Grid1D {
+  std::vector<double> r_out;      // shell outer radii
+  double r_min = 0;
+  double t_now = 0;               // grid clock (velocity = r / t_now)
+  std::vector<double> rho, T_gas;
+  std::vector<double> X;          // [nz][n_elems]
+  std::vector<int32_t> elems_Z, elems_A;
+  int n_zones() const { return (int)r_out.size(); }
+};
+
+// atomic data in the C-ABI layout (see sgpu_atoms); host vectors
+struct AtomicData {
+  std::vector<int32_t> lev_off, ion_off, lev_g, lev_ion, atom_zero_eps, line_ion, line_bin;
+  std::vector<double> lev_E, ion_chi, line_nu, line_gf, line_El;
+  std::vector<int64_t> line_bin_off;
+  int max_levels = 0;
+};
+
+class transport {
+ public:
+  transport() = default;
+  ~transport();
+  transport(const transport&) = delete;
+  transport& operator=(const transport&) = delete;
+
+  // transport::init (src/transport/transport_init.cpp:44-427), restricted to
+  // the hot path: frequency grid, tables, tallies, packet buffers
+  int init(const Params& p, const Grid1D& g, const AtomicData* atoms, int device = 0);
+  // transport::step (transport.cpp:26-140)
+  int step(double dt);
+
+  // results (copied to the host)
+  std::vector<double> J_nu() const;             // [nz][n_nu], normalised
+  std::vector<double> e_rad() const;
+  std::vector<double> e_abs() const;
+  std::vector<double> L_radio_dep() const;
+  std::vector<double> optical_spectrum() const; // flux per bin (raw counts of energy)
+  std::vector<double> abs_opacity() const;
+  int64_t n_particles() const { return n_particles_; }
+  int64_t n_escaped_last() const { return n_escaped_; }
+  const std::string& error() const { return err_; }
+  double t_now() const { return t_now_; }
+
+ private:
+  int set_opacity(double dt);
+  int wipe_radiation();
+  int emit_particles(double dt);
+  int propagate_all(double dt);
+  int clean_up_particle_vector();
+  int reduce_radiation(double dt);
+  int fail(int rc, const char* what);
+  void* dalloc(size_t bytes);
+  template <class T> T* upload(const std::vector<T>& v);
+  std::vector<double> download(const double* d, size_t n) const;
+
+  sgpu_ctx* ctx_ = nullptr;
+  Params p_;
+  int nz_ = 0, nn_ = 0, ne_ = 0;
+  double t_now_ = 0;
+  int first_step_ = 1;
+  uint64_t step_index_ = 0;
+  int64_t n_particles_ = 0, n_escaped_ = 0;
+  std::string err_;
+  std::vector<void*> owned_;
+  std::vector<double> nu_edges_;
+  sgpu_locate nu_{};
+  sgpu_grid grid_{};
+  sgpu_zone_state zs_{};
+  sgpu_atoms at_{};
+  bool have_atoms_ = false;
+  sgpu_opacity_params op_{};
+  sgpu_opacity_out out_{};
+  sgpu_tables tab_{};
+  sgpu_tallies ta_{};
+  sgpu_transport_params tp_{};
+  sgpu_packets pk_{}, scr_{};
+  double* d_rho_ = nullptr;
+  double* d_r_out_ = nullptr;
+  double* d_vol_ = nullptr;
+};
+
+}  // namespace sedona_gpu
