@@ -280,7 +280,8 @@ typedef struct sgpu_ctx sgpu_ctx;
 sgpu_ctx* sgpu_create(int device);
 void      sgpu_destroy(sgpu_ctx* ctx);
 /* Make the context issue its work on an existing hipStream_t (passed as void*);
- * NULL restores the context's own stream. */
+ * NULL selects the HIP null (default) stream.  The context starts on its own
+ * stream, created blocking so that it is ordered with the null stream. */
 int  sgpu_set_stream(sgpu_ctx* ctx, void* hip_stream);
 void* sgpu_get_stream(sgpu_ctx* ctx);
 int  sgpu_synchronize(sgpu_ctx* ctx);

@@ -71,7 +71,7 @@ sgpu_ctx* sgpu_create(int device)
   ctx->device = device;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->n_cu = prop.multiProcessorCount;
-  e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
+  e = hipStreamCreate(&ctx->own_stream);   // blocking: ordered with the null stream
   if (e != hipSuccess) { sgpu::hip_fail(e, "hipStreamCreate", __FILE__, __LINE__); delete ctx; return nullptr; }
   ctx->stream = ctx->own_stream;
   return ctx;
@@ -94,7 +94,7 @@ void sgpu_destroy(sgpu_ctx* ctx)
 int sgpu_set_stream(sgpu_ctx* ctx, void* s)
 {
   SGPU_REQUIRE(ctx, "null context");
-  ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+  ctx->stream = (hipStream_t)s;   // NULL selects the HIP null (default) stream
   return SGPU_OK;
 }
 

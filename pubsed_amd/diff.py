@@ -41,6 +41,9 @@ class DiffTransport:
         self.spec = P.tallies.optical.nu_grid
         self.n_spec = P.n_nu
         backend.sync()
+        # F is reported as a fraction of the emitted packet energy, so that
+        # losses and their gradients stay O(1) (the C-ABI works in erg/s)
+        self.scale = 1.0 / float(P.a["pk_e"][:n_packets].sum())
 
     def kappa_init(self, seed=7):
         rng = np.random.default_rng(seed)
@@ -61,13 +64,15 @@ class DiffTransport:
                                             abi.ptr(self.rec_bin) if record else None,
                                             abi.ptr(self.rec_w) if record else None),
               "sgpu_adjoint_forward")
-        return F
+        return F * self.scale
 
     def backward(self, kappa, dL_dF):
+        """dL/dkappa for the upstream gradient dL/dF of the scaled spectrum."""
         g = self.torch.zeros(self.nz * self.nn, dtype=self.torch.float64, device=kappa.device)
-        check(self.lib.sgpu_adjoint_backward(*self._args(kappa.contiguous()),
-                                             abi.ptr(dL_dF.contiguous().double()), abi.ptr(self.rec_bin),
-                                             abi.ptr(self.rec_w), abi.ptr(g)), "sgpu_adjoint_backward")
+        d = (dL_dF.double() * self.scale).contiguous()
+        check(self.lib.sgpu_adjoint_backward(*self._args(kappa.contiguous()), abi.ptr(d),
+                                             abi.ptr(self.rec_bin), abi.ptr(self.rec_w), abi.ptr(g)),
+              "sgpu_adjoint_backward")
         return g.view(self.nz, self.nn)
 
 

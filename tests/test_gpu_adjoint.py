@@ -62,11 +62,11 @@ def test_forward_and_gradient_match_cpu_reference(model):
     lib.or_adjoint_forward(ctypes.byref(G), ctypes.byref(nu), ctypes.byref(sp), kh.ctypes.data, rho, scat,
                            ctypes.byref(model.P.tparams), ctypes.byref(pk), model.n, Fh.ctypes.data,
                            rb.ctypes.data, rw.ctypes.data, None, None)
-    Fg = F.cpu().numpy()
+    Fg = F.cpu().numpy() / model.scale      # the model reports F / (emitted energy)
     assert np.array_equal(rb, model.rec_bin.cpu().numpy())
     assert np.max(np.abs(Fg - Fh)) <= 1e-10 * np.max(np.abs(Fh))
     gh = np.zeros(model.nz * model.nn)
-    d = np.ascontiguousarray(dLdF.cpu().numpy())
+    d = np.ascontiguousarray(dLdF.cpu().numpy()) * model.scale
     lib.or_adjoint_backward(ctypes.byref(G), ctypes.byref(nu), ctypes.byref(sp), kh.ctypes.data, rho, scat,
                             ctypes.byref(model.P.tparams), ctypes.byref(pk), model.n, d.ctypes.data,
                             rb.ctypes.data, rw.ctypes.data, gh.ctypes.data)
