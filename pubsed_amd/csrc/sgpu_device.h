@@ -88,6 +88,45 @@ __device__ __forceinline__ Rng make_rng(uint64_t seed, uint64_t id, uint32_t dra
   return r;
 }
 
+// FP32 draws, four per Philox block (24 bits of each output word); `draw`
+// is the next unused block, so a packet resumed from its stored counter
+// continues its stream without reusing a block.
+struct RngF {
+  uint32_t k0, k1, id_lo, id_hi, draw;
+  float b0, b1, b2, b3;
+  int left;
+  __device__ __forceinline__ float uniform_f()
+  {
+    if (left == 0) {
+      uint32_t c[4] = {draw++, 0u, id_lo, id_hi};
+      philox4x32(c, k0, k1);
+      const float s = 1.0f / 16777216.0f;
+      b0 = (float)(c[0] >> 8) * s; b1 = (float)(c[1] >> 8) * s;
+      b2 = (float)(c[2] >> 8) * s; b3 = (float)(c[3] >> 8) * s;
+      left = 4;
+    }
+    float v = b0;
+    b0 = b1; b1 = b2; b2 = b3;
+    --left;
+    return v;
+  }
+  template <class R> __device__ __forceinline__ R u() { return uniform_f(); }
+};
+
+__device__ __forceinline__ RngF make_rngf(uint64_t seed, uint64_t id, uint32_t draw)
+{
+  RngF r;
+  r.k0 = (uint32_t)seed; r.k1 = (uint32_t)(seed >> 32);
+  r.id_lo = (uint32_t)id; r.id_hi = (uint32_t)(id >> 32);
+  r.draw = draw;
+  r.b0 = r.b1 = r.b2 = r.b3 = 0.f;
+  r.left = 0;
+  return r;
+}
+
+template <class R> struct RngOf { typedef Rng type; };
+template <> struct RngOf<float> { typedef RngF type; };
+
 // ------------------------------------------------------------------
 // locate_array (src/utils/locate_array.cpp:191-242, locate_array.h:54-72)
 // ------------------------------------------------------------------

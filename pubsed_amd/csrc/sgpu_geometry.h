@@ -87,7 +87,7 @@ struct Pkt {
   double t, e;
   int ind, type;
   R dvds;
-  Rng rng;
+  typename RngOf<R>::type rng;
 };
 
 template <class R> __device__ __forceinline__ R rsqrt_(R v);
@@ -219,6 +219,21 @@ __device__ __forceinline__ int get_next_zone(const GridView<R>& g, const R* x, c
 {
   if (g.kind == SGPU_GRID_1D_SPHERE) return next_zone_1d(g, x, D, i, l);
   return next_zone_2d(g, x, D, i, l);
+}
+
+// Shortest boundary step.  An FP32 position resolves about 2^-24 |x|, and
+// r_o^2 - r^2 of a packet on a shell cancels to zero (or a few ulps, or a
+// negative value whose square root is NaN).  A packet skimming a shell then
+// gets boundary steps that do not move it and crosses back and forth between
+// the two zones without advancing, for up to millions of events.  The FP32
+// form therefore steps at least 2^-20 |x|_max across a boundary: x.D grows by
+// the step length with every step, so successive crossings make progress.
+// The FP64 form keeps the reference's step unchanged.
+__device__ __forceinline__ double floor_boundary_step(const Pkt<double>&, double l) { return l; }
+__device__ __forceinline__ float floor_boundary_step(const Pkt<float>& p, float l)
+{
+  const float m = fmaxf(fabsf(p.x[0]), fmaxf(fabsf(p.x[1]), fabsf(p.x[2])));
+  return fmaxf(l, m * 9.5367431640625e-7f);   // fmaxf also replaces a NaN step
 }
 
 // transport::do_dshift with homologous velocity (relativity.cpp:14-43)

@@ -15,6 +15,8 @@
 //                    accumulated in LDS per workgroup and flushed once,
 //                    J_nu and spectra as FP64 global atomics.
 #include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdlib>
 #include "sgpu_internal.h"
 #include "sgpu_geometry.h"
 
@@ -41,6 +43,7 @@ template <>
 struct Tab<float> {
   const float2* ke;     // (kappa = abs + scat, eps = abs / kappa)
   const float* cdf;
+  const int32_t* guide; // [nz][n_nu]: guide[z][k] = upper_bound(cdf_z, k / n_nu)
 };
 
 template <class R>
@@ -66,14 +69,84 @@ extern __shared__ double dyn_lds[];
 // global FP64 atomics.  Fields are addressed by (field, zone).
 enum { T_EABS = 0, T_FX, T_FY, T_FZ, T_FR, T_LDEP, T_NFIELDS };
 
+// tally array of a field index known only at run time (a switch, so that the
+// pointer table is not spilled to scratch memory for a dynamic index)
+__device__ __forceinline__ double* tally_field(const sgpu_tallies& t, int f)
+{
+  switch (f) {
+    case T_EABS: return t.e_abs;
+    case T_FX: return t.fx_rad;
+    case T_FY: return t.fy_rad;
+    case T_FZ: return t.fz_rad;
+    case T_FR: return t.fr_rad;
+    default: return t.L_radio_dep;
+  }
+}
+
 struct ZoneSink {
   double* g[T_NFIELDS];
   int nz;
   bool lds;
-  __device__ __forceinline__ void add(int field, int zone, double v) const
+  // accumulate the per-event zone tallies in registers while the packet stays
+  // in one zone and flush them when it leaves (the mixed form): a packet
+  // random-walking through an opaque zone then costs one atomic per field
+  // per zone visit instead of per event
+  bool accum;
+  int zone;
+  double acc[5];
+  // J_nu entry accumulated likewise while (zone, bin) is unchanged
+  double* J;
+  int64_t jkey;
+  double jacc;
+  __device__ __forceinline__ void add(int field, int zone_, double v) const
   {
-    if (lds) atomicAdd(&dyn_lds[field * nz + zone], v);
-    else atomic_add_f64(&g[field][zone], v);
+    if (lds) atomicAdd(&dyn_lds[field * nz + zone_], v);
+    else atomic_add_f64(&g[field][zone_], v);
+  }
+  __device__ __forceinline__ void jnu(int64_t k, double v)
+  {
+    if (!accum) { atomic_add_f64(&J[k], v); return; }
+    if (k != jkey) {
+      if (jkey >= 0) atomic_add_f64(&J[jkey], jacc);
+      jkey = k;
+      jacc = 0;
+    }
+    jacc += v;
+  }
+  __device__ __forceinline__ void flush()
+  {
+    if (zone >= 0) {
+#pragma unroll
+      for (int f = 0; f < 5; f++) {
+        if (acc[f] != 0) add(f, zone, acc[f]);
+        acc[f] = 0;
+      }
+    }
+    zone = -1;
+    if (jkey >= 0) atomic_add_f64(&J[jkey], jacc);
+    jkey = -1;
+    jacc = 0;
+  }
+  __device__ __forceinline__ void event(int z, bool photon, double eabs, double fx, double fy, double fz,
+                                        double fr)
+  {
+    if (!accum) {
+      if (photon) add(T_EABS, z, eabs);
+      add(T_FX, z, fx);
+      add(T_FY, z, fy);
+      add(T_FZ, z, fz);
+      add(T_FR, z, fr);
+      return;
+    }
+    if (z != zone) {
+      flush();
+      zone = z;
+    }
+    if (photon) acc[T_EABS] += eabs;
+    acc[T_FX] += fx;
+    acc[T_FY] += fy;
+    acc[T_FZ] += fz;
+    acc[T_FR] += fr;
   }
 };
 
@@ -120,13 +193,33 @@ __device__ __forceinline__ int get_opacity(const PropArgs<float>& A, const Pkt<f
 }
 
 // transport::sample_photon_frequency (emission.cpp:27-46)
-template <class R>
-__device__ __forceinline__ void sample_photon_frequency(const PropArgs<R>& A, Pkt<R>& p, Counts& c)
+__device__ __forceinline__ void sample_photon_frequency(const PropArgs<double>& A, Pkt<double>& p, Counts& c)
 {
   if (p.type == SGPU_PHOTON) {
     const int nn = A.nu.n;
-    int inu = min(cdf_sample(A.tab.cdf + (size_t)p.ind * nn, nn, p.rng.template u<R>()), nn - 1);
-    p.nu = ax_sample(A.nu, inu, p.rng.template u<R>());
+    int inu = min(cdf_sample(A.tab.cdf + (size_t)p.ind * nn, nn, p.rng.uniform()), nn - 1);
+    p.nu = ax_sample(A.nu, inu, p.rng.uniform());
+    c.redist++;
+  } else {
+    p.nu = 1;
+  }
+}
+
+// FP32 form: the same upper_bound(cdf, u) through a guide table (Chen & Asau
+// 1974): guide[k] = upper_bound(cdf, k/G), and every entry before it is
+// <= k/G <= u, so a short forward scan from guide[floor(u G)] lands on the
+// same bin as the binary search, in one or two loads instead of log2(n_nu)
+__device__ __forceinline__ void sample_photon_frequency(const PropArgs<float>& A, Pkt<float>& p, Counts& c)
+{
+  if (p.type == SGPU_PHOTON) {
+    const int nn = A.nu.n;
+    const float u = p.rng.uniform_f();
+    const size_t base = (size_t)p.ind * nn;
+    int k = min((int)(u * (float)nn), nn - 1);
+    int inu = A.tab.guide[base + k];
+    const float* cdf = A.tab.cdf + base;
+    while (inu < nn - 1 && cdf[inu] <= u) ++inu;
+    p.nu = ax_sample(A.nu, inu, p.rng.uniform_f());
     c.redist++;
   } else {
     p.nu = 1;
@@ -163,7 +256,7 @@ __device__ __forceinline__ void isotropic_scatter(const PropArgs<R>& A, Pkt<R>& 
 
 // transport::compton_scatter (scatter.cpp:76-133)
 template <class R>
-__device__ __forceinline__ void compton_scatter(const PropArgs<R>& A, Pkt<R>& p, const ZoneSink& zs,
+__device__ __forceinline__ void compton_scatter(const PropArgs<R>& A, Pkt<R>& p, ZoneSink& zs,
                                                 Counts& c)
 {
   lorentz_transform(A.g, p, 0);
@@ -188,7 +281,7 @@ __device__ __forceinline__ void compton_scatter(const PropArgs<R>& A, Pkt<R>& p,
 
 // transport::do_scatter (scatter.cpp:12-70)
 template <class R>
-__device__ __forceinline__ int do_scatter(const PropArgs<R>& A, Pkt<R>& p, R eps, const ZoneSink& zs,
+__device__ __forceinline__ int do_scatter(const PropArgs<R>& A, Pkt<R>& p, R eps, ZoneSink& zs,
                                           Counts& c)
 {
   int fate = SGPU_MOVING;
@@ -215,13 +308,14 @@ __device__ __forceinline__ int do_scatter(const PropArgs<R>& A, Pkt<R>& p, R eps
 // one iteration of transport::propagate_monte_carlo (transport.cpp:266-481)
 template <class R>
 __device__ __forceinline__ int mc_event(const PropArgs<R>& A, Pkt<R>& p, double tstop,
-                                        const ZoneSink& zs, Counts& c)
+                                        ZoneSink& zs, Counts& c)
 {
   const int nn = A.nu.n;
   const R cl = (R)C_LIGHT;
   c.events++;
   R d_bn = 0;
   int new_ind = get_next_zone(A.g, p.x, p.D, p.ind, &d_bn);
+  d_bn = floor_boundary_step(p, d_bn);
   R dshift = dshift_lab_to_comoving(A.g, p);
   R kappa, eps;
   int i_nu = get_opacity(A, p, dshift, &kappa, &eps);
@@ -244,16 +338,13 @@ __device__ __forceinline__ int mc_event(const PropArgs<R>& A, Pkt<R>& p, double 
   const double this_E = p.e * (double)this_d;
   const int z = p.ind;
   const double ds = (double)dshift, kap = (double)kappa;
-  if (p.type == SGPU_PHOTON) {
-    zs.add(T_EABS, z, this_E * ds * kap * (double)eps * ds * A.eps_imc[z]);
-    atomic_add_f64(&A.ta.J_nu[(size_t)z * (A.p.store_Jnu ? nn : 1) + (A.p.store_Jnu ? i_nu : 0)], this_E);
-  }
-  zs.add(T_FX, z, this_E * ds * kap * (double)p.D[0] * ds);
-  zs.add(T_FY, z, this_E * ds * kap * (double)p.D[1] * ds);
-  zs.add(T_FZ, z, this_E * ds * kap * (double)p.D[2] * ds);
+  const bool photon = p.type == SGPU_PHOTON;
+  if (photon) zs.jnu((int64_t)z * (A.p.store_Jnu ? nn : 1) + (A.p.store_Jnu ? i_nu : 0), this_E);
   R rr = rsqrt_(p.x[0] * p.x[0] + p.x[1] * p.x[1] + p.x[2] * p.x[2]);
   R xdotD = p.x[0] * p.D[0] + p.x[1] * p.D[1] + p.x[2] * p.D[2];
-  zs.add(T_FR, z, this_E * ds * kap * (double)xdotD / (double)rr * ds);
+  zs.event(z, photon, photon ? this_E * ds * kap * (double)eps * ds * A.eps_imc[z] : 0.0,
+           this_E * ds * kap * (double)p.D[0] * ds, this_E * ds * kap * (double)p.D[1] * ds,
+           this_E * ds * kap * (double)p.D[2] * ds, this_E * ds * kap * (double)xdotD / (double)rr * ds);
   p.x[0] += this_d * p.D[0];
   p.x[1] += this_d * p.D[1];
   p.x[2] += this_d * p.D[2];
@@ -304,6 +395,16 @@ __device__ __forceinline__ void spectrum_count(const sgpu_spectrum& s, double t,
   atomic_add_f64(&s.click[ind], 1.0);
 }
 
+__device__ __forceinline__ void init_rng(Rng& r, uint64_t seed, uint64_t id, uint32_t draw)
+{
+  r = make_rng(seed, id, draw);
+}
+
+__device__ __forceinline__ void init_rng(RngF& r, uint64_t seed, uint64_t id, uint32_t draw)
+{
+  r = make_rngf(seed, id, draw);
+}
+
 template <class R>
 __device__ __forceinline__ void load_packet(const PropArgs<R>& A, int64_t i, Pkt<R>& p)
 {
@@ -312,7 +413,7 @@ __device__ __forceinline__ void load_packet(const PropArgs<R>& A, int64_t i, Pkt
   p.D[0] = (R)k.Dx[i]; p.D[1] = (R)k.Dy[i]; p.D[2] = (R)k.Dz[i];
   p.t = k.t[i]; p.e = k.e[i]; p.nu = (R)k.nu[i];
   p.ind = k.ind[i]; p.type = k.type[i];
-  p.rng = make_rng(A.p.seed, k.id[i], k.draw[i]);
+  init_rng(p.rng, A.p.seed, k.id[i], k.draw[i]);
   p.dvds = 0;
 }
 
@@ -353,13 +454,15 @@ __device__ __forceinline__ void flush_counts(sgpu_counters* cnt, const Counts& c
 
 // PERSIST = false: one thread per packet (the plain drop-in).
 // PERSIST = true : lanes stride through the packets and refill on finish.
-template <class R, bool PERSIST>
-__global__ void __launch_bounds__(PROP_THREADS) propagate_kernel(PropArgs<R> A)
+// OCC: minimum waves per SIMD the register allocation must allow (1: no bound)
+template <class R, bool PERSIST, bool ACCUM, int OCC = 1>
+__global__ void __launch_bounds__(PROP_THREADS) __attribute__((amdgpu_waves_per_eu(OCC)))
+propagate_kernel(PropArgs<R> A)
 {
   const int nz = A.g.n_zones;
   const bool use_lds = A.lds_tallies;
   ZoneSink zs = {{A.ta.e_abs, A.ta.fx_rad, A.ta.fy_rad, A.ta.fz_rad, A.ta.fr_rad, A.ta.L_radio_dep},
-                 nz, use_lds};
+                 nz, use_lds, ACCUM, -1, {0, 0, 0, 0, 0}, A.ta.J_nu, -1, 0.0};
   if (use_lds) {
     for (int k = threadIdx.x; k < T_NFIELDS * nz; k += blockDim.x) dyn_lds[k] = 0;
     __syncthreads();
@@ -394,12 +497,13 @@ __global__ void __launch_bounds__(PROP_THREADS) propagate_kernel(PropArgs<R> A)
       i += stride;
     }
   }
+  zs.flush();
   flush_counts(A.cnt, c, mine);
   if (use_lds) {
     __syncthreads();
     for (int k = threadIdx.x; k < T_NFIELDS * nz; k += blockDim.x) {
       double v = dyn_lds[k];
-      if (v != 0) atomic_add_f64(&zs.g[k / nz][k % nz], v);
+      if (v != 0) atomic_add_f64(tally_field(A.ta, k / nz) + k % nz, v);
     }
   }
 }
@@ -415,6 +519,17 @@ __global__ void pack_tables_kernel(const double* abs, const double* scat, const 
   double op = a + s;
   ke[k] = make_float2((float)op, op == 0 ? 0.f : (float)(a / op));
   cdf32[k] = (float)cdf[k];
+}
+
+// guide table of each zone's FP32 emissivity CDF: guide[z][k] =
+// upper_bound(cdf_z, k / n_nu) (first entry larger than k / n_nu)
+__global__ void guide_kernel(const float* cdf32, int nz, int nn, int32_t* guide)
+{
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nz * nn) return;
+  const int z = (int)(t / nn), k = (int)(t - (size_t)z * nn);
+  const float v = (float)k / (float)nn;
+  guide[t] = min(upper_bound_dev(cdf32 + (size_t)z * nn, nn, v), nn - 1);
 }
 
 __global__ void to_f32_kernel(const double* src, float* dst, int n)
@@ -802,7 +917,6 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
                "unknown precision %d", params->precision);
   if (n == 0) return SGPU_OK;
   const int nz = grid->n_zones, nn = nu_grid->n;
-  timer_begin(ctx, "propagate");
   if (params->precision == SGPU_PREC_FP64) {
     PropArgs<double> A;
     A.g = make_grid<double>(*grid, grid->r_out.x, grid->x_out.x, grid->z_out.x);
@@ -810,7 +924,8 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
     A.tab = {tables->abs, tables->scat, tables->emis_cdf};
     A.eps_imc = tables->eps_imc; A.compton = tables->compton_opac; A.photoion = tables->photoion_opac;
     A.p = *params; A.pk = *packets; A.n = n; A.ta = *tallies; A.cnt = counters; A.lds_tallies = 0;
-    hipLaunchKernelGGL((propagate_kernel<double, false>), dim3(grid_blocks(n, PROP_THREADS)),
+    timer_begin(ctx, "propagate");
+    hipLaunchKernelGGL((propagate_kernel<double, false, false>), dim3(grid_blocks(n, PROP_THREADS)),
                        dim3(PROP_THREADS), 0, ctx->stream, A);
     SGPU_LAUNCH_CHECK();
   } else {
@@ -822,8 +937,12 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
     const int n_edges = nn + grid->r_out.n + grid->x_out.n + grid->z_out.n;
     float* e32 = (float*)scratch(ctx, "prop_edges32", sizeof(float) * n_edges, &st);
     if (st) return st;
+    int32_t* guide = (int32_t*)scratch(ctx, "prop_guide", sizeof(int32_t) * ntab, &st);
+    if (st) return st;
     hipLaunchKernelGGL(pack_tables_kernel, dim3(grid_blocks((int64_t)ntab, 256)), dim3(256), 0, ctx->stream,
                        tables->abs, tables->scat, tables->emis_cdf, params->omit_scattering, ntab, ke, cdf32);
+    hipLaunchKernelGGL(guide_kernel, dim3(grid_blocks((int64_t)ntab, 256)), dim3(256), 0, ctx->stream, cdf32, nz,
+                       nn, guide);
     SGPU_LAUNCH_CHECK();
     float* nu32 = e32;
     float* r32 = nu32 + nn;
@@ -843,16 +962,50 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
     PropArgs<float> A;
     A.g = make_grid<float>(*grid, r32, x32, z32);
     A.nu = make_axis<float>(*nu_grid, nu32);
-    A.tab = {ke, cdf32};
+    A.tab = {ke, cdf32, guide};
     A.eps_imc = tables->eps_imc; A.compton = tables->compton_opac; A.photoion = tables->photoion_opac;
     A.p = *params; A.pk = *packets; A.n = n; A.ta = *tallies; A.cnt = counters;
-    A.lds_tallies = nz <= LDS_ZONE_MAX;
+    // launch shape (tuning knobs for sweeps, defaults below):
+    //   SGPU_PROP_SHAPE  0 persistent lanes + register tallies (default)
+    //                    1 one packet per thread + register tallies
+    //                    2 one packet per thread, per-event atomics
+    //   SGPU_PROP_WG_PER_CU  persistent workgroups per CU (default 8)
+    //   SGPU_PROP_LDS    0: zone tallies as global atomics instead of LDS
+    //   SGPU_PROP_OCC    waves per SIMD the registers are bounded to (3, 4, 5)
+    const char* es = getenv("SGPU_PROP_SHAPE");
+    const int shape = es ? atoi(es) : 0;
+    const char* ew = getenv("SGPU_PROP_WG_PER_CU");
+    const int wg_per_cu = ew ? std::max(1, atoi(ew)) : 8;
+    const char* el = getenv("SGPU_PROP_LDS");
+    A.lds_tallies = nz <= LDS_ZONE_MAX && shape != 2 && !(el && atoi(el) == 0);
     const size_t lds = A.lds_tallies ? sizeof(double) * 6 * nz : 0;
-    // persistent lanes: enough workgroups to fill every CU several times over
-    int64_t want = (int64_t)ctx->n_cu * 8;
+    int64_t want = (int64_t)ctx->n_cu * wg_per_cu;
     int64_t need = grid_blocks(n, PROP_THREADS);
     unsigned blocks = (unsigned)(want < need ? want : need);
-    hipLaunchKernelGGL((propagate_kernel<float, true>), dim3(blocks), dim3(PROP_THREADS), lds, ctx->stream, A);
+    const char* eo = getenv("SGPU_PROP_OCC");
+    const int occ = eo ? atoi(eo) : 1;
+    timer_begin(ctx, "propagate");
+    if (shape == 0 && occ == 3)
+      hipLaunchKernelGGL((propagate_kernel<float, true, true, 3>), dim3(blocks), dim3(PROP_THREADS), lds,
+                         ctx->stream, A);
+    else if (shape == 0 && occ == 4)
+      hipLaunchKernelGGL((propagate_kernel<float, true, true, 4>), dim3(blocks), dim3(PROP_THREADS), lds,
+                         ctx->stream, A);
+    else if (shape == 0 && occ == 5)
+      hipLaunchKernelGGL((propagate_kernel<float, true, true, 5>), dim3(blocks), dim3(PROP_THREADS), lds,
+                         ctx->stream, A);
+    else if (shape == 1 && occ == 4)
+      hipLaunchKernelGGL((propagate_kernel<float, false, true, 4>), dim3((unsigned)need), dim3(PROP_THREADS), lds,
+                         ctx->stream, A);
+    else if (shape == 1)
+      hipLaunchKernelGGL((propagate_kernel<float, false, true>), dim3((unsigned)need), dim3(PROP_THREADS), lds,
+                         ctx->stream, A);
+    else if (shape == 2)
+      hipLaunchKernelGGL((propagate_kernel<float, false, false>), dim3((unsigned)need), dim3(PROP_THREADS), lds,
+                         ctx->stream, A);
+    else
+      hipLaunchKernelGGL((propagate_kernel<float, true, true>), dim3(blocks), dim3(PROP_THREADS), lds, ctx->stream,
+                         A);
     SGPU_LAUNCH_CHECK();
   }
   timer_end(ctx, "propagate");

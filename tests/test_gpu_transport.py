@@ -145,3 +145,40 @@ def test_compaction_and_normalisation(gpu):
         g, h = Pg.get(k), Ph.get(k)
         err = np.max(np.abs(g - h) / np.maximum(np.abs(h), 1e-300))
         assert err <= 1e-12, (k, err)
+
+
+def test_mixed_tangent_packet_on_shell_advances(gpu):
+    """A packet sitting exactly on a zone shell and moving tangentially gets a
+    zero-length boundary step from both zones (r_o^2 - r^2 = 0); with steps
+    that do not move it, it would cross back and forth between the two zones
+    for ever.  The FP32 form floors boundary steps at 2^-20 |x|, so these
+    packets leave an optically thin grid in a few events."""
+    import torch
+    P = runtime.Problem(runtime.Space("cuda:0"), "c1_grey", grey=1e-12)
+    gpu.opacity_build(P)
+    gpu.wipe(P)
+    r_out = P.get("r_out").astype(np.float32).astype(np.float64)   # FP32-exact shells
+    dirs = [(0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
+    shells = [1, P.nz // 3, P.nz // 2, P.nz - 2]
+    rows = [(r_out[k], d) for k in shells for d in dirs]
+    n = len(rows)
+    P.alloc_packets(n)
+    arr = {"x": np.array([r for r, _ in rows]), "y": np.zeros(n), "z": np.zeros(n),
+           "Dx": np.array([d[0] for _, d in rows], float), "Dy": np.array([d[1] for _, d in rows], float),
+           "Dz": np.array([d[2] for _, d in rows], float), "t": np.full(n, P.t_now), "e": np.ones(n),
+           "nu": np.full(n, 1e15), "ind": np.zeros(n, np.int32), "type": np.full(n, abi.PHOTON, np.int32),
+           "fate": np.zeros(n, np.int32), "draw": np.zeros(n, np.uint32),
+           "id": np.arange(n, dtype=np.uint64)}
+    P.load_packets(arr)
+    P.tparams.precision = abi.PREC_MIXED
+    P.tparams.max_events = 100000          # bounds the loop the floor prevents
+    cnt = P.space.zeros(10, "i64")
+    print("case mixed tangent packets on shells", flush=True)
+    gpu.propagate(P, n, cnt)
+    gpu.sync()
+    fate = P.packets_numpy(n)["fate"]
+    c = P.space.to_numpy(cnt)
+    print("fates", fate, "events", c[1])
+    assert np.all(fate == abi.ESCAPED), fate
+    assert c[1] <= 100 * n
+    torch.cuda.synchronize()
