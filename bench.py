@@ -37,27 +37,68 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--packets", type=int, default=10_000_000)
     ap.add_argument("--config", default="c2_line")
-    ap.add_argument("--plain-packets", type=int, default=0,
-                    help="packets of the plain FP64 comparison step (0: same as --packets)")
+    ap.add_argument("--plain-packets", type=int, default=1_000_000,
+                    help="packets of the plain FP64 comparison step (its rate is linear in the "
+                         "packet count from about 4e5 packets up; 0: same as --packets)")
+    ap.add_argument("--traffic-from", default=os.path.join(ROOT, "profiles", "propagate_mixed_c2_line.md"),
+                    help="rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the propagation kernel")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (the HBM counter calibration) and exit")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-nlte", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     return ap.parse_args()
 
 
+S_STATE = 96   # packet record in HBM: 9 FP64 + 3 int32 + uint32 draw + uint64 id
+
+
 def packet_bytes(c, nn, precision):
-    """Algorithmic HBM bytes per packet of the propagation kernel:
-    2 S_state + n_step (w_kappa + 8) + n_redist w_cdf ceil(log2 n_nu) + f_esc 16,
-    S_state = 96 B (9 FP64 + 3 int32 + uint32 + uint64 per packet), w_kappa the
-    opacity gather per event (16 B FP64 abs+scat, 8 B packed FP32 kappa,eps),
-    8 B the J_nu accumulate per event, w_cdf 8 (FP64) / 4 (FP32) bytes per
-    binary-search probe, 16 B of spectrum per escape."""
+    """Algorithmic HBM bytes per packet of the propagation kernel,
+    B_pkt = 2 S_state + n_step (2 w_kappa + 8) + n_redist 8 ceil(log2 n_nu) + f_esc 16:
+    the packet record read and written once, per event the opacity gather
+    (w_kappa = 8 B per FP64 entry, 4 B per FP32 entry) and an 8-byte J_nu
+    accumulate, per redistribution the binary search of the FP64 emissivity
+    CDF, 16 B of spectrum per escape.  (The FP32 form samples the CDF through
+    a guide table, one 4-byte entry and one or two 4-byte CDF probes instead
+    of 12 probes: fewer bytes than this figure counts.)"""
     n = max(c["n_packets"], 1)
     n_step = c["n_events"] / n
     n_redist = c["n_redist"] / n
     f_esc = c["n_escaped"] / n
-    wk, wc = (16, 8) if precision == 0 else (8, 4)
-    return 2 * 96 + n_step * (wk + 8) + n_redist * wc * math.ceil(math.log2(nn)) + f_esc * 16
+    wk = 8 if precision == 0 else 4
+    return 2 * S_STATE + n_step * (2 * wk + 8) + n_redist * 8 * math.ceil(math.log2(nn)) + f_esc * 16
+
+
+def read_traffic(path, n_packets):
+    """HBM bytes per launch of the propagation kernel from the committed
+    rocprofv3 summary: its `traffic_json:` line holds FETCH_SIZE and
+    WRITE_SIZE per packet, already divided by the copy-kernel calibration
+    factors.  Scaled to this run's packet count; None without a summary."""
+    if not path or not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_json:"):
+                t = json.loads(line.split(":", 1)[1])
+                per = t["fetch_bytes_per_packet"] + t["write_bytes_per_packet"]
+                return per * n_packets, t
+    return None, None
+
+
+def copy_calibration():
+    """One 1 GiB device-to-device copy, three times: FETCH_SIZE and
+    WRITE_SIZE of this kernel against its 2^30 bytes give the counters'
+    scale factors on this GPU."""
+    import torch
+    n = (1 << 30) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": 1 << 30, "launches": 3}))
 
 
 def main():
@@ -66,6 +107,9 @@ def main():
         print("bench.py measures one GPU: this build runs on a single MI355X and has no "
               "multi-GPU path (--gpus %d requested)" % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration:
+        copy_calibration()
+        return
     import numpy as np
     import torch
     from pubsed_amd import runtime, abi, workloads as W
@@ -93,18 +137,19 @@ def main():
             step(precision, n)
         torch.cuda.synchronize()
         cnt.zero_()
-        prop_ms = []
+        prop_ms, op_ms = [], []
         t0 = time.perf_counter()
         for _ in range(steps):
             step(precision, n, cnt)
             prop_ms.append(gpu.last_ms("propagate"))
+            op_ms.append(gpu.last_ms("opacity"))
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         c = dict(zip(abi.COUNTER_FIELDS, cnt.cpu().tolist()))
-        return dt, prop_ms, c
+        return dt, prop_ms, c, op_ms
 
     print("# bench: config %s, %d packets/step, mixed precision" % (a.config, N), file=sys.stderr, flush=True)
-    dt, prop_ms, c = timed(abi.PREC_MIXED, N, a.steps, a.warmup)
+    dt, prop_ms, c, op_ms = timed(abi.PREC_MIXED, N, a.steps, a.warmup)
     value = a.steps * N / dt
     ms_step = 1e3 * dt / a.steps
     prop_avg = float(np.mean(prop_ms))
@@ -115,10 +160,21 @@ def main():
     print("# mixed: %.3f s/step, propagate %.1f ms, events/packet %.1f" %
           (dt / a.steps, prop_avg, per_pkt["n_events"]), file=sys.stderr, flush=True)
 
-    # plain FP64 drop-in at equal packet count
-    Np = a.plain_packets or N
+    traffic, tinfo = read_traffic(a.traffic_from, N)
+    traffic_src = None
+    if tinfo:
+        traffic_src = "%s (rocprofv3 --pmc at %d packets, per-packet bytes scaled to %d)" % (
+            os.path.relpath(a.traffic_from, ROOT), tinfo["packets"], N)
+    op_avg = float(np.mean(op_ms))
+    n_lines = P.cfg.get("n_lines", 0) if P.cfg.get("fuzz") else 0
+    opacity = {"kernel_ms": op_avg, "zones": P.nz, "lines": n_lines,
+               "zone_builds_per_s": P.nz / (op_avg * 1e-3) if op_avg > 0 else None,
+               "line_exp_per_s": 3.0 * n_lines * P.nz / (op_avg * 1e-3) if op_avg > 0 else None}
+
+    # plain FP64 drop-in (its rate is linear in the packet count at these sizes)
+    Np = min(a.plain_packets or N, N)
     print("# plain FP64: %d packets" % Np, file=sys.stderr, flush=True)
-    dtp, prop_p, cp = timed(abi.PREC_FP64, Np, 1, 0)
+    dtp, prop_p, cp, _ = timed(abi.PREC_FP64, Np, 1, 0)
     plain = Np / dtp
     esc_m = c["n_escaped"] / max(c["n_packets"], 1)
     esc_p = cp["n_escaped"] / max(cp["n_packets"], 1)
@@ -152,9 +208,10 @@ def main():
         "escape_fraction": {"mixed": esc_m, "plain": esc_p},
         "per_packet": per_pkt,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "propagate_kernel<float,true>", "kernel_ms": prop_avg,
-                     "bytes_per_packet": bpk},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "propagate_kernel<float, true, true, 1>", "kernel_ms": prop_avg,
+                     "bytes_per_packet": bpk, "traffic_source": traffic_src},
+        "opacity_build": opacity,
         "nlte": nlte,
         "cpu_baseline": cpu,
     }
@@ -228,6 +285,8 @@ def bench_cpu(config, seconds):
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         pass
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
     ob = OracleBackend(n_threads=cores)
     P = runtime.Problem(runtime.Space(None), config)
     ob.opacity_build(P)
