@@ -35,7 +35,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--packets", type=int, default=10_000_000)
+    ap.add_argument("--packets", type=int, default=None,
+                    help="packets per step (default: the configuration's own count)")
     ap.add_argument("--config", default="c2_line")
     ap.add_argument("--plain-packets", type=int, default=1_000_000,
                     help="packets of the plain FP64 comparison step (its rate is linear in the "
@@ -119,7 +120,10 @@ def main():
     gpu = GpuBackend(0)
     gpu.use_torch_stream()
     gpu.enable_timing(True)
-    N = a.packets
+    N = a.packets or runtime.CONFIGS[a.config]["n_packets"]
+    if not runtime.CONFIGS[a.config]["steady"]:
+        print(json.dumps(bench_time_dependent(a, gpu)))
+        return
     P = runtime.Problem(runtime.Space(dev), a.config)
     P.alloc_packets(N)
     cnt = torch.zeros(10, dtype=torch.int64, device=dev)
@@ -216,6 +220,81 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def bench_time_dependent(a, gpu):
+    """Time-dependent configuration (the 2D ellipsoidal outflow): every step
+    is the homologous hydro step, opacity build, wipe, emission of N new
+    packets after the census, propagation to t + dt, compaction and
+    normalisation (runtime.time_step).  Census packets persist, so the timed
+    step (after the warm-up steps) propagates the census plus N.  `value`
+    is packets propagated per second of the whole step."""
+    import numpy as np
+    import torch
+    from pubsed_amd import runtime, abi, workloads as W
+    cfg = runtime.CONFIGS[a.config]
+    dt = cfg.get("dt", 0.1 * W.DAY)
+    cnt = torch.zeros(10, dtype=torch.int64, device="cuda:0")
+
+    def run(precision, n_emit, steps, warm):
+        P = runtime.Problem(runtime.Space("cuda:0"), a.config)
+        cap = n_emit * (steps + warm)
+        P.alloc_packets(cap)
+        scr = P.alloc_packets(cap, name="scr")
+        P.tparams.precision = precision
+        for k in range(warm):
+            runtime.time_step(gpu, P, dt, n_emit, scr, k)
+            print("# warm-up step %d: %d packets on the grid" % (k, P.n_packets), file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        cnt.zero_()
+        n_prop, prop_ms, op_ms = 0, [], []
+        t0 = time.perf_counter()
+        for k in range(warm, warm + steps):
+            n_prop += P.n_packets + n_emit
+            runtime.time_step(gpu, P, dt, n_emit, scr, k, cnt)
+            prop_ms.append(gpu.last_ms("propagate"))
+            op_ms.append(gpu.last_ms("opacity"))
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        c = dict(zip(abi.COUNTER_FIELDS, cnt.cpu().tolist()))
+        return P, el, n_prop, prop_ms, op_ms, c
+
+    N = a.packets or cfg["n_packets"]
+    print("# bench: config %s, %d new packets per step, dt %.3g s, mixed precision" % (a.config, N, dt),
+          file=sys.stderr, flush=True)
+    P, el, n_prop, prop_ms, op_ms, c = run(abi.PREC_MIXED, N, a.steps, a.warmup)
+    value = n_prop / el
+    prop_avg = float(np.mean(prop_ms))
+    bpk = packet_bytes(c, P.n_nu, 1)
+    achieved = bpk * (n_prop / a.steps) / (prop_avg * 1e-3) / 1e9
+    print("# mixed: %.3f s/step, propagate %.1f ms, %d packets propagated" % (el / a.steps, prop_avg, n_prop),
+          file=sys.stderr, flush=True)
+    Np = min(a.plain_packets or N, N)
+    print("# plain FP64: %d new packets per step" % Np, file=sys.stderr, flush=True)
+    _, elp, n_prop_p, _, _, cp = run(abi.PREC_FP64, Np, 1, a.warmup)
+    plain = n_prop_p / elp
+    per = {k: c[k] / max(c["n_packets"], 1) for k in ("n_events", "n_binshift", "n_redist", "n_escaped",
+                                                       "n_stopped", "n_scatter")}
+    return {
+        "metric": "packets/s & NLTE solves/s: batched/mixed vs plain GPU path at matched MC noise",
+        "value": value, "unit": "packets/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * el / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 geometry / f64 tallies",
+        "data": "synthetic (seeded ellipsoidal ejecta and line list)",
+        "config": {"workload": "2D axisymmetric ellipsoidal outflow: 128 x 256 (varpi, z) zones, axis ratio "
+                               "2, 1e5 fuzz lines, 4096 log bins, time-dependent with census",
+                   "new_packets_per_step": N, "dt_s": dt, "zones": P.nz, "nu_bins": P.n_nu,
+                   "packets_on_grid_after": P.n_packets},
+        "plain_fp64_packets_per_s": plain, "plain_new_packets_per_step": Np,
+        "mixed_over_plain": value / plain, "per_packet": per,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "propagate_kernel<float, true, true, 1>", "kernel_ms": prop_avg,
+                     "bytes_per_packet": bpk},
+        "opacity_build": {"kernel_ms": float(np.mean(op_ms)), "zones": P.nz,
+                          "lines": P.cfg.get("n_lines", 0)},
+        "nlte": None, "cpu_baseline": None,
+    }
 
 
 def bench_nlte(gpu, torch, W, abi, np):

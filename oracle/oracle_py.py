@@ -139,6 +139,13 @@ class OracleBackend:
         return self.lib.or_normalize_tallies(ctypes.byref(P.grid), ctypes.byref(P.nu),
                                              ctypes.byref(P.tparams), ctypes.byref(P.tallies))
 
+    def homologous_step(self, P, dt):
+        e = (P.grid.t_now + dt) / P.grid.t_now
+        a = P.a
+        self.lib.or_homologous_step(ctypes.byref(P.grid), ptr(a["rho"]), ptr(a.get("r_out")),
+                                    ptr(a.get("x_out")), ptr(a.get("z_out")), ptr(a["vol"]), e)
+        P.grid.t_now = P.grid.t_now + dt
+
     def nlte_solve(self, n, rates, lte, b, pop=None):
         return self.lib.or_nlte_solve(n, ptr(rates), ptr(lte), ptr(b), ptr(pop))
 

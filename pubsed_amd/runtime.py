@@ -283,6 +283,33 @@ class Problem:
             dst[...] = v
 
 
+def time_step(backend, P, dt, n_emit, scratch_pk, step_index, counters=None):
+    """One time step of a time-dependent run in the reference's order
+    (SedonaClass::evolve_system, SedonaClass.cpp:346-355, then
+    transport::step, transport.cpp:26-140): the homologous hydro step moves
+    the grid to t + dt first, then opacities, wipe, emission appended after
+    the census packets, propagation to t + dt, compaction (census packets
+    persist) and normalisation; the transport clock then advances by dt.
+    Returns (packets emitted, packets alive after the step, escaped)."""
+    backend.homologous_step(P, dt)
+    P.tparams.dt = dt
+    P.op_params.dt = dt
+    backend.opacity_build(P)
+    backend.wipe(P)
+    n0 = P.n_packets
+    got = backend.emit(P, n_emit, id_base=(step_index + 1) << 32, offset=n0)
+    n = n0 + got
+    backend.propagate(P, n, counters)
+    alive, esc = backend.compact(P, n, scratch_pk)
+    P.n_packets = alive
+    backend.normalize(P)
+    P.t_now += dt
+    P.tparams.t_now = P.t_now
+    P.op_params.t_now = P.t_now
+    P.op_params.first_step = 0
+    return got, alive, esc
+
+
 class GpuBackend:
     """Runs the reference's step functions through libsedona_gpu.so."""
 
@@ -357,6 +384,18 @@ class GpuBackend:
         return check(self.lib.sgpu_normalize_tallies(self.ctx, ctypes.byref(P.grid), ctypes.byref(P.nu),
                                                      ctypes.byref(P.tparams), ctypes.byref(P.tallies)),
                      "sgpu_normalize_tallies")
+
+    def homologous_step(self, P, dt):
+        """hydro_homologous::step: rho /= e^3, edges and volumes scaled by
+        e = (t+dt)/t, the grid's clock advanced by dt."""
+        from . import check
+        e = (P.grid.t_now + dt) / P.grid.t_now
+        a = P.a
+        check(self.lib.sgpu_homologous_step(self.ctx, ctypes.byref(P.grid), abi.ptr(a["rho"]),
+                                            abi.ptr(a.get("r_out")), abi.ptr(a.get("x_out")),
+                                            abi.ptr(a.get("z_out")), abi.ptr(a["vol"]), e),
+              "sgpu_homologous_step")
+        P.grid.t_now = P.grid.t_now + dt
 
     def nlte_solve(self, n, rates, lte, b, pop, work):
         from . import check

@@ -274,3 +274,23 @@ def test_thread_count_independence():
         assert np.array_equal(out[0][0][k], out[1][0][k])
     assert np.allclose(out[0][1], out[1][1], rtol=1e-12, atol=0)
     assert np.allclose(out[0][2], out[1][2], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("name", ["c1_grey", "c4_2d"])
+def test_homologous_step_conserves_mass_and_scales_edges(name):
+    """hydro_homologous::step: rho / e^3 with volumes ~ e^3 keeps every zone's
+    mass; edges scale by e exactly; the grid clock advances by dt."""
+    over = {"nx": 8, "nzz": 16, "nu": (1e14, 1e16, 64), "n_lines": 100} if name == "c4_2d" else {}
+    P = runtime.Problem(runtime.Space(None), name, **over)
+    ob = OracleBackend()
+    m0 = P.get("rho") * P.get("vol")
+    edges = [k for k in ("r_out", "x_out", "z_out") if k in P.a]
+    e0 = {k: P.get(k).copy() for k in edges}
+    t0 = P.grid.t_now
+    dt = 0.7 * W.DAY
+    ob.homologous_step(P, dt)
+    e = (t0 + dt) / t0
+    assert P.grid.t_now == t0 + dt
+    assert np.allclose(P.get("rho") * P.get("vol"), m0, rtol=1e-12, atol=0)
+    for k in edges:
+        assert np.array_equal(P.get(k), e0[k] * e), k
