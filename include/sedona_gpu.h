@@ -147,6 +147,20 @@ typedef struct {
   const int32_t* line_bin;         /* [n_lines] nu-grid bin               */
   const int64_t* line_bin_off;     /* [n_atoms*(n_nu+1)] CSR: lines of (atom a, bin b)
                                       are [off[a*(n_nu+1)+b], off[a*(n_nu+1)+b+1]) */
+  /* level-resolved (bound-bound) lines, AtomicLine (AtomicData.h:29-41) with
+   * the derived quantities of AtomicData.cpp:349-375: nu = dE/h, B_ul, B_lu,
+   * f_lu, bin = locate_within_bounds(nu).  Sorted by (atom, bin), keeping
+   * each atom's line order within a bin; NULL / 0 when absent.            */
+  int64_t n_blines;
+  const int32_t* bline_l;          /* [n_blines] lower level (index over all atoms) */
+  const int32_t* bline_u;          /* [n_blines] upper level                */
+  const double*  bline_nu;         /* [n_blines] Hz                         */
+  const double*  bline_f;          /* [n_blines] oscillator strength f_lu   */
+  const double*  bline_A;          /* [n_blines] Einstein A_ul (s^-1)       */
+  const double*  bline_Bul;        /* [n_blines] Einstein B_ul              */
+  const double*  bline_Blu;        /* [n_blines] Einstein B_lu              */
+  const int32_t* bline_bin;        /* [n_blines] nu-grid bin                */
+  const int64_t* bline_bin_off;    /* [n_atoms*(n_nu+1)] CSR over (atom, bin) */
 } sgpu_atoms;
 
 /* Opacity-build parameters (names of sedona_defaults.lua / transport members). */
@@ -167,6 +181,8 @@ typedef struct {
   int32_t first_step;              /* transport::first_step_: the state solve and
                                       opacities then use the UNclamped T_gas
                                       (transport_opacity.cpp:109-120)            */
+  int32_t use_line_expansion;      /* opacity_line_expansion: level-resolved lines
+                                      (GasState::line_expansion_opacity)          */
 } sgpu_opacity_params;
 
 /* Opacity-build outputs.  Tables [nz][n_nu]; scalars [nz]; optional

@@ -3,6 +3,7 @@
  * per-zone opacity / emissivity build (transport::set_opacity and the
  * GasState / AtomicSpecies functions it calls).
  */
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -342,6 +343,44 @@ int or_opacity_build(const sgpu_locate* nu, const sgpu_zone_state* zs,
         for (int j = 0; j < nn; j++) {
           scat[j] += es;
           abs[j] += 1e-20 * p->epsilon * es;
+        }
+      }
+      /* GasState::line_expansion_opacity (GasState_opacities.cpp:424-454) with
+       * AtomicSpecies::line_expansion_opacity (AtomicSpecies_opacities.cpp:329-375) */
+      if (p->use_line_expansion && at->n_blines > 0) {
+        for (int j = 0; j < nn; j++) { opac[j] = 0; aopac[j] = 0; }
+        for (int a = 0; a < at->n_atoms; a++) {
+          double this_eps = at->atom_zero_eps && at->atom_zero_eps[a] ? 0 : p->epsilon;
+          double n_dens = dens * mf[a] / (zs->elem_A[a] * OR_M_P);
+          for (int j = 0; j < nn; j++) atom_opac[j] = 0;
+          const int64_t* off = at->bline_bin_off + (size_t)a * (nn + 1);
+          for (int64_t l = off[0]; l < off[nn]; ++l) {
+            int ll = at->bline_l[l], lu = at->bline_u[l];
+            double nl = ln[ll], nup = ln[lu];
+            double gl = 1.0 * at->lev_g[ll], gu = 1.0 * at->lev_g[lu];
+            if (nl < DBL_MIN) continue;
+            double nu_0 = at->bline_nu[l];
+            double lam = OR_C / nu_0;
+            double tau = nl * n_dens * OR_SIGMA_TOT * at->bline_f[l] * time * lam;
+            tau = tau * (1 - nup * gl / (nl * gu));
+            if (nup * gl > nl * gu) tau = 0;   /* laser regime */
+            double etau = exp(-tau);
+            if (!isnan(etau)) atom_opac[at->bline_bin[l]] += (1 - etau);
+          }
+          for (int j = 0; j < nn; j++)
+            atom_opac[j] = atom_opac[j] * or_center(nu, j) / or_delta(nu, j) / OR_C / time;
+          for (int j = 0; j < nn; j++) {
+            opac[j] += atom_opac[j];
+            aopac[j] += atom_opac[j] * this_eps;
+          }
+        }
+        for (int j = 0; j < nn; j++) {
+          abs[j] += aopac[j];
+          scat[j] += opac[j] - aopac[j];
+          double nuc = or_center(nu, j);
+          double ezeta = exp(1.0 * OR_H * nuc / OR_K / temp);
+          double bb = 2.0 * nuc * nuc * nuc * OR_H / OR_C / OR_C / (ezeta - 1);
+          emis[j] += bb * aopac[j];
         }
       }
       if (p->use_fuzz_expansion) {

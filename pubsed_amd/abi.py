@@ -45,7 +45,10 @@ class Atoms(C.Structure):
                 ("lev_off", P), ("ion_off", P),
                 ("lev_E", P), ("lev_g", P), ("lev_ion", P), ("ion_chi", P),
                 ("atom_zero_eps", P), ("n_lines", i64), ("line_nu", P), ("line_gf", P),
-                ("line_El", P), ("line_ion", P), ("line_bin", P), ("line_bin_off", P)]
+                ("line_El", P), ("line_ion", P), ("line_bin", P), ("line_bin_off", P),
+                ("n_blines", i64), ("bline_l", P), ("bline_u", P), ("bline_nu", P), ("bline_f", P),
+                ("bline_A", P), ("bline_Bul", P), ("bline_Blu", P), ("bline_bin", P),
+                ("bline_bin_off", P)]
 
 
 class OpacityParams(C.Structure):
@@ -53,7 +56,8 @@ class OpacityParams(C.Structure):
                 ("temp_max", f64), ("maximum_opacity", f64), ("line_velocity_width", f64),
                 ("fleck_alpha", f64), ("use_electron_scattering", i32),
                 ("use_fuzz_expansion", i32), ("omit_scattering", i32),
-                ("omit_composition_decay", i32), ("radiative_eq", i32), ("first_step", i32)]
+                ("omit_composition_decay", i32), ("radiative_eq", i32), ("first_step", i32),
+                ("use_line_expansion", i32)]
 
 
 class OpacityOut(C.Structure):

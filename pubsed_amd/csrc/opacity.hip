@@ -21,6 +21,7 @@
 // fuzz_expansion_opacity (:464-494) and fuzzline_expansion_opacity
 // (src/opacity/AtomicSpecies_opacities.cpp:292-324), get_planck_mean /
 // get_rosseland_mean (GasState_opacities.cpp:564-666).
+#include <cfloat>
 #include "sgpu_internal.h"
 #include "sgpu_physics.h"
 
@@ -40,6 +41,8 @@ struct StateArgs {
   double* nion;        // [nz][n_ion]  n_dens * ion_frac
   double* upart;       // [nz][n_ion]  partition function
   sgpu_opacity_out out;
+  double* levn;        // [nz][n_lev]  level populations (fractions of the atom), or NULL
+  double* ndens;       // [nz][n_atoms] atom number densities, or NULL
 };
 
 struct ZoneLds {
@@ -210,16 +213,21 @@ __global__ void __launch_bounds__(WAVE) state_kernel(StateArgs A)
       if (A.out.ion_part) A.out.ion_part[(size_t)z * A.n_ion + k] = L.U[k];
     }
   }
-  if (A.out.lev_n) {
+  if (A.out.lev_n || A.levn) {
     for (int at_i = 0; at_i < at.n_atoms; ++at_i) {
       int i0 = at.ion_off[at_i];
       for (int l = at.lev_off[at_i] + lane; l < at.lev_off[at_i + 1]; l += WAVE) {
         int ion = at.lev_ion[l];
         double v = L.frac[i0 + ion] * (double)at.lev_g[l] * L.bz[l] / L.U[i0 + ion];
-        A.out.lev_n[(size_t)z * A.n_lev + l] = v < 1e-30 ? 1e-30 : v;
+        v = v < 1e-30 ? 1e-30 : v;
+        if (A.out.lev_n) A.out.lev_n[(size_t)z * A.n_lev + l] = v;
+        if (A.levn) A.levn[(size_t)z * A.n_lev + l] = v;
       }
     }
   }
+  if (A.ndens)
+    for (int at_i = lane; at_i < at.n_atoms; at_i += WAVE)
+      A.ndens[(size_t)z * at.n_atoms + at_i] = dens * L.mf[at_i] / (zs.elem_A[at_i] * M_PROTON);
   if (lane == 0) {
     A.out.n_elec[z] = root;
     if (A.out.zone_status) A.out.zone_status[z] = err;
@@ -239,6 +247,9 @@ struct BinArgs {
   double* scat;
   double* ednu;    // emissivity * dnu, scanned in place by K3
   const double* n_elec;
+  const double* levn;    // [nz][n_lev] (line expansion only)
+  const double* ndens;   // [nz][n_atoms]
+  int n_lev;
 };
 
 __global__ void __launch_bounds__(256) bins_kernel(BinArgs A)
@@ -266,6 +277,40 @@ __global__ void __launch_bounds__(256) bins_kernel(BinArgs A)
       double es = THOMSON * A.n_elec[z];
       sc += es;
       ab += 1e-20 * eps * es;
+    }
+    // GasState::line_expansion_opacity (GasState_opacities.cpp:424-454),
+    // AtomicSpecies::line_expansion_opacity (AtomicSpecies_opacities.cpp:329-375):
+    // the bin's level-resolved lines of each atom in their stored order
+    if (A.p.use_line_expansion && A.at.n_blines > 0) {
+      const double time = A.p.t_now;
+      const double* ln = A.levn + (size_t)z * A.n_lev;
+      double opac = 0, aopac = 0;
+      for (int a = 0; a < A.at.n_atoms; ++a) {
+        const double this_eps = (A.at.atom_zero_eps && A.at.atom_zero_eps[a]) ? 0 : eps;
+        const double n_dens = A.ndens[(size_t)z * A.at.n_atoms + a];
+        const int64_t* off = A.at.bline_bin_off + (size_t)a * (nn + 1);
+        double atom_opac = 0;
+        for (int64_t l = off[j]; l < off[j + 1]; ++l) {
+          const int ll = A.at.bline_l[l], lu = A.at.bline_u[l];
+          const double nl = ln[ll], nup = ln[lu];
+          const double gl = 1.0 * A.at.lev_g[ll], gu = 1.0 * A.at.lev_g[lu];
+          if (nl < DBL_MIN) continue;
+          const double lam = C_LIGHT / A.at.bline_nu[l];
+          double tau = nl * n_dens * SIGMA_TOT * A.at.bline_f[l] * time * lam;
+          tau = tau * (1 - nup * gl / (nl * gu));
+          if (nup * gl > nl * gu) tau = 0;   // laser regime
+          const double etau = exp(-tau);
+          if (!isnan(etau)) atom_opac += (1 - etau);
+        }
+        atom_opac = atom_opac * nuc / dnu / C_LIGHT / time;
+        opac += atom_opac;
+        aopac += atom_opac * this_eps;
+      }
+      ab += aopac;
+      sc += opac - aopac;
+      double ezeta = exp(1.0 * H_PLANCK * nuc / K_BOLTZ / T);
+      double bb = 2.0 * nuc * nuc * nuc * H_PLANCK / C_LIGHT / C_LIGHT / (ezeta - 1);
+      em += bb * aopac;
     }
     if (A.p.use_fuzz_expansion) {
       const double time = A.p.t_now;
@@ -491,13 +536,24 @@ extern "C" int sgpu_opacity_build(sgpu_ctx* ctx, const sgpu_locate* nu_grid,
   if (st) return st;
   int32_t* worst = (int32_t*)scratch(ctx, "op_worst", sizeof(int32_t), &st);
   if (st) return st;
+  const bool lexp = params->use_line_expansion && at.n_atoms > 0 && at.n_blines > 0;
+  double* levn = nullptr;
+  double* ndens = nullptr;
+  if (lexp) {
+    SGPU_REQUIRE(at.bline_l && at.bline_u && at.bline_nu && at.bline_f && at.bline_bin && at.bline_bin_off,
+                 "line expansion needs the level-resolved line arrays");
+    levn = (double*)scratch(ctx, "op_levn", sizeof(double) * (size_t)nz * n_lev, &st);
+    if (st) return st;
+    ndens = (double*)scratch(ctx, "op_ndens", sizeof(double) * (size_t)nz * at.n_atoms, &st);
+    if (st) return st;
+  }
 
   timer_begin(ctx, "opacity");
-  StateArgs sa{*zones, at, *params, n_lev, n_ion, temp_used, nion, upart, *out};
+  StateArgs sa{*zones, at, *params, n_lev, n_ion, temp_used, nion, upart, *out, levn, ndens};
   hipLaunchKernelGGL(state_kernel, dim3(nz), dim3(WAVE), 0, ctx->stream, sa);
   SGPU_LAUNCH_CHECK();
   BinArgs ba{*nu_grid, *zones, at, *params, n_ion, temp_used, nion, upart,
-             out->abs, out->scat, out->emis_cdf, out->n_elec};
+             out->abs, out->scat, out->emis_cdf, out->n_elec, levn, ndens, n_lev};
   hipLaunchKernelGGL(bins_kernel, dim3((nu_grid->n + 255) / 256, nz), dim3(256), 0, ctx->stream, ba);
   SGPU_LAUNCH_CHECK();
   FinArgs fa{*nu_grid, *zones, *params, temp_used, *out};

@@ -171,6 +171,7 @@ int transport::set_opacity(double dt)
   op_.fleck_alpha = p_.transport_fleck_alpha;
   op_.use_electron_scattering = p_.opacity_electron_scattering;
   op_.use_fuzz_expansion = p_.opacity_fuzz_expansion;
+  op_.use_line_expansion = p_.opacity_line_expansion;
   op_.radiative_eq = p_.transport_radiative_equilibrium;
   op_.first_step = first_step_;
   int rc = sgpu_opacity_build(ctx_, &nu_, &zs_, have_atoms_ ? &at_ : nullptr, &op_, &out_);

@@ -28,6 +28,7 @@ struct Params {
   double opacity_epsilon = 1.0;
   int opacity_electron_scattering = 0;
   int opacity_fuzz_expansion = 0;
+  int opacity_line_expansion = 0;
   double opacity_maximum_opacity = 1e40;
   double limits_temp_min = 1000;
   double limits_temp_max = 1e8;

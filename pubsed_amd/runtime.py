@@ -161,7 +161,9 @@ class Problem:
                                    seed=seed)
             self.atoms_np = at
             for k in ("lev_off", "ion_off", "lev_E", "lev_g", "lev_ion", "ion_chi", "atom_zero_eps",
-                      "line_nu", "line_gf", "line_El", "line_ion", "line_bin", "line_bin_off"):
+                      "line_nu", "line_gf", "line_El", "line_ion", "line_bin", "line_bin_off",
+                      "bline_l", "bline_u", "bline_nu", "bline_f", "bline_A", "bline_Bul",
+                      "bline_Blu", "bline_bin", "bline_bin_off"):
                 a["at_" + k] = S.asarray(at[k])
             self.n_lev = int(at["lev_off"][-1])
             self.n_ion = int(at["ion_off"][-1])
@@ -172,11 +174,16 @@ class Problem:
                                    len(at["line_nu"]),
                                    *[abi.ptr(a["at_" + k]) for k in
                                      ("line_nu", "line_gf", "line_El", "line_ion", "line_bin",
-                                      "line_bin_off")])
+                                      "line_bin_off")],
+                                   len(at["bline_nu"]),
+                                   *[abi.ptr(a["at_" + k]) for k in
+                                     ("bline_l", "bline_u", "bline_nu", "bline_f", "bline_A",
+                                      "bline_Bul", "bline_Blu", "bline_bin", "bline_bin_off")])
         self.t_now = ej["t"]
         self.op_params = abi.OpacityParams(
             self.t_now, 1.0 if cfg["steady"] else cfg.get("dt", 0.1 * W.DAY), cfg["epsilon"],
-            1000.0, 1e8, 1e40, 0.0, 0.0, cfg["escat"], cfg["fuzz"], 0, 0, cfg["radiative_eq"], 0)
+            1000.0, 1e8, 1e40, 0.0, 0.0, cfg["escat"], cfg["fuzz"], 0, 0, cfg["radiative_eq"], 0,
+            cfg.get("lexp", 0))
         for k in ("abs", "scat", "emis_cdf"):
             a[k] = S.zeros(nz * nn, "f64")
         for k in ("L_thermal", "planck_mean", "rosseland_mean", "n_elec", "compton_opac",

@@ -190,7 +190,9 @@ def synthetic_atoms(nu_grid, elem_Z, n_lines=100000, n_ions=4, levels_per_ion=12
         start = np.searchsorted(atom, a, side="left")
         off[a, 0] = start
         off[a, 1:] = start + np.cumsum(counts)
-    return dict(n_atoms=n_atoms, lev_off=np.array(lev_off, np.int32),
+    bl = level_lines(nu_grid, np.array(lev_off), np.array(lev_E), np.array(lev_g), np.array(lev_ion),
+                     rng)
+    return dict(**bl, n_atoms=n_atoms, lev_off=np.array(lev_off, np.int32),
                 ion_off=np.array(ion_off, np.int32), lev_E=np.array(lev_E, np.float64),
                 lev_g=np.array(lev_g, np.int32), lev_ion=np.array(lev_ion, np.int32),
                 ion_chi=np.array(ion_chi, np.float64),
@@ -200,6 +202,56 @@ def synthetic_atoms(nu_grid, elem_Z, n_lines=100000, n_ions=4, levels_per_ion=12
                 line_El=El.astype(np.float64), line_ion=ion.astype(np.int32),
                 line_bin=bins.astype(np.int32), line_bin_off=off.ravel(),
                 line_atom=atom.astype(np.int32))
+
+
+# physical_constants.h conversion factors used by the derived line data
+EV_TO_ERGS = 1.60217646e-12
+CM_TO_ANGS = 1.0e8
+ANGS_TO_CM = 1.0e-8
+H_PLANCK = 6.6260755e-27
+SIGMA_TOT = 0.0265400193567
+
+
+def level_lines(nu_grid, lev_off, lev_E, lev_g, lev_ion, rng, A_min=1e4, A_max=1e8):
+    """Level-resolved lines between every pair of levels of one ion with
+    E_u > E_l, Einstein A_ul log-uniform on [A_min, A_max] s^-1, and the
+    quantities AtomicData.cpp:349-375 derives from them, in its operation
+    order: nu = dE ev_to_ergs / h, B_ul = A c^2 / (2 h nu^3),
+    B_lu = B_ul g_u / g_l, lam = c / nu * cm_to_angs,
+    f_lu = (lam angs_to_cm)^2 A g_u / g_l / (8 pi sigma_tot) with pi = 3.14159,
+    bin = locate_within_bounds(nu) (lines off the grid go to its first or
+    last bin).  Sorted by (atom, bin) keeping each atom's line order."""
+    ll, lu, at = [], [], []
+    n_atoms = len(lev_off) - 1
+    for a in range(n_atoms):
+        for l in range(lev_off[a], lev_off[a + 1]):
+            for u in range(l + 1, lev_off[a + 1]):
+                if lev_ion[u] == lev_ion[l] and lev_E[u] > lev_E[l]:
+                    ll.append(l); lu.append(u); at.append(a)
+    ll, lu, at = np.array(ll, np.int32), np.array(lu, np.int32), np.array(at, np.int32)
+    A = 10.0 ** rng.uniform(math.log10(A_min), math.log10(A_max), len(ll))
+    dE = lev_E[lu] - lev_E[ll]
+    nu = dE * EV_TO_ERGS / H_PLANCK
+    gl, gu = lev_g[ll].astype(np.float64), lev_g[lu].astype(np.float64)
+    Bul = A * C_LIGHT * C_LIGHT / 2.0 / H_PLANCK / nu / nu / nu
+    Blu = Bul * gu / gl
+    lam = C_LIGHT / nu * CM_TO_ANGS
+    lam_cm = lam * ANGS_TO_CM
+    f = lam_cm * lam_cm * A * gu / gl / (8 * PI_REF * SIGMA_TOT)
+    gx = nu_grid["x"]
+    nn = nu_grid["n"]
+    bins = np.minimum(np.searchsorted(gx, nu, side="right"), nn - 1)
+    order = np.lexsort((np.arange(len(nu)), bins, at))
+    ll, lu, at, A, nu, f, Bul, Blu, bins = (x[order] for x in (ll, lu, at, A, nu, f, Bul, Blu, bins))
+    off = np.zeros((n_atoms, nn + 1), np.int64)
+    for a in range(n_atoms):
+        counts = np.bincount(bins[at == a], minlength=nn)
+        start = np.searchsorted(at, a, side="left")
+        off[a, 0] = start
+        off[a, 1:] = start + np.cumsum(counts)
+    return dict(bline_l=ll, bline_u=lu, bline_nu=nu, bline_f=f, bline_A=A, bline_Bul=Bul,
+                bline_Blu=Blu, bline_bin=bins.astype(np.int32), bline_bin_off=off.ravel(),
+                bline_atom=at)
 
 
 def nlte_systems(n_sys=1024, n_min=20, n_max=200, T=1.0e4, seed=20261203,

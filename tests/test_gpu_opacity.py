@@ -72,3 +72,19 @@ def test_first_step_uses_unclamped_temperature(gpu):
     gpu.sync()
     OracleBackend().opacity_build(Ph)
     assert relerr(Pg.get("abs"), Ph.get("abs"), 1e-300) <= RTOL
+
+
+@pytest.mark.parametrize("fuzz", [0, 1])
+def test_level_resolved_line_expansion_matches_oracle(gpu, fuzz):
+    """GasState::line_expansion_opacity (level-resolved lines from the LTE level
+    populations), alone and beside the fuzz lines, 32 zones."""
+    Pg, Ph = run_both("c2_line", gpu, nz=32, lexp=1, fuzz=fuzz)
+    assert Ph.atoms.n_blines > 0
+    for k in KEYS_TABLE + KEYS_ZONE:
+        e = relerr(Pg.get(k), Ph.get(k), floor=1e-300)
+        print(k, e)
+        assert e <= RTOL, (k, e)
+    # the level lines change the tables (they are not a no-op)
+    P0 = runtime.Problem(runtime.Space(None), "c2_line", nz=32, lexp=0, fuzz=fuzz)
+    OracleBackend().opacity_build(P0)
+    assert not np.allclose(P0.get("abs"), Ph.get("abs"))

@@ -197,3 +197,101 @@ def test_gamma_ray_coefficients():
     assert np.allclose(P.get("compton_opac"), comp, rtol=1e-14)
     assert np.allclose(P.get("photoion_opac"), photo, rtol=1e-14)
     assert np.all(P.get("eps_imc") == 1.0)
+
+
+# ---------------------------------------------------------------------------
+# level-resolved line expansion opacity (GasState::line_expansion_opacity,
+# AtomicSpecies_opacities.cpp:329-375)
+# ---------------------------------------------------------------------------
+EV_ERG = 1.60217646e-12
+
+
+def add_level_lines(z, lines):
+    """Attach level-resolved lines [(l, u, f)] (level indices of the HYD atom)
+    to a OneZone and switch the line-expansion term on; nu = (E_u - E_l) / h
+    (the absolute value for a deliberately inverted pair)."""
+    E = z.akeep["lev_E"]
+    nn = z.nu["n"]
+    ll = np.array([l for l, _, _ in lines], np.int32)
+    lu = np.array([u for _, u, _ in lines], np.int32)
+    f = np.array([x for _, _, x in lines], float)
+    nu = np.abs(E[lu] - E[ll]) * EV_ERG / H
+    bins = np.minimum(np.searchsorted(z.nu["x"], nu, side="right"), nn - 1).astype(np.int32)
+    order = np.argsort(bins, kind="stable")
+    ll, lu, f, nu, bins = ll[order], lu[order], f[order], nu[order], bins[order]
+    off = np.zeros(nn + 1, np.int64)
+    off[1:] = np.cumsum(np.bincount(bins, minlength=nn))
+    zero = np.zeros(len(ll))
+    z.bl = dict(ll=ll, lu=lu, f=f, nu=nu, bins=bins, off=off, zero=zero)
+    A = z.A
+    A.n_blines = len(ll)
+    A.bline_l, A.bline_u, A.bline_nu, A.bline_f = ll.ctypes.data, lu.ctypes.data, nu.ctypes.data, f.ctypes.data
+    A.bline_A = A.bline_Bul = A.bline_Blu = zero.ctypes.data
+    A.bline_bin, A.bline_bin_off = bins.ctypes.data, off.ctypes.data
+    z.p.use_line_expansion = 1
+    return bins
+
+
+def test_level_line_weak_limit_is_integrated_line_strength():
+    """tau << 1: the bin holds sigma_tot f n_l (1 - e^{-h nu/kT}) lambda nu_c / (c dnu):
+    the integrated line opacity over the bin width, with the LTE stimulated
+    emission factor (n_u g_l / (n_l g_u) = e^{-h nu / kT} for Boltzmann levels)."""
+    T, dens = 1.0e4, 1e-13
+    z = OneZone([], T=T, dens=dens)
+    f = 1e-11      # tau ~ 1e-7: linear to 5e-8, and 1 - e^{-tau} keeps ~1e-9 of precision
+    bins = add_level_lines(z, [(1, 2, f)])
+    o = z.run()
+    b = bins[0]
+    nu0 = z.bl["nu"][0]
+    n_l = o["lev_n"][1] * dens / MP
+    want_tau = 0.0265400193567 * f * n_l * (C / nu0) * z.t * (1 - math.exp(-H * nu0 / KB / T))
+    assert 1e-9 < want_tau < 1e-6
+    want = want_tau * z.unit(b)
+    got = o["abs"][b] + o["scat"][b]
+    # 1e-5: the Boltzmann factors use k_ev, h nu / kT uses k and ev_to_ergs
+    # (the reference's constants agree to 6e-6)
+    assert abs(got - want) <= 1e-5 * want
+    assert abs(o["abs"][b] - 0.6 * got) <= 1e-12 * got
+    mask = np.ones(z.nu["n"], bool)
+    mask[b] = False
+    assert np.all((o["abs"] + o["scat"])[mask] == 0)
+
+
+def test_level_lines_saturate_and_add():
+    """tau >> 1: each line puts exactly nu_c / (dnu c t) into its bin."""
+    z = OneZone([])
+    bins = add_level_lines(z, [(0, 1, 1e6), (0, 2, 1e6), (1, 2, 1e6)])
+    o = z.run()
+    tot = o["abs"] + o["scat"]
+    for b in set(bins.tolist()):
+        k = int(np.sum(bins == b))
+        assert abs(tot[b] - k * z.unit(b)) <= 1e-12 * tot[b], b
+
+
+def test_inverted_level_pair_is_dropped():
+    """The laser branch: a pair whose 'upper' level lies below its 'lower' one
+    has n_u g_l > n_l g_u in LTE, so tau = 0 and the line adds nothing."""
+    z = OneZone([])
+    bins = add_level_lines(z, [(2, 1, 1e6)])
+    o = z.run()
+    assert np.all(o["abs"] + o["scat"] == 0)
+    z2 = OneZone([])
+    bins2 = add_level_lines(z2, [(2, 1, 1e6), (1, 2, 1e-16)])
+    o2 = z2.run()
+    tot = o2["abs"] + o2["scat"]
+    assert tot[bins2[0]] > 0 and np.count_nonzero(tot) == 1
+
+
+def test_level_and_fuzz_lines_add():
+    nu0 = 5.0e14
+    fuzz = [(nu0, 1e-3, 0.2)]
+    za = OneZone(fuzz)
+    oa = za.run()
+    zb = OneZone([])
+    add_level_lines(zb, [(1, 2, 1e-4), (0, 1, 1e-2)])
+    ob = zb.run()
+    zc = OneZone(fuzz)
+    add_level_lines(zc, [(1, 2, 1e-4), (0, 1, 1e-2)])
+    oc = zc.run()
+    for k in ("abs", "scat"):
+        assert np.allclose(oc[k], oa[k] + ob[k], rtol=1e-13, atol=0), k
