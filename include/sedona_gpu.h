@@ -32,6 +32,8 @@
  *                                                             src/opacity/AtomicSpecies_state.cpp:131-196
  *   sgpu_nlte_batched       (no reference counterpart: all systems of a step in one launch)
  *   sgpu_homologous_step    hydro_homologous::step + grid expand src/hydro/hydro_homologous.cpp:27-61
+ *   sgpu_nlte_set_rates     AtomicSpecies::set_rates            src/opacity/AtomicSpecies_state.cpp:294-432
+ *   sgpu_nlte_populations   end of AtomicSpecies::solve_nlte    src/opacity/AtomicSpecies_state.cpp:178-193
  *   sgpu_adjoint_*          (no reference counterpart: differentiable transport)
  */
 #ifndef SEDONA_GPU_H
@@ -410,6 +412,37 @@ int sgpu_nlte_batched(sgpu_ctx* ctx, int32_t nsys, const int32_t* n,
                       const double* rates, const double* lte,
                       double* b, double* pop,
                       int32_t precision, double tol, int32_t* iters);
+
+/* Rate assembly of AtomicSpecies::set_rates (AtomicSpecies_state.cpp:294-432)
+ * for a batch of (zone, atom) systems, in the layout sgpu_nlte_batched reads.
+ * System s is atom sys_atom[s] at electron density ne[s] and temperature
+ * T[s]; its levels are the atom's levels (local index i = global level
+ * - lev_off[atom]).  Per-system inputs are indexed by GLOBAL line / level:
+ *   line_J [s*n_blines + l]   mean intensity J-bar of line l
+ *   Pic, Rci [s*n_levels + k] photoionisation and recombination rate
+ *                             coefficients of level k (calculate_radiative_rates)
+ *   lev_lte [s*n_levels + k]  LTE population fraction (solve_lte)
+ * Terms, in the reference's order: B_ul J + A and B_lu J per line; van
+ * Regemorter collisions (f_lu floored at 1e-3, zero upward rate beyond
+ * zeta = 700); collisional ionisation / recombination; R_ci n_e and P_ic
+ * (R_ci of E = 0 levels dropped when no_ground_recomb); every rate out of
+ * level i times lev_lte[i].  Outputs rates[rate_off[s] + i*n + j] and
+ * lte[vec_off[s] + i]; n[s] must equal the atom's level count.  Each matrix
+ * entry is summed by one thread in the reference's term order. */
+int sgpu_nlte_set_rates(sgpu_ctx* ctx, const sgpu_atoms* atoms, int32_t nsys,
+                        const int32_t* sys_atom, const double* ne, const double* T,
+                        const double* line_J, const double* Pic, const double* Rci,
+                        const double* lev_lte, int32_t use_collisions, int32_t no_ground_recomb,
+                        const int32_t* n, const int64_t* rate_off, const int64_t* vec_off,
+                        double* rates, double* lte);
+
+/* End of AtomicSpecies::solve_nlte (AtomicSpecies_state.cpp:178-193):
+ * lev_n = max(b * lev_lte, 1e-30) per level and ion_frac[s*n_ions + ion] =
+ * sum of lev_n over the ion's levels (global ion index; other atoms' entries
+ * are left untouched). */
+int sgpu_nlte_populations(sgpu_ctx* ctx, const sgpu_atoms* atoms, int32_t nsys,
+                          const int32_t* sys_atom, const int64_t* vec_off,
+                          const double* b, const double* lte, double* lev_n, double* ion_frac);
 
 /* Choose launch shapes for sgpu_nlte_batched by timing candidates on the
  * given batch (project-written autotuner).  The choice is kept in the

@@ -113,6 +113,11 @@ void     or_compton_sample(double nu, int64_t n, uint64_t seed, double* cost_out
 
 /* ---- NLTE (src/opacity/AtomicSpecies_state.cpp:131-196) ---- */
 int      or_nlte_solve(int n, const double* rates, const double* lte, double* b, double* pop);
+int      or_nlte_set_rates(const sgpu_atoms* at, int a, double ne, double T, const double* line_J,
+                           const double* Pic, const double* Rci, const double* lev_lte,
+                           int use_collisions, int no_ground_recomb, double* rates, double* lte);
+void     or_nlte_populations(const sgpu_atoms* at, int a, const double* b, const double* lte,
+                             double* lev_n, double* ion_frac);
 int      or_lu_decomp(int n, double* A, int* perm);
 void     or_lu_solve(int n, const double* LU, const int* perm, const double* rhs, double* x);
 

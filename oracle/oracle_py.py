@@ -65,6 +65,9 @@ def load():
         lib.or_homologous_step.argtypes = [P(abi.Grid), V, V, V, V, V, D]
         lib.or_nlte_solve.restype = I
         lib.or_nlte_solve.argtypes = [I, V, V, V, V]
+        lib.or_nlte_set_rates.restype = I
+        lib.or_nlte_set_rates.argtypes = [P(abi.Atoms), I, D, D, V, V, V, V, I, I, V, V]
+        lib.or_nlte_populations.argtypes = [P(abi.Atoms), I, V, V, V, V]
         lib.or_lu_decomp.restype = I
         lib.or_lu_decomp.argtypes = [I, V, V]
         lib.or_lu_solve.argtypes = [I, V, V, V, V]

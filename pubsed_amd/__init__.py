@@ -55,6 +55,9 @@ def _declare(lib):
         "sgpu_nlte_autotune": (I, [V, I32, V, V, V, V, V, V, V, V, ctypes.c_char_p,
                                    ctypes.c_size_t]),
         "sgpu_homologous_step": (I, [V, P(abi.Grid), V, V, V, V, V, D]),
+        "sgpu_nlte_set_rates": (I, [V, P(abi.Atoms), I32, V, V, V, V, V, V, V, I32, I32, V, V, V,
+                                    V, V]),
+        "sgpu_nlte_populations": (I, [V, P(abi.Atoms), I32, V, V, V, V, V, V]),
         "sgpu_adjoint_forward": (I, [V, P(abi.Grid), P(abi.Locate), P(abi.Locate), V, V, V,
                                      P(abi.TransportParams), P(abi.Packets), I64, V, V, V]),
         "sgpu_adjoint_backward": (I, [V, P(abi.Grid), P(abi.Locate), P(abi.Locate), V, V, V,

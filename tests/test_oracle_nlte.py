@@ -101,3 +101,181 @@ def test_two_level_atom_in_diluted_field(W_dil):
     OracleBackend().nlte_solve(3, np.ascontiguousarray(Rl), lte, b, pop)
     want = g1 / g2 * math.exp(eta) * (1 + math.exp(-eta) * (W_dil - 1)) / W_dil
     assert abs(pop[0] / pop[1] / want - 1) < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# rate assembly, AtomicSpecies::set_rates (AtomicSpecies_state.cpp:294-432)
+# ---------------------------------------------------------------------------
+import ctypes
+from pubsed_amd import runtime
+
+
+def lte_problem(nz=3, T=None):
+    """A small line-expansion problem after its LTE state solve: per zone the
+    temperature, n_e and LTE level fractions (the inputs of set_rates)."""
+    P = runtime.Problem(runtime.Space(None), "c2_line", nz=nz, n_lines=200)
+    if T is not None:
+        P.a["T"][:] = T
+    OracleBackend().opacity_build(P)
+    at = P.atoms
+    lev = P.get("lev_n").reshape(nz, at.n_levels)
+    return P, at, P.get("T"), P.get("n_elec"), lev
+
+
+def continuum_of(P, k):
+    A = P.atoms_np
+    a = np.searchsorted(A["lev_off"], k, side="right") - 1
+    for j in range(A["lev_off"][a], A["lev_off"][a + 1]):
+        if A["lev_ion"][j] == A["lev_ion"][k] + 1:
+            return j
+    return -1
+
+
+def lte_inputs(P, lev, z, ne, T, seed=3):
+    """J-bar = B_nu(T) at every line, and photoionisation rates with the
+    recombination coefficients that balance them in LTE:
+    P_ic n_i* = R_ci n_e n_c*."""
+    A = P.atoms_np
+    nu = A["bline_nu"]
+    J = 2 * H * nu ** 3 / C ** 2 / np.expm1(H * nu / KB / T)
+    rng = np.random.default_rng(seed)
+    nl = len(A["lev_E"])
+    Pic = 10.0 ** rng.uniform(-4, 0, nl)
+    Rci = np.zeros(nl)
+    for k in range(nl):
+        c = continuum_of(P, k)
+        if c >= 0:
+            Rci[k] = Pic[k] * lev[z, k] / (ne * lev[z, c])
+    return J, Pic, Rci
+
+
+def set_rates(P, a, ne, T, J, Pic, Rci, lte_all, coll=1, ngr=0):
+    lib = oracle_py.load()
+    A = P.atoms_np
+    n = int(A["lev_off"][a + 1] - A["lev_off"][a])
+    R = np.zeros(n * n)
+    l = np.zeros(n)
+    got = lib.or_nlte_set_rates(ctypes.byref(P.atoms), a, ne, T, J.ctypes.data, Pic.ctypes.data,
+                                Rci.ctypes.data, np.ascontiguousarray(lte_all).ctypes.data, coll, ngr,
+                                R.ctypes.data, l.ctypes.data)
+    assert got == n
+    return R.reshape(n, n), l
+
+
+def solve(R, l):
+    n = len(l)
+    b = np.zeros(n)
+    pop = np.zeros(n)
+    st = oracle_py.load().or_nlte_solve(n, np.ascontiguousarray(R).ctypes.data, l.ctypes.data,
+                                        b.ctypes.data, pop.ctypes.data)
+    assert st == 0
+    return b
+
+
+@pytest.mark.parametrize("coll", [0, 1])
+def test_set_rates_in_lte_field_gives_unit_departure(coll):
+    """J-bar = B_nu(T), LTE-balanced photo rates (and, with coll=1, van
+    Regemorter and collisional ionisation/recombination, whose up/down ratio
+    is the Boltzmann / Saha ratio): the populations solving the assembled
+    system are the LTE ones, b = 1.  The bar 2e-3 covers the reference's
+    constants, which are consistent only to ~1e-5 per kT (k_ev against
+    k/ev_to_ergs; 2.7/5.5908e-16 against the Saha factor with pi = 3.14159);
+    a wrong index, sign or missing factor moves b by O(1).  It applies to
+    levels holding at least 1e-6 of the atom: the departure-coefficient
+    system of set_rates (every row scaled by an LTE population spanning
+    1e-30..1) has a condition number ~1e18, so the b of levels populated at
+    1e-15 are set by rounding; every population error |b - 1| lte stays
+    within 2e-4 of the atom's total all the same."""
+    P, at, T, ne, lev = lte_problem()
+    for z in range(P.nz):
+        J, Pic, Rci = lte_inputs(P, lev, z, ne[z], T[z])
+        for a in range(at.n_atoms):
+            R, l = set_rates(P, a, ne[z], T[z], J, Pic, Rci, lev[z], coll=coll)
+            b = solve(R, l)
+            big = l >= 1e-6 * l.sum()
+            err = np.max(np.abs(b[big] - 1))
+            pop_err = np.max(np.abs(b - 1) * l) / l.sum()
+            print(z, a, coll, err, pop_err)
+            assert err <= 2e-3, (z, a, err)
+            assert pop_err <= 2e-4, (z, a, pop_err)
+
+
+def test_set_rates_terms_and_flags():
+    """Single-term checks against the formulae's physical content: the
+    Einstein relation between the radiative terms of one line, the van
+    Regemorter up/down ratio = Boltzmann, the f floor, no_ground_recomb, and
+    the factor lte_i on every rate out of level i."""
+    P, at, T, ne, lev = lte_problem(nz=1)
+    A = P.atoms_np
+    z, a = 0, 0
+    l0 = int(A["lev_off"][a])
+    n = int(A["lev_off"][a + 1]) - l0
+    nb = len(A["bline_nu"])
+    zero_l = np.zeros(len(A["lev_E"]))
+    ones = np.ones(len(A["lev_E"]))
+    # radiative only, J = 0: only A_ul on downward entries
+    R, _ = set_rates(P, a, ne[z], T[z], np.zeros(nb), zero_l, zero_l, ones, coll=0)
+    mine = np.where((A["bline_l"] >= l0) & (A["bline_l"] < l0 + n))[0]
+    Aexp = np.zeros((n, n))
+    for k in mine:
+        Aexp[A["bline_u"][k] - l0, A["bline_l"][k] - l0] += A["bline_A"][k]
+    assert np.array_equal(R, Aexp)
+    # van Regemorter terms (collisions on minus off): C_up / C_down is the
+    # Boltzmann ratio g_u / g_l e^{-dE/kT}
+    R = set_rates(P, a, ne[z], T[z], np.zeros(nb), zero_l, zero_l, ones, coll=1)[0] - Aexp
+    k = mine[0]
+    ll, lu = A["bline_l"][k] - l0, A["bline_u"][k] - l0
+    gl, gu = A["lev_g"][ll + l0], A["lev_g"][lu + l0]
+    dE = (A["lev_E"][lu + l0] - A["lev_E"][ll + l0]) * EV
+    ratio = R[ll, lu] / R[lu, ll]
+    assert abs(ratio / (gu / gl * math.exp(-dE / KB / T[z])) - 1) <= 1e-12
+    # the f floor: f < 1e-3 behaves as f = 1e-3
+    small = [k for k in mine if A["bline_f"][k] < 1e-3]
+    if small:
+        k = small[0]
+        ll, lu = A["bline_l"][k] - l0, A["bline_u"][k] - l0
+        zeta = dE_k = (A["lev_E"][lu + l0] - A["lev_E"][ll + l0]) * EV / KB / T[z]
+        want = 3.9 / zeta * T[z] ** -1.5 * ne[z] * 1e-3 * A["lev_g"][ll + l0] / A["lev_g"][lu + l0]
+        dup = [j for j in mine if A["bline_l"][j] == A["bline_l"][k] and A["bline_u"][j] == A["bline_u"][k]]
+        if len(dup) == 1:
+            assert abs(R[lu, ll] / want - 1) <= 1e-12
+    # no_ground_recomb removes R_ci n_e from ground levels only
+    Rci = np.full(len(A["lev_E"]), 2.0)
+    R0, _ = set_rates(P, a, ne[z], T[z], np.zeros(nb), zero_l, Rci, ones, coll=0, ngr=0)
+    R1, _ = set_rates(P, a, ne[z], T[z], np.zeros(nb), zero_l, Rci, ones, coll=0, ngr=1)
+    d = R0 - R1
+    for i in range(n):
+        c = continuum_of(P, i + l0)
+        if c < 0:
+            continue
+        want = 2.0 * ne[z] if A["lev_E"][i + l0] == 0 else 0.0
+        assert d[c - l0, i] == want
+    d[[continuum_of(P, i + l0) - l0 for i in range(n) if continuum_of(P, i + l0) >= 0],
+      [i for i in range(n) if continuum_of(P, i + l0) >= 0]] = 0
+    assert np.all(d == 0)
+    # every rate out of level i carries lte_i
+    w = np.linspace(0.5, 2.0, len(A["lev_E"]))
+    Rw, lw = set_rates(P, a, ne[z], T[z], np.zeros(nb), zero_l, Rci, w, coll=1)
+    Ru, _ = set_rates(P, a, ne[z], T[z], np.zeros(nb), zero_l, Rci, ones, coll=1)
+    assert np.allclose(Rw, Ru * w[l0:l0 + n, None], rtol=1e-15, atol=0)
+    assert np.array_equal(lw, w[l0:l0 + n])
+
+
+def test_populations_sum_to_ion_fractions():
+    P, at, T, ne, lev = lte_problem(nz=1)
+    A = P.atoms_np
+    lib = oracle_py.load()
+    for a in range(at.n_atoms):
+        l0, l1 = int(A["lev_off"][a]), int(A["lev_off"][a + 1])
+        n = l1 - l0
+        b = np.linspace(0.5, 1.5, n)
+        lte = lev[0, l0:l1].copy()
+        ln = np.zeros(n)
+        ni = int(A["ion_off"][a + 1] - A["ion_off"][a])
+        fr = np.zeros(ni)
+        lib.or_nlte_populations(ctypes.byref(P.atoms), a, b.ctypes.data, lte.ctypes.data, ln.ctypes.data,
+                                fr.ctypes.data)
+        assert np.array_equal(ln, np.maximum(b * lte, 1e-30))
+        ions = A["lev_ion"][l0:l1]
+        for s in range(ni):
+            assert abs(fr[s] - ln[ions == s].sum()) <= 1e-15 * max(fr[s], 1e-300)
