@@ -113,101 +113,145 @@ def test_uneven_sizes_edge(gpu):
     assert (it[:-1] > 0).all()
 
 
-def test_set_rates_solve_and_populations_match_oracle(gpu):
-    """Rate assembly (set_rates) for every (zone, atom) system of a small
-    line-expansion problem from its LTE state, seeded J-bar / P_ic / R_ci;
-    then the batched FP64 solve and the populations.  Rates within 1e-13 of
-    the oracle (device pow / exp against glibc's); populations within 1e-10
-    of the atom's total (the departure-coefficient systems have condition
-    numbers ~1e18, so b of levels populated at 1e-15 is set by rounding on
-    either side; levels holding >= 1e-6 of the atom agree to 1e-8)."""
-    import ctypes
-    import torch
-    from pubsed_amd import runtime, check
-    from oracle import oracle_py
-    nz = 6
+def rate_case(nz=6, near_lte=False, seed=5):
+    """Inputs of set_rates for every (zone, atom) system of a small
+    line-expansion problem from its LTE state: seeded J-bar / P_ic / R_ci,
+    or (near_lte) J-bar = 0.7 B_nu(T) with P_ic / R_ci balanced at LTE."""
+    from pubsed_amd import runtime
+    from test_oracle_nlte import lte_inputs
     Ph = runtime.Problem(runtime.Space(None), "c2_line", nz=nz, n_lines=200)
-    Pg = runtime.Problem(runtime.Space("cuda:0"), "c2_line", nz=nz, n_lines=200)
     OracleBackend().opacity_build(Ph)
     A = Ph.atoms_np
     na, nl, nb = Ph.atoms.n_atoms, Ph.atoms.n_levels, Ph.atoms.n_blines
     lev = Ph.get("lev_n").reshape(nz, nl)
     T, ne = Ph.get("T"), Ph.get("n_elec")
-    rng = np.random.default_rng(5)
     sys_atom = np.array([a for z in range(nz) for a in range(na)], np.int32)
     sys_zone = np.array([z for z in range(nz) for a in range(na)], np.int32)
     ns = len(sys_atom)
+    rng = np.random.default_rng(seed)
+    if near_lte:
+        J, Pic, Rci = np.zeros((ns, nb)), np.zeros((ns, nl)), np.zeros((ns, nl))
+        for s in range(ns):
+            z = sys_zone[s]
+            J[s], Pic[s], Rci[s] = lte_inputs(Ph, lev, z, ne[z], T[z], seed=s)
+        J *= 0.7
+    else:
+        J = 10.0 ** rng.uniform(-8, -3, (ns, nb))
+        Pic = 10.0 ** rng.uniform(-4, 0, (ns, nl))
+        Rci = 10.0 ** rng.uniform(-14, -10, (ns, nl))
     n = np.diff(A["lev_off"])[sys_atom].astype(np.int32)
-    rate_off = np.concatenate([[0], np.cumsum(n.astype(np.int64) ** 2)])
-    vec_off = np.concatenate([[0], np.cumsum(n.astype(np.int64))])
-    J = 10.0 ** rng.uniform(-8, -3, (ns, nb))
-    Pic = 10.0 ** rng.uniform(-4, 0, (ns, nl))
-    Rci = 10.0 ** rng.uniform(-14, -10, (ns, nl))
-    lte_in = lev[sys_zone]
-    ne_s, T_s = ne[sys_zone].copy(), T[sys_zone].copy()
-    # oracle
+    return dict(Ph=Ph, A=A, sys_atom=sys_atom, ne=ne[sys_zone].copy(), T=T[sys_zone].copy(), J=J, Pic=Pic,
+                Rci=Rci, lte_in=lev[sys_zone], n=n,
+                rate_off=np.concatenate([[0], np.cumsum(n.astype(np.int64) ** 2)]),
+                vec_off=np.concatenate([[0], np.cumsum(n.astype(np.int64))]))
+
+
+def oracle_rates(c):
+    import ctypes
+    from oracle import oracle_py
     lib = oracle_py.load()
-    R_h = np.zeros(rate_off[-1])
-    l_h = np.zeros(vec_off[-1])
-    b_h = np.zeros(vec_off[-1])
-    ln_h = np.zeros((ns, nl))
+    Ph, A = c["Ph"], c["A"]
+    ns = len(c["sys_atom"])
+    R_h, l_h, b_h = np.zeros(c["rate_off"][-1]), np.zeros(c["vec_off"][-1]), np.zeros(c["vec_off"][-1])
+    ln_h = np.zeros((ns, Ph.atoms.n_levels))
     fr_h = np.zeros((ns, Ph.atoms.n_ions))
     for s in range(ns):
-        a = int(sys_atom[s])
-        R = np.zeros(n[s] * n[s])
-        l = np.zeros(n[s])
-        lib.or_nlte_set_rates(ctypes.byref(Ph.atoms), a, ne_s[s], T_s[s], np.ascontiguousarray(J[s]).ctypes.data,
-                              np.ascontiguousarray(Pic[s]).ctypes.data, np.ascontiguousarray(Rci[s]).ctypes.data,
-                              np.ascontiguousarray(lte_in[s]).ctypes.data, 1, 1, R.ctypes.data, l.ctypes.data)
-        R_h[rate_off[s]:rate_off[s + 1]] = R
-        l_h[vec_off[s]:vec_off[s + 1]] = l
-        b = np.zeros(n[s])
-        lib.or_nlte_solve(int(n[s]), R.ctypes.data, l.ctypes.data, b.ctypes.data, None)
-        b_h[vec_off[s]:vec_off[s + 1]] = b
+        a, n = int(c["sys_atom"][s]), int(c["n"][s])
+        R, l, b = np.zeros(n * n), np.zeros(n), np.zeros(n)
+        lib.or_nlte_set_rates(ctypes.byref(Ph.atoms), a, c["ne"][s], c["T"][s],
+                              *[np.ascontiguousarray(c[k][s]).ctypes.data for k in ("J", "Pic", "Rci", "lte_in")],
+                              1, 1, R.ctypes.data, l.ctypes.data)
+        lib.or_nlte_solve(n, R.ctypes.data, l.ctypes.data, b.ctypes.data, None)
+        ro, vo = c["rate_off"], c["vec_off"]
+        R_h[ro[s]:ro[s + 1]], l_h[vo[s]:vo[s + 1]], b_h[vo[s]:vo[s + 1]] = R, l, b
         l0, l1 = A["lev_off"][a], A["lev_off"][a + 1]
         i0, i1 = A["ion_off"][a], A["ion_off"][a + 1]
-        ln = np.zeros(n[s])
-        fr = np.zeros(i1 - i0)
+        ln, fr = np.zeros(n), np.zeros(i1 - i0)
         lib.or_nlte_populations(ctypes.byref(Ph.atoms), a, b.ctypes.data, l.ctypes.data, ln.ctypes.data,
                                 fr.ctypes.data)
-        ln_h[s, l0:l1] = ln
-        fr_h[s, i0:i1] = fr
-    # GPU
+        ln_h[s, l0:l1], fr_h[s, i0:i1] = ln, fr
+    return R_h, l_h, b_h, ln_h, fr_h
+
+
+def gpu_rates(gpu, c):
+    import ctypes
+    import torch
+    from pubsed_amd import runtime, check
+    Ph = c["Ph"]
+    Pg = runtime.Problem(runtime.Space("cuda:0"), "c2_line", nz=Ph.nz, n_lines=200)
+    ns = len(c["sys_atom"])
     dev = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to("cuda:0")
-    d = {k: dev(v) for k, v in dict(sys_atom=sys_atom, ne=ne_s, T=T_s, J=J, Pic=Pic, Rci=Rci, lte_in=lte_in,
-                                      n=n, rate_off=rate_off, vec_off=vec_off).items()}
-    R_g = torch.zeros(int(rate_off[-1]), dtype=torch.float64, device="cuda:0")
-    l_g = torch.zeros(int(vec_off[-1]), dtype=torch.float64, device="cuda:0")
+    d = {k: dev(c[k]) for k in ("sys_atom", "ne", "T", "J", "Pic", "Rci", "lte_in", "n", "rate_off", "vec_off")}
+    R_g = torch.zeros(int(c["rate_off"][-1]), dtype=torch.float64, device="cuda:0")
+    l_g = torch.zeros(int(c["vec_off"][-1]), dtype=torch.float64, device="cuda:0")
     b_g = torch.zeros_like(l_g)
-    ln_g = torch.zeros((ns, nl), dtype=torch.float64, device="cuda:0")
+    ln_g = torch.zeros((ns, Ph.atoms.n_levels), dtype=torch.float64, device="cuda:0")
     fr_g = torch.zeros((ns, Ph.atoms.n_ions), dtype=torch.float64, device="cuda:0")
     p = abi.ptr
-    print("case set_rates", ns, "systems", flush=True)
-    check(gpu.lib.sgpu_nlte_set_rates(gpu.ctx, ctypes.byref(Pg.atoms), ns, p(d["sys_atom"]), p(d["ne"]),
-                                      p(d["T"]), p(d["J"]), p(d["Pic"]), p(d["Rci"]), p(d["lte_in"]), 1, 1,
-                                      p(d["n"]), p(d["rate_off"]), p(d["vec_off"]), p(R_g), p(l_g)),
+    check(gpu.lib.sgpu_nlte_set_rates(gpu.ctx, ctypes.byref(Pg.atoms), ns,
+                                      *[p(d[k]) for k in ("sys_atom", "ne", "T", "J", "Pic", "Rci", "lte_in")],
+                                      1, 1, p(d["n"]), p(d["rate_off"]), p(d["vec_off"]), p(R_g), p(l_g)),
           "sgpu_nlte_set_rates")
-    gpu.sync()
-    Rg = R_g.cpu().numpy()
-    nzm = R_h != 0
-    assert np.array_equal(Rg != 0, nzm)
-    assert np.max(np.abs(Rg[nzm] - R_h[nzm]) / np.abs(R_h[nzm])) <= 1e-13
-    assert np.array_equal(l_g.cpu().numpy(), l_h)
-    print("case batched solve + populations", flush=True)
     gpu.nlte_batched(ns, d["n"], d["rate_off"], d["vec_off"], R_g, l_g, b_g, None, abi.NLTE_FP64, 0.0, None)
     check(gpu.lib.sgpu_nlte_populations(gpu.ctx, ctypes.byref(Pg.atoms), ns, p(d["sys_atom"]), p(d["vec_off"]),
                                         p(b_g), p(l_g), p(ln_g), p(fr_g)), "sgpu_nlte_populations")
     gpu.sync()
-    lng, frg = ln_g.cpu().numpy(), fr_g.cpu().numpy()
-    bg = b_g.cpu().numpy()
-    for s in range(ns):
-        a = int(sys_atom[s])
+    return [x.cpu().numpy() for x in (R_g, l_g, b_g, ln_g, fr_g)]
+
+
+def backward_error(c, R, l, b, s):
+    n = int(c["n"][s])
+    ro, vo = c["rate_off"], c["vec_off"]
+    Rm = R[ro[s]:ro[s + 1]].reshape(n, n)
+    lv, bv = l[vo[s]:vo[s + 1]], b[vo[s]:vo[s + 1]]
+    M = Rm.T.copy()
+    np.fill_diagonal(M, -Rm.sum(1))
+    M[n - 1, :] = lv
+    rhs = np.zeros(n)
+    rhs[-1] = 1
+    return np.max(np.abs(M @ bv - rhs)) / (np.max(np.abs(M).sum(1)) * np.max(np.abs(bv)))
+
+
+def test_set_rates_matches_oracle_and_solves_stably(gpu):
+    """Rate assembly (set_rates) for every (zone, atom) system from seeded
+    J-bar / P_ic / R_ci: every rate within 1e-13 of the oracle (device pow /
+    exp against glibc's), the same sparsity, the same lte.  These seeded
+    rates make systems with condition numbers up to 1e40 (rows scaled by LTE
+    populations spanning 1e-30..1; the oracle's LU and LAPACK's part by
+    5e-3 there), so the batched solve is held to what a backward-stable
+    solver guarantees whatever the conditioning: backward error <= 1e-13."""
+    c = rate_case()
+    print("case set_rates seeded", len(c["sys_atom"]), "systems", flush=True)
+    R_h, l_h, _, _, _ = oracle_rates(c)
+    R_g, l_g, b_g, _, _ = gpu_rates(gpu, c)
+    nzm = R_h != 0
+    assert np.array_equal(R_g != 0, nzm)
+    assert np.max(np.abs(R_g[nzm] - R_h[nzm]) / np.abs(R_h[nzm])) <= 1e-13
+    assert np.array_equal(l_g, l_h)
+    for s in range(len(c["sys_atom"])):
+        assert backward_error(c, R_g, l_g, b_g, s) <= 1e-13, s
+
+
+def test_set_rates_solve_and_populations_near_lte_match_oracle(gpu):
+    """Near-LTE inputs (J-bar = 0.7 B_nu(T), P_ic / R_ci balanced at LTE):
+    populations and ion fractions within 1e-7 of the atom's total, b of
+    levels holding >= 1e-6 of the atom within 1e-5 (the oracle's LU and
+    LAPACK agree to 2e-8 and 2e-6 on these systems)."""
+    c = rate_case(near_lte=True)
+    print("case set_rates near LTE", flush=True)
+    R_h, l_h, b_h, ln_h, fr_h = oracle_rates(c)
+    R_g, l_g, b_g, ln_g, fr_g = gpu_rates(gpu, c)
+    nzm = R_h != 0
+    assert np.max(np.abs(R_g[nzm] - R_h[nzm]) / np.abs(R_h[nzm])) <= 1e-13
+    A, vo = c["A"], c["vec_off"]
+    for s in range(len(c["sys_atom"])):
+        a = int(c["sys_atom"][s])
         l0, l1 = A["lev_off"][a], A["lev_off"][a + 1]
-        tot = ln_h[s, l0:l1].sum()
-        assert np.max(np.abs(lng[s, l0:l1] - ln_h[s, l0:l1])) <= 1e-10 * tot, s
         i0, i1 = A["ion_off"][a], A["ion_off"][a + 1]
-        assert np.max(np.abs(frg[s, i0:i1] - fr_h[s, i0:i1])) <= 1e-10 * tot, s
-        lv = l_h[vec_off[s]:vec_off[s + 1]]
+        tot = ln_h[s, l0:l1].sum()
+        assert np.max(np.abs(ln_g[s, l0:l1] - ln_h[s, l0:l1])) <= 1e-7 * tot, s
+        assert np.max(np.abs(fr_g[s, i0:i1] - fr_h[s, i0:i1])) <= 1e-7 * tot, s
+        lv = l_h[vo[s]:vo[s + 1]]
         big = lv >= 1e-6 * lv.sum()
-        bh = b_h[vec_off[s]:vec_off[s + 1]]
-        assert np.max(np.abs(bg[vec_off[s]:vec_off[s + 1]][big] - bh[big]) / np.abs(bh[big])) <= 1e-8, s
+        bh, bg = b_h[vo[s]:vo[s + 1]], b_g[vo[s]:vo[s + 1]]
+        assert np.max(np.abs(bg[big] - bh[big]) / np.abs(bh[big])) <= 1e-5, s
