@@ -49,7 +49,7 @@ __device__ __forceinline__ int ax_locate(const Axis<float>& g, float xv)
   if (xv < g.min) return 0;
   int ind;
   if (g.type == SGPU_LOC_LIN) ind = (int)floorf((xv - g.min) / g.del);
-  else if (g.type == SGPU_LOC_LOG) ind = (int)floorf(__logf(xv / g.min) * g.inv_log1p);
+  else if (g.type == SGPU_LOC_LOG) ind = (int)floorf(__logf(__fdividef(xv, g.min)) * g.inv_log1p);
   else if (g.type == SGPU_LOC_NONE) ind = 0;
   else ind = upper_bound_dev(g.x, n, xv);
   if (ind < 0) ind = 0;
@@ -103,10 +103,31 @@ template <> __device__ __forceinline__ void rsincos<double>(double v, double* s,
 }
 template <> __device__ __forceinline__ void rsincos<float>(float v, float* s, float* c)
 {
-  *s = sinf(v); *c = cosf(v);
+  __sincosf(v, s, c);   // hardware sine / cosine (v in [0, 2 pi): ~2^-21 absolute)
 }
 
 template <class R> __device__ __forceinline__ R rinf() { return (R)INFINITY; }
+
+// Division, 1/sqrt and the time of flight.  The FP64 form keeps IEEE
+// division and square root in the reference's operation order (it follows
+// the CPU path packet for packet); the FP32 form uses the hardware
+// reciprocal and reciprocal square root (1-2 ulp), far inside what its
+// statistical parity resolves, and multiplies by 1/c instead of dividing.
+template <class R> __device__ __forceinline__ R qdiv(R a, R b);
+template <> __device__ __forceinline__ double qdiv<double>(double a, double b) { return a / b; }
+template <> __device__ __forceinline__ float qdiv<float>(float a, float b) { return __fdividef(a, b); }
+template <class R> __device__ __forceinline__ R inv_sqrt(R v);
+template <> __device__ __forceinline__ double inv_sqrt<double>(double v) { return 1 / sqrt(v); }
+template <> __device__ __forceinline__ float inv_sqrt<float>(float v) { return __frsqrt_rn(v); }
+// ds / c in seconds, accumulated into the FP64 packet time
+__device__ __forceinline__ double flight_time(double ds) { return ds / C_LIGHT; }
+__device__ __forceinline__ double flight_time(float ds) { return (double)ds * (1.0 / C_LIGHT); }
+// acc * num / den for an FP64 tally term
+__device__ __forceinline__ double mul_ratio(double acc, double num, double den) { return acc * num / den; }
+__device__ __forceinline__ double mul_ratio(double acc, float num, float den)
+{
+  return acc * (double)__fdividef(num, den);
+}
 
 // grid_1D_sphere::get_zone / grid_2D_cyln::get_zone
 template <class R>
@@ -153,11 +174,11 @@ __device__ __forceinline__ int next_zone_1d(const GridView<R>& g, const R* x, co
   const R tiny_offset = (R)(1 + 1e-6);
   if ((l_out < l_in) || (l_in < 0)) {
     ind = i + 1;
-    if (ind == g.n_zones) { ind = -2; *l = l_out / tiny_offset; }
+    if (ind == g.n_zones) { ind = -2; *l = qdiv(l_out, tiny_offset); }
     else *l = l_out * tiny_offset;
   } else {
     ind = ind_in;
-    if (ind_in == -1) *l = l_in / tiny_offset;
+    if (ind_in == -1) *l = qdiv(l_in, tiny_offset);
     else *l = l_in * tiny_offset;
   }
   return ind;
@@ -240,34 +261,35 @@ __device__ __forceinline__ float floor_boundary_step(const Pkt<float>& p, float 
 template <class R>
 __device__ __forceinline__ R dshift_lab_to_comoving(const GridView<R>& g, Pkt<R>& p)
 {
-  R v0 = p.x[0] / g.t_now, v1 = p.x[1] / g.t_now, v2 = p.x[2] / g.t_now;
+  R v0 = qdiv(p.x[0], g.t_now), v1 = qdiv(p.x[1], g.t_now), v2 = qdiv(p.x[2], g.t_now);
   const R c = (R)C_LIGHT;
-  R beta2 = (v0 * v0 + v1 * v1 + v2 * v2) / c / c;
+  R beta2 = qdiv(qdiv(v0 * v0 + v1 * v1 + v2 * v2, c), c);
   R vdd = v0 * p.D[0] + v1 * p.D[1] + v2 * p.D[2];
-  R gamma = 1 / rsqrt_(1 - beta2);
-  p.dvds = 1 / g.t_now;
-  return gamma * (1 - vdd / c);
+  R gamma = inv_sqrt(1 - beta2);
+  p.dvds = qdiv((R)1, g.t_now);
+  return gamma * (1 - qdiv(vdd, c));
 }
 
 // transport::lorentz_transform (relativity.cpp:84-127); D divided by |D|^2
 template <class R>
 __device__ __forceinline__ void lorentz_transform(const GridView<R>& g, Pkt<R>& p, int tolab)
 {
-  R v[3] = {p.x[0] / g.t_now, p.x[1] / g.t_now, p.x[2] / g.t_now};
-  p.dvds = 1 / g.t_now;
+  R v[3] = {qdiv(p.x[0], g.t_now), qdiv(p.x[1], g.t_now), qdiv(p.x[2], g.t_now)};
+  p.dvds = qdiv((R)1, g.t_now);
   if (tolab) { v[0] *= -1; v[1] *= -1; v[2] *= -1; }
   const R c = (R)C_LIGHT;
-  R beta2 = (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / c / c;
+  R beta2 = qdiv(qdiv(v[0] * v[0] + v[1] * v[1] + v[2] * v[2], c), c);
   R vdd = v[0] * p.D[0] + v[1] * p.D[1] + v[2] * p.D[2];
-  R gamma = 1 / rsqrt_(1 - beta2);
-  R dshift = gamma * (1 - vdd / c);
+  R gamma = inv_sqrt(1 - beta2);
+  R dshift = gamma * (1 - qdiv(vdd, c));
   p.e *= (double)dshift;
   p.nu *= dshift;
 #pragma unroll
   for (int k = 0; k < 3; k++)
-    p.D[k] = 1 / dshift * (p.D[k] - gamma * v[k] / c * (1 - gamma * vdd / c / (gamma + 1)));
+    p.D[k] = qdiv((R)1, dshift) *
+             (p.D[k] - qdiv(gamma * v[k], c) * (1 - qdiv(qdiv(gamma * vdd, c), gamma + 1)));
   R norm = p.D[0] * p.D[0] + p.D[1] * p.D[1] + p.D[2] * p.D[2];
-  p.D[0] /= norm; p.D[1] /= norm; p.D[2] /= norm;
+  p.D[0] = qdiv(p.D[0], norm); p.D[1] = qdiv(p.D[1], norm); p.D[2] = qdiv(p.D[2], norm);
 }
 
 template <class R>

@@ -39,11 +39,17 @@ struct Tab<double> {
   const double* cdf;
 };
 
+// one column of a zone's Walker alias table: column k keeps bin k with
+// probability q, else gives bin a
+struct __align__(8) AliasEntry {
+  float q;
+  int32_t a;
+};
+
 template <>
 struct Tab<float> {
-  const float2* ke;     // (kappa = abs + scat, eps = abs / kappa)
-  const float* cdf;
-  const int32_t* guide; // [nz][n_nu]: guide[z][k] = upper_bound(cdf_z, k / n_nu)
+  const float2* ke;          // (kappa = abs + scat, eps = abs / kappa)
+  const AliasEntry* alias;   // [nz][n_nu] alias tables of the emissivity distributions
 };
 
 template <class R>
@@ -98,6 +104,10 @@ struct ZoneSink {
   double* J;
   int64_t jkey;
   double jacc;
+  // e_abs of an event is (its absorbed energy) * eps_imc[zone]: applied per
+  // event in the direct form (the reference's operation order), once per
+  // zone visit at the flush in the accumulating form
+  const double* eps_imc;
   __device__ __forceinline__ void add(int field, int zone_, double v) const
   {
     if (lds) atomicAdd(&dyn_lds[field * nz + zone_], v);
@@ -116,6 +126,7 @@ struct ZoneSink {
   __device__ __forceinline__ void flush()
   {
     if (zone >= 0) {
+      acc[T_EABS] *= eps_imc[zone];
 #pragma unroll
       for (int f = 0; f < 5; f++) {
         if (acc[f] != 0) add(f, zone, acc[f]);
@@ -131,7 +142,7 @@ struct ZoneSink {
                                         double fr)
   {
     if (!accum) {
-      if (photon) add(T_EABS, z, eabs);
+      if (photon) add(T_EABS, z, eabs * eps_imc[z]);
       add(T_FX, z, fx);
       add(T_FY, z, fy);
       add(T_FZ, z, fz);
@@ -205,20 +216,19 @@ __device__ __forceinline__ void sample_photon_frequency(const PropArgs<double>& 
   }
 }
 
-// FP32 form: the same upper_bound(cdf, u) through a guide table (Chen & Asau
-// 1974): guide[k] = upper_bound(cdf, k/G), and every entry before it is
-// <= k/G <= u, so a short forward scan from guide[floor(u G)] lands on the
-// same bin as the binary search, in one or two loads instead of log2(n_nu)
+// FP32 form: the same distribution over bins -- p_k = cdf_k - cdf_{k-1}, the
+// mass upper_bound(cdf, u) gives bin k -- sampled with a Walker alias table
+// (Walker 1977; Vose 1991): a uniform column k and a uniform coin against q_k,
+// one 8-byte load instead of a binary search over log2(n_nu) dependent loads
 __device__ __forceinline__ void sample_photon_frequency(const PropArgs<float>& A, Pkt<float>& p, Counts& c)
 {
   if (p.type == SGPU_PHOTON) {
     const int nn = A.nu.n;
-    const float u = p.rng.uniform_f();
-    const size_t base = (size_t)p.ind * nn;
-    int k = min((int)(u * (float)nn), nn - 1);
-    int inu = A.tab.guide[base + k];
-    const float* cdf = A.tab.cdf + base;
-    while (inu < nn - 1 && cdf[inu] <= u) ++inu;
+    const float u_col = p.rng.uniform_f();
+    const float u_coin = p.rng.uniform_f();
+    const int k = min((int)(u_col * (float)nn), nn - 1);
+    const AliasEntry e = A.tab.alias[(size_t)p.ind * nn + k];
+    const int inu = u_coin < e.q ? k : e.a;
     p.nu = ax_sample(A.nu, inu, p.rng.uniform_f());
     c.redist++;
   } else {
@@ -232,11 +242,11 @@ __device__ __forceinline__ void isotropic_scatter(const PropArgs<R>& A, Pkt<R>& 
 {
   const GridView<R>& g = A.g;
   const R cl = (R)C_LIGHT;
-  R V[3] = {p.x[0] / g.t_now, p.x[1] / g.t_now, p.x[2] / g.t_now};
+  R V[3] = {qdiv(p.x[0], g.t_now), qdiv(p.x[1], g.t_now), qdiv(p.x[2], g.t_now)};
   R vdotD = V[0] * p.D[0] + V[1] * p.D[1] + V[2] * p.D[2];
-  R beta2 = (V[0] * V[0] + V[1] * V[1] + V[2] * V[2]) / cl / cl;
-  R gamma = 1 / rsqrt_(1 - beta2);
-  R dshift_in = gamma * (1 - vdotD / cl);
+  R beta2 = qdiv(qdiv(V[0] * V[0] + V[1] * V[1] + V[2] * V[2], cl), cl);
+  R gamma = inv_sqrt(1 - beta2);
+  R dshift_in = gamma * (1 - qdiv(vdotD, cl));
   p.e = p.e * (double)dshift_in;
   p.nu = p.nu * dshift_in;
   R Dn[3];
@@ -244,12 +254,13 @@ __device__ __forceinline__ void isotropic_scatter(const PropArgs<R>& A, Pkt<R>& 
   if (redist) sample_photon_frequency(A, p, c);
   for (int i = 0; i < 3; i++) V[i] = -1 * V[i];
   R vdp = (Dn[0] * V[0] + Dn[1] * V[1] + Dn[2] * V[2]);
-  R vd_out = gamma * (1 - vdp / cl);
+  R vd_out = gamma * (1 - qdiv(vdp, cl));
 #pragma unroll
   for (int k = 0; k < 3; k++)
-    p.D[k] = 1 / vd_out * (Dn[k] - gamma * V[k] / cl * (1 - gamma * vdp / cl / (gamma + 1)));
+    p.D[k] = qdiv((R)1, vd_out) *
+             (Dn[k] - qdiv(gamma * V[k], cl) * (1 - qdiv(qdiv(gamma * vdp, cl), gamma + 1)));
   R norm = rsqrt_(p.D[0] * p.D[0] + p.D[1] * p.D[1] + p.D[2] * p.D[2]);
-  p.D[0] = p.D[0] / norm; p.D[1] = p.D[1] / norm; p.D[2] = p.D[2] / norm;
+  p.D[0] = qdiv(p.D[0], norm); p.D[1] = qdiv(p.D[1], norm); p.D[2] = qdiv(p.D[2], norm);
   p.e = p.e * (double)vd_out;
   p.nu = p.nu * vd_out;
 }
@@ -289,7 +300,7 @@ __device__ __forceinline__ int do_scatter(const PropArgs<R>& A, Pkt<R>& p, R eps
     if (p.rng.template u<R>() > eps) isotropic_scatter(A, p, 0, c);
     else {
       R z2 = p.rng.template u<R>();
-      if ((z2 > (R)A.eps_imc[p.ind]) || (A.p.radiative_eq)) isotropic_scatter(A, p, 1, c);
+      if (A.p.radiative_eq || (z2 > (R)A.eps_imc[p.ind])) isotropic_scatter(A, p, 1, c);
       else fate = SGPU_ABSORBED;
     }
   } else if (p.type == SGPU_GAMMARAY) {
@@ -319,14 +330,14 @@ __device__ __forceinline__ int mc_event(const PropArgs<R>& A, Pkt<R>& p, double 
   R dshift = dshift_lab_to_comoving(A.g, p);
   R kappa, eps;
   int i_nu = get_opacity(A, p, dshift, &kappa, &eps);
-  R d_nu = ax_delta(A.nu, i_nu) / p.nu * cl / p.dvds;
+  R d_nu = qdiv(qdiv(ax_delta(A.nu, i_nu), p.nu) * cl, p.dvds);
   if (p.dvds == 0) d_nu = rinf<R>();
   if (d_nu < 0) d_nu = -1 * d_nu;
   bool binshift = false;
   if (d_nu < d_bn) { d_bn = d_nu; new_ind = p.ind; binshift = true; }
   R tot_lab = kappa * dshift;
   R tau_r = -1 * rlog<R>(1 - p.rng.template u<R>());
-  R d_sc = tau_r / tot_lab;
+  R d_sc = qdiv(tau_r, tot_lab);
   if (tot_lab == 0) d_sc = rinf<R>();
   double d_tm = (tstop - p.t) * C_LIGHT;
   if (A.p.steady_state) d_tm = INFINITY;
@@ -342,13 +353,13 @@ __device__ __forceinline__ int mc_event(const PropArgs<R>& A, Pkt<R>& p, double 
   if (photon) zs.jnu((int64_t)z * (A.p.store_Jnu ? nn : 1) + (A.p.store_Jnu ? i_nu : 0), this_E);
   R rr = rsqrt_(p.x[0] * p.x[0] + p.x[1] * p.x[1] + p.x[2] * p.x[2]);
   R xdotD = p.x[0] * p.D[0] + p.x[1] * p.D[1] + p.x[2] * p.D[2];
-  zs.event(z, photon, photon ? this_E * ds * kap * (double)eps * ds * A.eps_imc[z] : 0.0,
+  zs.event(z, photon, photon ? this_E * ds * kap * (double)eps * ds : 0.0,
            this_E * ds * kap * (double)p.D[0] * ds, this_E * ds * kap * (double)p.D[1] * ds,
-           this_E * ds * kap * (double)p.D[2] * ds, this_E * ds * kap * (double)xdotD / (double)rr * ds);
+           this_E * ds * kap * (double)p.D[2] * ds, mul_ratio(this_E * ds * kap, xdotD, rr) * ds);
   p.x[0] += this_d * p.D[0];
   p.x[1] += this_d * p.D[1];
   p.x[2] += this_d * p.D[2];
-  p.t = p.t + (double)this_d / C_LIGHT;
+  p.t = p.t + flight_time(this_d);
   int fate = SGPU_MOVING;
   if (event == 1) {
     c.boundary++;
@@ -452,21 +463,42 @@ __device__ __forceinline__ void flush_counts(sgpu_counters* cnt, const Counts& c
   }
 }
 
+// copy an FP32 axis' edges into LDS at `ax` and point `dst` at them.  The
+// pointer is replaced unconditionally (an empty axis gets an empty range),
+// so every edge access provably addresses LDS and compiles to ds_read
+// rather than a flat load.
+__device__ __forceinline__ void stage_axis(Axis<float>& dst, const Axis<float>& src, float*& ax)
+{
+  const int n = (src.n > 0 && src.x) ? src.n : 0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) ax[k] = src.x[k];
+  dst.x = ax;
+  ax += n;
+}
+
 // PERSIST = false: one thread per packet (the plain drop-in).
 // PERSIST = true : lanes stride through the packets and refill on finish.
 // OCC: minimum waves per SIMD the register allocation must allow (1: no bound)
 template <class R, bool PERSIST, bool ACCUM, int OCC = 1>
 __global__ void __launch_bounds__(PROP_THREADS) __attribute__((amdgpu_waves_per_eu(OCC)))
-propagate_kernel(PropArgs<R> A)
+propagate_kernel(PropArgs<R> A0)
 {
+  PropArgs<R> A = A0;
   const int nz = A.g.n_zones;
   const bool use_lds = A.lds_tallies;
   ZoneSink zs = {{A.ta.e_abs, A.ta.fx_rad, A.ta.fy_rad, A.ta.fz_rad, A.ta.fr_rad, A.ta.L_radio_dep},
-                 nz, use_lds, ACCUM, -1, {0, 0, 0, 0, 0}, A.ta.J_nu, -1, 0.0};
-  if (use_lds) {
+                 nz, use_lds, ACCUM, -1, {0, 0, 0, 0, 0}, A.ta.J_nu, -1, 0.0, A.eps_imc};
+  if (use_lds)
     for (int k = threadIdx.x; k < T_NFIELDS * nz; k += blockDim.x) dyn_lds[k] = 0;
-    __syncthreads();
+  if constexpr (sizeof(R) == 4) {
+    // the FP32 form reads the frequency and zone edges from LDS: every event
+    // locates and samples its bin and measures its zone through them
+    float* ax = (float*)(dyn_lds + (use_lds ? T_NFIELDS * nz : 0));
+    stage_axis(A.nu, A0.nu, ax);
+    stage_axis(A.g.r_out, A0.g.r_out, ax);
+    stage_axis(A.g.x_out, A0.g.x_out, ax);
+    stage_axis(A.g.z_out, A0.g.z_out, ax);
   }
+  __syncthreads();
   Counts c = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   const double tstop = A.p.t_now + A.p.dt;
   const int64_t stride = PERSIST ? (int64_t)gridDim.x * blockDim.x : A.n;
@@ -508,9 +540,8 @@ propagate_kernel(PropArgs<R> A)
   }
 }
 
-// FP32 (kappa, eps) table and CDF for the mixed form
-__global__ void pack_tables_kernel(const double* abs, const double* scat, const double* cdf,
-                                   int omit_scat, size_t n, float2* ke, float* cdf32)
+// FP32 (kappa, eps) table of the mixed form
+__global__ void pack_tables_kernel(const double* abs, const double* scat, int omit_scat, size_t n, float2* ke)
 {
   size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
@@ -518,18 +549,55 @@ __global__ void pack_tables_kernel(const double* abs, const double* scat, const 
   double s = omit_scat ? 0.0 : scat[k];
   double op = a + s;
   ke[k] = make_float2((float)op, op == 0 ? 0.f : (float)(a / op));
-  cdf32[k] = (float)cdf[k];
 }
 
-// guide table of each zone's FP32 emissivity CDF: guide[z][k] =
-// upper_bound(cdf_z, k / n_nu) (first entry larger than k / n_nu)
-__global__ void guide_kernel(const float* cdf32, int nz, int nn, int32_t* guide)
+// Alias table of one zone's emissivity distribution (Vose 1991), in FP64,
+// one workgroup per zone with the scaled probabilities and the two work
+// stacks in LDS (nn <= ALIAS_LDS_MAX); lane 0 pairs the columns.  The
+// distribution is the one upper_bound over the normalised CDF samples,
+// including the last bin taking the remainder when cdf[nn-1] < 1 (the
+// min(upper_bound, nn - 1) clamp); a zone with no emission sends every
+// column to the last bin, as upper_bound on an all-zero CDF does.
+constexpr int ALIAS_LDS_MAX = 4096;
+
+__global__ void __launch_bounds__(64) alias_kernel(const double* cdf, int nn, AliasEntry* tab)
 {
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)nz * nn) return;
-  const int z = (int)(t / nn), k = (int)(t - (size_t)z * nn);
-  const float v = (float)k / (float)nn;
-  guide[t] = min(upper_bound_dev(cdf32 + (size_t)z * nn, nn, v), nn - 1);
+  extern __shared__ double alds[];
+  double* sc = alds;                              // scaled probabilities
+  int32_t* small = (int32_t*)(alds + nn);
+  int32_t* large = small + nn;
+  const int z = blockIdx.x;
+  const double* c = cdf + (size_t)z * nn;
+  AliasEntry* t = tab + (size_t)z * nn;
+  double part = 0;
+  for (int k = threadIdx.x; k < nn; k += blockDim.x) {
+    double lo = k == 0 ? 0.0 : c[k - 1];
+    double pk = (k == nn - 1 ? 1.0 : c[k]) - lo;
+    if (!(pk > 0)) pk = 0;   // also a NaN CDF
+    sc[k] = pk;
+    part += pk;
+  }
+  const double tot = wave_sum(part);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  if (!(tot > 0)) {
+    for (int k = 0; k < nn; k++) t[k] = AliasEntry{0.f, nn - 1};
+    return;
+  }
+  int ns = 0, nl = 0;
+  const double scale = (double)nn / tot;
+  for (int k = 0; k < nn; k++) {
+    sc[k] *= scale;
+    if (sc[k] < 1.0) small[ns++] = k; else large[nl++] = k;
+  }
+  while (ns > 0 && nl > 0) {
+    const int l = small[--ns], g = large[--nl];
+    t[l] = AliasEntry{(float)sc[l], g};
+    sc[g] = (sc[g] + sc[l]) - 1.0;
+    if (sc[g] < 1.0) small[ns++] = g; else large[nl++] = g;
+  }
+  while (nl > 0) { const int g = large[--nl]; t[g] = AliasEntry{1.f, g}; }
+  while (ns > 0) { const int l = small[--ns]; t[l] = AliasEntry{1.f, l}; }
 }
 
 __global__ void to_f32_kernel(const double* src, float* dst, int n)
@@ -932,17 +1000,17 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
     const size_t ntab = (size_t)nz * nn;
     float2* ke = (float2*)scratch(ctx, "prop_ke", sizeof(float2) * ntab, &st);
     if (st) return st;
-    float* cdf32 = (float*)scratch(ctx, "prop_cdf32", sizeof(float) * ntab, &st);
+    SGPU_REQUIRE(nn <= ALIAS_LDS_MAX, "the mixed form samples frequencies through alias tables built "
+                 "in LDS: n_nu = %d exceeds %d", nn, ALIAS_LDS_MAX);
+    AliasEntry* alias = (AliasEntry*)scratch(ctx, "prop_alias", sizeof(AliasEntry) * ntab, &st);
     if (st) return st;
     const int n_edges = nn + grid->r_out.n + grid->x_out.n + grid->z_out.n;
     float* e32 = (float*)scratch(ctx, "prop_edges32", sizeof(float) * n_edges, &st);
     if (st) return st;
-    int32_t* guide = (int32_t*)scratch(ctx, "prop_guide", sizeof(int32_t) * ntab, &st);
-    if (st) return st;
     hipLaunchKernelGGL(pack_tables_kernel, dim3(grid_blocks((int64_t)ntab, 256)), dim3(256), 0, ctx->stream,
-                       tables->abs, tables->scat, tables->emis_cdf, params->omit_scattering, ntab, ke, cdf32);
-    hipLaunchKernelGGL(guide_kernel, dim3(grid_blocks((int64_t)ntab, 256)), dim3(256), 0, ctx->stream, cdf32, nz,
-                       nn, guide);
+                       tables->abs, tables->scat, params->omit_scattering, ntab, ke);
+    hipLaunchKernelGGL(alias_kernel, dim3(nz), dim3(64), (sizeof(double) + 2 * sizeof(int32_t)) * nn,
+                       ctx->stream, tables->emis_cdf, nn, alias);
     SGPU_LAUNCH_CHECK();
     float* nu32 = e32;
     float* r32 = nu32 + nn;
@@ -962,7 +1030,7 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
     PropArgs<float> A;
     A.g = make_grid<float>(*grid, r32, x32, z32);
     A.nu = make_axis<float>(*nu_grid, nu32);
-    A.tab = {ke, cdf32, guide};
+    A.tab = {ke, alias};
     A.eps_imc = tables->eps_imc; A.compton = tables->compton_opac; A.photoion = tables->photoion_opac;
     A.p = *params; A.pk = *packets; A.n = n; A.ta = *tallies; A.cnt = counters;
     // launch shape (tuning knobs for sweeps, defaults below):
@@ -978,7 +1046,8 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
     const int wg_per_cu = ew ? std::max(1, atoi(ew)) : 8;
     const char* el = getenv("SGPU_PROP_LDS");
     A.lds_tallies = nz <= LDS_ZONE_MAX && shape != 2 && !(el && atoi(el) == 0);
-    const size_t lds = A.lds_tallies ? sizeof(double) * 6 * nz : 0;
+    const size_t lds = (A.lds_tallies ? sizeof(double) * T_NFIELDS * nz : 0) + sizeof(float) * n_edges;
+    SGPU_REQUIRE(lds <= 96 * 1024, "mixed-form LDS (zone tallies + %d FP32 edges) exceeds 96 KiB", n_edges);
     int64_t want = (int64_t)ctx->n_cu * wg_per_cu;
     int64_t need = grid_blocks(n, PROP_THREADS);
     unsigned blocks = (unsigned)(want < need ? want : need);

@@ -148,7 +148,10 @@ def test_time_dependent_census_2d(gpu, precision):
     at the end of a step persist into the next.  FP64: census sets and
     tallies follow the oracle; FP32 geometry: census counts and deposition
     within packet noise."""
-    n_emit, steps = 2000, 3
+    # FP64 follows the oracle path by path; FP32 is compared through totals,
+    # whose packet noise (measured on the oracle with different Philox keys:
+    # 0.54 % for e_abs at 8000 packets per step) needs the larger count
+    n_emit, steps = (2000 if precision == abi.PREC_FP64 else 8000), 3
     cap = n_emit * steps
     dt = 0.5 * W.DAY
     Pg, Ph, ob = pair(gpu, cap, dt=dt)
@@ -177,4 +180,4 @@ def test_time_dependent_census_2d(gpu, precision):
             sig = np.sqrt(pg * (1 - pg) / tot + ph * (1 - ph) / tot)
             assert abs(pg - ph) <= 5 * sig + 1e-12
             g, h = Pg.get("e_abs").sum(), Ph.get("e_abs").sum()
-            assert abs(g - h) <= 0.05 * abs(h)
+            assert abs(g - h) <= 5 * 1.414 * 0.0054 * abs(h), (g, h)

@@ -182,3 +182,42 @@ def test_mixed_tangent_packet_on_shell_advances(gpu):
     assert np.all(fate == abi.ESCAPED), fate
     assert c[1] <= 100 * n
     torch.cuda.synchronize()
+
+
+def test_mixed_line_spectrum_matches_fp64(gpu):
+    """Line-expansion configuration (redistribution-dominated: every
+    absorption re-emits from the zone emissivity), FP32 form (alias-table
+    frequency sampling, FP32 geometry) against the FP64 drop-in on the same
+    GPU at equal packet counts: the escaping spectrum in 32 coarse bands
+    (chi-square with each packet weighing its energy, per-band Poisson
+    noise from the click counts) and the zone-integrated J_nu / e_abs within
+    packet noise."""
+    n = 20000
+    res = {}
+    for prec in (abi.PREC_FP64, abi.PREC_MIXED):
+        P = runtime.Problem(runtime.Space("cuda:0"), "c2_line", nz=32, n_lines=20000)
+        P.alloc_packets(n)
+        P.tparams.precision = prec
+        gpu.opacity_build(P)
+        gpu.wipe(P)
+        gpu.emit(P, n)
+        print("case line spectrum precision", prec, flush=True)
+        gpu.propagate(P, n)
+        gpu.sync()
+        res[prec] = {k: P.get(k) for k in ("opt_flux", "opt_click", "J_nu", "e_abs")}
+        res[prec]["fate"] = P.packets_numpy(n)["fate"]
+    a, b = res[abi.PREC_FP64], res[abi.PREC_MIXED]
+    assert np.all(a["fate"] == abi.ESCAPED) and np.all(b["fate"] == abi.ESCAPED)
+    fa, fb = a["opt_flux"].reshape(32, -1).sum(1), b["opt_flux"].reshape(32, -1).sum(1)
+    ca, cb = a["opt_click"].reshape(32, -1).sum(1), b["opt_click"].reshape(32, -1).sum(1)
+    live = (ca + cb) > 20
+    # each band's flux is (mean packet energy) x clicks: Poisson in the clicks
+    wa, wb = fa[live] / np.maximum(ca[live], 1), fb[live] / np.maximum(cb[live], 1)
+    sig2 = wa ** 2 * ca[live] + wb ** 2 * cb[live]
+    chi2 = float(np.sum((fa[live] - fb[live]) ** 2 / sig2))
+    dof = int(live.sum())
+    print("bands", dof, "chi2", chi2)
+    assert chi2 <= dof + 5 * np.sqrt(2 * dof), (chi2, dof)
+    for k in ("J_nu", "e_abs"):
+        g, h = a[k].sum(), b[k].sum()
+        assert abs(g - h) <= 0.03 * abs(g), (k, g, h)
