@@ -76,6 +76,7 @@ struct Rng {
     return (float)(c[0] >> 8) * (1.0f / 16777216.0f);
   }
   template <class R> __device__ __forceinline__ R u();
+  __device__ __forceinline__ void begin_event() {}
 };
 template <> __device__ __forceinline__ double Rng::u<double>() { return uniform(); }
 template <> __device__ __forceinline__ float Rng::u<float>() { return uniform_f(); }
@@ -112,6 +113,12 @@ struct RngF {
     return v;
   }
   template <class R> __device__ __forceinline__ R u() { return uniform_f(); }
+  // Start every event on a fresh block.  Events take 1 to 8 draws, so the
+  // lanes' buffers would drift out of step and some lane would refill at
+  // almost every draw site, making the whole wave run Philox there; from a
+  // common start the refills of lanes on the same path coincide.  The unused
+  // draws of a block are skipped (the stream stays disjoint and resumable).
+  __device__ __forceinline__ void begin_event() { left = 0; }
 };
 
 __device__ __forceinline__ RngF make_rngf(uint64_t seed, uint64_t id, uint32_t draw)

@@ -49,7 +49,7 @@ __device__ __forceinline__ int ax_locate(const Axis<float>& g, float xv)
   if (xv < g.min) return 0;
   int ind;
   if (g.type == SGPU_LOC_LIN) ind = (int)floorf((xv - g.min) / g.del);
-  else if (g.type == SGPU_LOC_LOG) ind = (int)floorf(__logf(__fdividef(xv, g.min)) * g.inv_log1p);
+  else if (g.type == SGPU_LOC_LOG) ind = (int)floorf(__logf(xv * __builtin_amdgcn_rcpf(g.min)) * g.inv_log1p);
   else if (g.type == SGPU_LOC_NONE) ind = 0;
   else ind = upper_bound_dev(g.x, n, xv);
   if (ind < 0) ind = 0;
@@ -115,7 +115,10 @@ template <class R> __device__ __forceinline__ R rinf() { return (R)INFINITY; }
 // statistical parity resolves, and multiplies by 1/c instead of dividing.
 template <class R> __device__ __forceinline__ R qdiv(R a, R b);
 template <> __device__ __forceinline__ double qdiv<double>(double a, double b) { return a / b; }
-template <> __device__ __forceinline__ float qdiv<float>(float a, float b) { return __fdividef(a, b); }
+template <> __device__ __forceinline__ float qdiv<float>(float a, float b)
+{
+  return a * __builtin_amdgcn_rcpf(b);   // v_rcp_f32 (1 ulp); HIP's __fdividef divides exactly
+}
 template <class R> __device__ __forceinline__ R inv_sqrt(R v);
 template <> __device__ __forceinline__ double inv_sqrt<double>(double v) { return 1 / sqrt(v); }
 template <> __device__ __forceinline__ float inv_sqrt<float>(float v) { return __frsqrt_rn(v); }
@@ -126,7 +129,7 @@ __device__ __forceinline__ double flight_time(float ds) { return (double)ds * (1
 __device__ __forceinline__ double mul_ratio(double acc, double num, double den) { return acc * num / den; }
 __device__ __forceinline__ double mul_ratio(double acc, float num, float den)
 {
-  return acc * (double)__fdividef(num, den);
+  return acc * (double)(num * __builtin_amdgcn_rcpf(den));
 }
 
 // grid_1D_sphere::get_zone / grid_2D_cyln::get_zone

@@ -324,6 +324,7 @@ __device__ __forceinline__ int mc_event(const PropArgs<R>& A, Pkt<R>& p, double 
   const int nn = A.nu.n;
   const R cl = (R)C_LIGHT;
   c.events++;
+  p.rng.begin_event();
   R d_bn = 0;
   int new_ind = get_next_zone(A.g, p.x, p.D, p.ind, &d_bn);
   d_bn = floor_boundary_step(p, d_bn);
@@ -477,12 +478,15 @@ __device__ __forceinline__ void stage_axis(Axis<float>& dst, const Axis<float>& 
 
 // PERSIST = false: one thread per packet (the plain drop-in).
 // PERSIST = true : lanes stride through the packets and refill on finish.
-// OCC: minimum waves per SIMD the register allocation must allow (1: no bound)
-template <class R, bool PERSIST, bool ACCUM, int OCC = 1>
+// OCC: minimum waves per SIMD the register allocation must allow (1: no bound).
+// KIND: the grid kind when known at compile time (0: read from the arguments);
+// a fixed kind removes the other geometry from the loop body.
+template <class R, bool PERSIST, bool ACCUM, int OCC = 1, int KIND = 0>
 __global__ void __launch_bounds__(PROP_THREADS) __attribute__((amdgpu_waves_per_eu(OCC)))
 propagate_kernel(PropArgs<R> A0)
 {
   PropArgs<R> A = A0;
+  if constexpr (KIND != 0) A.g.kind = KIND;
   const int nz = A.g.n_zones;
   const bool use_lds = A.lds_tallies;
   ZoneSink zs = {{A.ta.e_abs, A.ta.fx_rad, A.ta.fy_rad, A.ta.fz_rad, A.ta.fr_rad, A.ta.L_radio_dep},
@@ -1072,9 +1076,12 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
     else if (shape == 2)
       hipLaunchKernelGGL((propagate_kernel<float, false, false>), dim3((unsigned)need), dim3(PROP_THREADS), lds,
                          ctx->stream, A);
+    else if (grid->kind == SGPU_GRID_1D_SPHERE)
+      hipLaunchKernelGGL((propagate_kernel<float, true, true, 1, SGPU_GRID_1D_SPHERE>), dim3(blocks),
+                         dim3(PROP_THREADS), lds, ctx->stream, A);
     else
-      hipLaunchKernelGGL((propagate_kernel<float, true, true>), dim3(blocks), dim3(PROP_THREADS), lds, ctx->stream,
-                         A);
+      hipLaunchKernelGGL((propagate_kernel<float, true, true, 1, SGPU_GRID_2D_CYLN>), dim3(blocks),
+                         dim3(PROP_THREADS), lds, ctx->stream, A);
     SGPU_LAUNCH_CHECK();
   }
   timer_end(ctx, "propagate");
