@@ -68,6 +68,52 @@ struct PropArgs {
   int lds_tallies;   // zone scalars accumulated in LDS
 };
 
+// The propagation kernel's argument block in kernarg memory (it is the
+// kernel's only argument), through an opaque copy of the segment pointer:
+// the compiler cannot hoist loads through it out of the event loop, so the
+// cold fields (packet arrays, spectra, gamma-ray coefficients) are fetched
+// with scalar loads where they are used instead of being held -- and
+// spilled -- in SGPRs across the loop.
+template <class R>
+using KernargArgs = const __attribute__((address_space(4))) PropArgs<R>*;
+
+template <class R>
+__device__ __forceinline__ KernargArgs<R> cold_args()
+{
+  KernargArgs<R> p = (KernargArgs<R>)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+// member-wise copies out of kernarg memory (the implicit copy constructors
+// take generic references)
+#define SGPU_KA __attribute__((address_space(4)))
+__device__ __forceinline__ sgpu_packets ka_packets(const SGPU_KA sgpu_packets& k)
+{
+  sgpu_packets r;
+  r.capacity = k.capacity;
+  r.x = k.x; r.y = k.y; r.z = k.z; r.Dx = k.Dx; r.Dy = k.Dy; r.Dz = k.Dz;
+  r.t = k.t; r.e = k.e; r.nu = k.nu; r.ind = k.ind; r.type = k.type; r.fate = k.fate;
+  r.draw = k.draw; r.id = k.id;
+  return r;
+}
+
+__device__ __forceinline__ sgpu_locate ka_locate(const SGPU_KA sgpu_locate& l)
+{
+  sgpu_locate r;
+  r.n = l.n; r.type = l.type; r.min = l.min; r.del = l.del; r.x = l.x;
+  return r;
+}
+
+__device__ __forceinline__ sgpu_spectrum ka_spectrum(const SGPU_KA sgpu_spectrum& s)
+{
+  sgpu_spectrum r;
+  r.time_grid = ka_locate(s.time_grid); r.nu_grid = ka_locate(s.nu_grid);
+  r.mu_grid = ka_locate(s.mu_grid); r.phi_grid = ka_locate(s.phi_grid);
+  r.flux = s.flux; r.click = s.click;
+  return r;
+}
+
 // dynamic LDS of the propagation and emission kernels
 extern __shared__ double dyn_lds[];
 
@@ -176,8 +222,10 @@ __device__ __forceinline__ int get_opacity(const PropArgs<double>& A, const Pkt<
     *opac = a + s;
     *eps = (*opac == 0) ? 0 : a / *opac;
   } else {
-    double co = A.compton[p.ind] * klein_nishina(p.nu);
-    double po = A.photoion[p.ind] * pow(p.nu, -3.5);
+    const double* compton = cold_args<double>()->compton;
+    const double* photoion = cold_args<double>()->photoion;
+    double co = compton[p.ind] * klein_nishina(p.nu);
+    double po = photoion[p.ind] * pow(p.nu, -3.5);
     *opac = co + po;
     *eps = po / (co + po);
   }
@@ -195,8 +243,10 @@ __device__ __forceinline__ int get_opacity(const PropArgs<float>& A, const Pkt<f
     *opac = ke.x;
     *eps = ke.y;
   } else {
-    float co = (float)A.compton[p.ind] * klein_nishina(p.nu);
-    float po = (float)A.photoion[p.ind] * __powf(p.nu, -3.5f);
+    const double* compton = cold_args<float>()->compton;
+    const double* photoion = cold_args<float>()->photoion;
+    float co = (float)compton[p.ind] * klein_nishina(p.nu);
+    float po = (float)photoion[p.ind] * __powf(p.nu, -3.5f);
     *opac = co + po;
     *eps = po / (co + po);
   }
@@ -420,24 +470,29 @@ __device__ __forceinline__ void init_rng(RngF& r, uint64_t seed, uint64_t id, ui
 template <class R>
 __device__ __forceinline__ void load_packet(const PropArgs<R>& A, int64_t i, Pkt<R>& p)
 {
-  const sgpu_packets& k = A.pk;
+  const sgpu_packets k = ka_packets(cold_args<R>()->pk);
   p.x[0] = (R)k.x[i]; p.x[1] = (R)k.y[i]; p.x[2] = (R)k.z[i];
   p.D[0] = (R)k.Dx[i]; p.D[1] = (R)k.Dy[i]; p.D[2] = (R)k.Dz[i];
   p.t = k.t[i]; p.e = k.e[i]; p.nu = (R)k.nu[i];
   p.ind = k.ind[i]; p.type = k.type[i];
-  init_rng(p.rng, A.p.seed, k.id[i], k.draw[i]);
+  init_rng(p.rng, cold_args<R>()->p.seed, k.id[i], k.draw[i]);
   p.dvds = 0;
 }
 
 template <class R>
 __device__ __forceinline__ void finish_packet(const PropArgs<R>& A, int64_t i, Pkt<R>& p, int fate, Counts& c)
 {
-  const sgpu_packets& k = A.pk;
+  const sgpu_packets k = ka_packets(cold_args<R>()->pk);
   if (fate == SGPU_ESCAPED) {
     double x0 = p.x[0], x1 = p.x[1], x2 = p.x[2], D0 = p.D[0], D1 = p.D[1], D2 = p.D[2];
     double t_obs = p.t - (x0 * D0 + x1 * D1 + x2 * D2) / C_LIGHT;
-    if (p.type == SGPU_PHOTON) spectrum_count(A.ta.optical, t_obs, (double)p.nu, p.e, D0, D1, D2);
-    if (p.type == SGPU_GAMMARAY) spectrum_count(A.ta.gamma, t_obs, (double)p.nu, p.e, D0, D1, D2);
+    if (p.type == SGPU_PHOTON) {
+      const sgpu_spectrum sp = ka_spectrum(cold_args<R>()->ta.optical);
+      spectrum_count(sp, t_obs, (double)p.nu, p.e, D0, D1, D2);
+    } else if (p.type == SGPU_GAMMARAY) {
+      const sgpu_spectrum sp = ka_spectrum(cold_args<R>()->ta.gamma);
+      spectrum_count(sp, t_obs, (double)p.nu, p.e, D0, D1, D2);
+    }
     p.t = t_obs;
     c.escaped++;
   } else if (fate == SGPU_ABSORBED) c.absorbed++;
@@ -1041,13 +1096,14 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
     //   SGPU_PROP_SHAPE  0 persistent lanes + register tallies (default)
     //                    1 one packet per thread + register tallies
     //                    2 one packet per thread, per-event atomics
-    //   SGPU_PROP_WG_PER_CU  persistent workgroups per CU (default 8)
+    //   SGPU_PROP_WG_PER_CU  persistent workgroups per CU (default 12)
     //   SGPU_PROP_LDS    0: zone tallies as global atomics instead of LDS
-    //   SGPU_PROP_OCC    waves per SIMD the registers are bounded to (3, 4, 5)
+    //   SGPU_PROP_OCC    1: no register bound (2 waves per SIMD); default 3
+    //                    (168 VGPRs, no extra spill)
     const char* es = getenv("SGPU_PROP_SHAPE");
     const int shape = es ? atoi(es) : 0;
     const char* ew = getenv("SGPU_PROP_WG_PER_CU");
-    const int wg_per_cu = ew ? std::max(1, atoi(ew)) : 8;
+    const int wg_per_cu = ew ? std::max(1, atoi(ew)) : 12;
     const char* el = getenv("SGPU_PROP_LDS");
     A.lds_tallies = nz <= LDS_ZONE_MAX && shape != 2 && !(el && atoi(el) == 0);
     const size_t lds = (A.lds_tallies ? sizeof(double) * T_NFIELDS * nz : 0) + sizeof(float) * n_edges;
@@ -1056,27 +1112,22 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
     int64_t need = grid_blocks(n, PROP_THREADS);
     unsigned blocks = (unsigned)(want < need ? want : need);
     const char* eo = getenv("SGPU_PROP_OCC");
-    const int occ = eo ? atoi(eo) : 1;
+    const bool occ3 = !(eo && atoi(eo) == 1);
+    const bool d1 = grid->kind == SGPU_GRID_1D_SPHERE;
     timer_begin(ctx, "propagate");
-    if (shape == 0 && occ == 3)
-      hipLaunchKernelGGL((propagate_kernel<float, true, true, 3>), dim3(blocks), dim3(PROP_THREADS), lds,
-                         ctx->stream, A);
-    else if (shape == 0 && occ == 4)
-      hipLaunchKernelGGL((propagate_kernel<float, true, true, 4>), dim3(blocks), dim3(PROP_THREADS), lds,
-                         ctx->stream, A);
-    else if (shape == 0 && occ == 5)
-      hipLaunchKernelGGL((propagate_kernel<float, true, true, 5>), dim3(blocks), dim3(PROP_THREADS), lds,
-                         ctx->stream, A);
-    else if (shape == 1 && occ == 4)
-      hipLaunchKernelGGL((propagate_kernel<float, false, true, 4>), dim3((unsigned)need), dim3(PROP_THREADS), lds,
-                         ctx->stream, A);
-    else if (shape == 1)
+    if (shape == 1)
       hipLaunchKernelGGL((propagate_kernel<float, false, true>), dim3((unsigned)need), dim3(PROP_THREADS), lds,
                          ctx->stream, A);
     else if (shape == 2)
       hipLaunchKernelGGL((propagate_kernel<float, false, false>), dim3((unsigned)need), dim3(PROP_THREADS), lds,
                          ctx->stream, A);
-    else if (grid->kind == SGPU_GRID_1D_SPHERE)
+    else if (occ3 && d1)
+      hipLaunchKernelGGL((propagate_kernel<float, true, true, 3, SGPU_GRID_1D_SPHERE>), dim3(blocks),
+                         dim3(PROP_THREADS), lds, ctx->stream, A);
+    else if (occ3)
+      hipLaunchKernelGGL((propagate_kernel<float, true, true, 3, SGPU_GRID_2D_CYLN>), dim3(blocks),
+                         dim3(PROP_THREADS), lds, ctx->stream, A);
+    else if (d1)
       hipLaunchKernelGGL((propagate_kernel<float, true, true, 1, SGPU_GRID_1D_SPHERE>), dim3(blocks),
                          dim3(PROP_THREADS), lds, ctx->stream, A);
     else
