@@ -48,10 +48,12 @@ __device__ __forceinline__ void philox4x32(uint32_t c[4], uint32_t k0, uint32_t 
 {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    uint32_t lo0 = 0xD2511F53u * c[0];
-    uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
-    uint32_t lo1 = 0xCD9E8D57u * c[2];
-    uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+    // one 32x32->64 product per multiplier (v_mad_u64_u32: both halves in
+    // one quarter-rate instruction, where mul_lo + mul_hi take two)
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+    uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
     uint32_t n0 = hi1 ^ c[1] ^ k0;
     uint32_t n2 = hi0 ^ c[3] ^ k1;
     c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
