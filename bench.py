@@ -45,6 +45,8 @@ def parse():
                     help="rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the propagation kernel")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (the HBM counter calibration) and exit")
+    ap.add_argument("--nlte-only", action="store_true",
+                    help="only the batched NLTE measurement (for profiling its kernel)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-nlte", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -110,6 +112,16 @@ def main():
         sys.exit(2)
     if a.copy_calibration:
         copy_calibration()
+        return
+    if a.nlte_only:
+        import numpy as np
+        import torch
+        from pubsed_amd import abi, workloads as W
+        from pubsed_amd.runtime import GpuBackend
+        gpu = GpuBackend(0)
+        gpu.use_torch_stream()
+        gpu.enable_timing(True)
+        print(json.dumps(bench_nlte(gpu, torch, W, abi, np)))
         return
     import numpy as np
     import torch
@@ -213,7 +225,7 @@ def main():
         "per_packet": per_pkt,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "propagate_kernel<float, true, true, 1>", "kernel_ms": prop_avg,
+                     "kernel": "propagate_kernel<float, true, true, 3, 1>", "kernel_ms": prop_avg,
                      "bytes_per_packet": bpk, "traffic_source": traffic_src},
         "opacity_build": opacity,
         "nlte": nlte,
@@ -289,7 +301,7 @@ def bench_time_dependent(a, gpu):
         "mixed_over_plain": value / plain, "per_packet": per,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "propagate_kernel<float, true, true, 1>", "kernel_ms": prop_avg,
+                     "kernel": "propagate_kernel<float, true, true, 3, 1>", "kernel_ms": prop_avg,
                      "bytes_per_packet": bpk},
         "opacity_build": {"kernel_ms": float(np.mean(op_ms)), "zones": P.nz,
                           "lines": P.cfg.get("n_lines", 0)},
