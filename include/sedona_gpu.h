@@ -32,6 +32,7 @@
  *                                                             src/opacity/AtomicSpecies_state.cpp:131-196
  *   sgpu_nlte_batched       (no reference counterpart: all systems of a step in one launch)
  *   sgpu_homologous_step    hydro_homologous::step + grid expand src/hydro/hydro_homologous.cpp:27-61
+ *   sgpu_solve_eq_temperature transport::solve_eq_temperature src/transport/solve_equilibrium.cpp:66-320
  *   sgpu_nlte_set_rates     AtomicSpecies::set_rates            src/opacity/AtomicSpecies_state.cpp:294-432
  *   sgpu_nlte_populations   end of AtomicSpecies::solve_nlte    src/opacity/AtomicSpecies_state.cpp:178-193
  *   sgpu_adjoint_*          (no reference counterpart: differentiable transport)
@@ -452,6 +453,22 @@ int sgpu_nlte_autotune(sgpu_ctx* ctx, int32_t nsys, const int32_t* n,
                        const double* rates, const double* lte,
                        double* b, double* pop, int32_t* iters,
                        char* report, size_t report_len);
+
+/* ------------------------------------------------------------------ */
+/* radiative-equilibrium gas temperature                               */
+/* ------------------------------------------------------------------ */
+/* transport::solve_eq_temperature (solve_equilibrium.cpp:66-94): per zone,
+ * Brent on [temp_min, temp_max] (tolerance 1e-2 K, EPS 3e-8, at most 100
+ * iterations, then 0.0 as the reference returns) for the T at which the
+ * Kirchhoff emission sum_nu 4 pi abs_opacity[z][nu] B_nu(T) dnu (one bin:
+ * 4 pi abs sigma T^4 / pi) equals the e_abs tally -- the LTE balance with
+ * the step's opacities (solve_flag 0, :112-168).  With set_Tgas_to_Trad,
+ * T = (e_rad / a)^(1/4).  T_gas[nz] is written; n_eval[nz] (may be NULL)
+ * receives the balance evaluations of each zone. */
+int sgpu_solve_eq_temperature(sgpu_ctx* ctx, const sgpu_locate* nu_grid, int32_t n_zones,
+                              const double* abs_opacity, const double* e_abs, const double* e_rad,
+                              double temp_min, double temp_max, int32_t set_Tgas_to_Trad,
+                              double* T_gas, int32_t* n_eval);
 
 /* ------------------------------------------------------------------ */
 /* homologous expansion                                                */

@@ -87,6 +87,9 @@ int      or_opacity_build(const sgpu_locate* nu_grid, const sgpu_zone_state* zs,
                           sgpu_opacity_out* out);
 double   or_klein_nishina(double x);
 double   or_blackbody_nu(double T, double nu);
+void     or_solve_eq_temperature(const sgpu_locate* nu, int nz, const double* abs_opacity,
+                                 const double* e_abs, const double* e_rad, double tmin, double tmax,
+                                 int set_Tgas_to_Trad, double* T_gas, int* n_eval);
 
 /* ---- transport (src/transport, src/grid) ---- */
 int      or_get_zone(const sgpu_grid* g, const double* x);

@@ -68,6 +68,7 @@ def load():
         lib.or_nlte_set_rates.restype = I
         lib.or_nlte_set_rates.argtypes = [P(abi.Atoms), I, D, D, V, V, V, V, I, I, V, V]
         lib.or_nlte_populations.argtypes = [P(abi.Atoms), I, V, V, V, V]
+        lib.or_solve_eq_temperature.argtypes = [P(abi.Locate), I, V, V, V, D, D, I, V, V]
         lib.or_lu_decomp.restype = I
         lib.or_lu_decomp.argtypes = [I, V, V]
         lib.or_lu_solve.argtypes = [I, V, V, V, V]
@@ -141,6 +142,12 @@ class OracleBackend:
     def normalize(self, P):
         return self.lib.or_normalize_tallies(ctypes.byref(P.grid), ctypes.byref(P.nu),
                                              ctypes.byref(P.tparams), ctypes.byref(P.tallies))
+
+    def solve_eq_temperature(self, P, set_Tgas_to_Trad=0, n_eval=None):
+        """transport::solve_eq_temperature: P's T array from its e_abs tally."""
+        self.lib.or_solve_eq_temperature(ctypes.byref(P.nu), P.nz, ptr(P.a["abs"]), ptr(P.a["e_abs"]),
+                                         ptr(P.a["e_rad"]), P.op_params.temp_min, P.op_params.temp_max,
+                                         set_Tgas_to_Trad, ptr(P.a["T"]), ptr(n_eval))
 
     def homologous_step(self, P, dt):
         e = (P.grid.t_now + dt) / P.grid.t_now

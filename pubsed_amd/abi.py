@@ -121,7 +121,7 @@ EXPORTED_SYMBOLS = [
     "sgpu_emit_radioactive", "sgpu_propagate", "sgpu_compact", "sgpu_wipe_radiation",
     "sgpu_normalize_tallies", "sgpu_nlte_solve", "sgpu_nlte_batched", "sgpu_nlte_autotune",
     "sgpu_homologous_step", "sgpu_adjoint_forward", "sgpu_adjoint_backward",
-    "sgpu_nlte_set_rates", "sgpu_nlte_populations",
+    "sgpu_nlte_set_rates", "sgpu_nlte_populations", "sgpu_solve_eq_temperature",
 ]
 
 

@@ -239,6 +239,15 @@ int transport::step(double dt)
   if ((rc = propagate_all(dt))) return rc;
   if ((rc = clean_up_particle_vector())) return rc;
   if ((rc = reduce_radiation(dt))) return rc;
+  // transport.cpp:126-131: the radiative-equilibrium temperature for the
+  // next step (solve_equilibrium.cpp:66-94); the zone temperatures are
+  // device data of the gas state
+  if (p_.transport_radiative_equilibrium == 1 && p_.transport_fix_Tgas_during_transport == 0) {
+    rc = sgpu_solve_eq_temperature(ctx_, &nu_, nz_, out_.abs, ta_.e_abs, ta_.e_rad, p_.limits_temp_min,
+                                   p_.limits_temp_max, p_.transport_set_Tgas_to_Trad, (double*)zs_.T_gas,
+                                   nullptr);
+    if (rc < 0) return fail(rc, "solve_eq_temperature");
+  }
   if (sgpu_synchronize(ctx_) != SGPU_OK) return fail(SGPU_ERR_HIP, "synchronize");
   if (!tp_.steady_state) t_now_ += dt;
   first_step_ = 0;

@@ -20,6 +20,8 @@ struct Params {
   int n_nu = 1024;                                  // bins (del chosen to give exactly n_nu)
   int transport_steady_iterate = 1;
   int transport_radiative_equilibrium = 0;
+  int transport_fix_Tgas_during_transport = 0;
+  int transport_set_Tgas_to_Trad = 0;
   int transport_store_Jnu = 1;
   int transport_boundary_in_reflect = 0;
   int transport_boundary_out_reflect = 0;

@@ -296,7 +296,8 @@ def time_step(backend, P, dt, n_emit, scratch_pk, step_index, counters=None):
     transport::step, transport.cpp:26-140): the homologous hydro step moves
     the grid to t + dt first, then opacities, wipe, emission appended after
     the census packets, propagation to t + dt, compaction (census packets
-    persist) and normalisation; the transport clock then advances by dt.
+    persist), normalisation and, in radiative equilibrium, the gas
+    temperature for the next step; the transport clock then advances by dt.
     Returns (packets emitted, packets alive after the step, escaped)."""
     backend.homologous_step(P, dt)
     P.tparams.dt = dt
@@ -310,6 +311,8 @@ def time_step(backend, P, dt, n_emit, scratch_pk, step_index, counters=None):
     alive, esc = backend.compact(P, n, scratch_pk)
     P.n_packets = alive
     backend.normalize(P)
+    if P.cfg["radiative_eq"]:
+        backend.solve_eq_temperature(P)   # transport.cpp:126-131
     P.t_now += dt
     P.tparams.t_now = P.t_now
     P.op_params.t_now = P.t_now
@@ -391,6 +394,14 @@ class GpuBackend:
         return check(self.lib.sgpu_normalize_tallies(self.ctx, ctypes.byref(P.grid), ctypes.byref(P.nu),
                                                      ctypes.byref(P.tparams), ctypes.byref(P.tallies)),
                      "sgpu_normalize_tallies")
+
+    def solve_eq_temperature(self, P, set_Tgas_to_Trad=0, n_eval=None):
+        """transport::solve_eq_temperature: P's T array from its e_abs tally."""
+        from . import check
+        return check(self.lib.sgpu_solve_eq_temperature(
+            self.ctx, ctypes.byref(P.nu), P.nz, abi.ptr(P.a["abs"]), abi.ptr(P.a["e_abs"]),
+            abi.ptr(P.a["e_rad"]), P.op_params.temp_min, P.op_params.temp_max, set_Tgas_to_Trad,
+            abi.ptr(P.a["T"]), abi.ptr(n_eval)), "sgpu_solve_eq_temperature")
 
     def homologous_step(self, P, dt):
         """hydro_homologous::step: rho /= e^3, edges and volumes scaled by
