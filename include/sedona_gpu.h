@@ -33,6 +33,7 @@
  *   sgpu_nlte_batched       (no reference counterpart: all systems of a step in one launch)
  *   sgpu_homologous_step    hydro_homologous::step + grid expand src/hydro/hydro_homologous.cpp:27-61
  *   sgpu_solve_eq_temperature transport::solve_eq_temperature src/transport/solve_equilibrium.cpp:66-320
+ *   sgpu_nlte_radiative_rates AtomicSpecies::calculate_radiative_rates src/opacity/AtomicSpecies_state.cpp:202-287
  *   sgpu_nlte_set_rates     AtomicSpecies::set_rates            src/opacity/AtomicSpecies_state.cpp:294-432
  *   sgpu_nlte_populations   end of AtomicSpecies::solve_nlte    src/opacity/AtomicSpecies_state.cpp:178-193
  *   sgpu_adjoint_*          (no reference counterpart: differentiable transport)
@@ -436,6 +437,22 @@ int sgpu_nlte_set_rates(sgpu_ctx* ctx, const sgpu_atoms* atoms, int32_t nsys,
                         const double* lev_lte, int32_t use_collisions, int32_t no_ground_recomb,
                         const int32_t* n, const int64_t* rate_off, const int64_t* vec_off,
                         double* rates, double* lte);
+
+/* AtomicSpecies::calculate_radiative_rates (AtomicSpecies_state.cpp:202-287)
+ * for a batch of (zone, atom) systems: from the zone's mean intensity
+ * J_nu[sys_zone[s]][n_nu] (the transport tally) and gas temperature T[s],
+ * the photoionisation and recombination rate coefficients of every level
+ * with a continuum (hydrogenic cross-sections, AtomicData.cpp:713-746),
+ * summed over bins 1..n_nu-1, and each line's J-bar = the trapezoid over
+ * x in [-5, 5] (step 0.05) of J(nu_0 + x nu_D) phi_Voigt(x, A / (4 pi nu_D)),
+ * nu_D = nu_0 sqrt(2 k T / (m_p elem_A[atom])) / c (or line_velocity_width
+ * when > 0; GasState.cpp:218-220).  Outputs are indexed as
+ * sgpu_nlte_set_rates reads them: Pic / Rci [s*n_levels + level],
+ * line_J [s*n_blines + line]; only the system's atom's entries are written. */
+int sgpu_nlte_radiative_rates(sgpu_ctx* ctx, const sgpu_atoms* atoms, const sgpu_locate* nu_grid,
+                              int32_t nsys, const int32_t* sys_atom, const int32_t* sys_zone,
+                              const double* T, const int32_t* elem_A, double line_velocity_width,
+                              const double* J_nu, double* Pic, double* Rci, double* line_J);
 
 /* End of AtomicSpecies::solve_nlte (AtomicSpecies_state.cpp:178-193):
  * lev_n = max(b * lev_lte, 1e-30) per level and ion_frac[s*n_ions + ion] =

@@ -121,6 +121,11 @@ int      or_nlte_set_rates(const sgpu_atoms* at, int a, double ne, double T, con
                            int use_collisions, int no_ground_recomb, double* rates, double* lte);
 void     or_nlte_populations(const sgpu_atoms* at, int a, const double* b, const double* lte,
                              double* lev_n, double* ion_frac);
+double   or_voigt_profile(double x, double a);
+double   or_photo_cs(double E_ion, double chi_ion, double E_lev, int ion, double E_ev);
+void     or_nlte_radiative_rates(const sgpu_atoms* at, int a, const sgpu_locate* nu, const double* J_nu,
+                                 double T, int elem_A, double line_velocity_width, double* Pic,
+                                 double* Rci, double* line_J);
 int      or_lu_decomp(int n, double* A, int* perm);
 void     or_lu_solve(int n, const double* LU, const int* perm, const double* rhs, double* x);
 

@@ -180,3 +180,128 @@ void or_nlte_populations(const sgpu_atoms* at, int a, const double* b, const dou
   }
   for (int i = 0; i < n; i++) ion_frac[at->lev_ion[i + l0]] += lev_n[i];
 }
+
+/* Voigt profile approximation of VoigtProfile::getProfile
+ * (src/transport/VoigtProfile.cpp:37-55), with its own constants */
+double or_voigt_profile(double x, double a)
+{
+  double sqrt_pi = 1.77245385091;
+  double pi = 3.14159265359;
+  double xsq = x * x;
+  double c = (xsq - 0.855) / (xsq + 3.42);
+  double q;
+  if (c < 0) q = 0;
+  else {
+    double pic = 5.674 * c * c * c * c - 9.207 * c * c * c + 4.421 * c * c + 0.1117 * c;
+    q = (1 + 21 / xsq) * a / pi / (xsq + 1) * pic;
+  }
+  double H = q * sqrt_pi + exp(-xsq);
+  return H / sqrt_pi;
+}
+
+/* the hydrogenic photoionisation cross-section table of a level
+ * (AtomicData.cpp:713-746: xy_array::init(E_ion, 10 E_ion, 9 E_ion / 100),
+ * 100 points sigma = 6.3e-18 s_fac (E / E_ion)^-3, a 101st point left at 0
+ * when init makes 101), read by xy_array::value_at_with_zero_edges
+ * (xy_array.cpp:42-104: bisection locate, linear interpolation, 0 outside) */
+double or_photo_cs(double E_ion, double chi_ion, double E_lev, int ion, double E_ev)
+{
+  const int npts = 100;
+  const double fmax = 10;
+  double E_max = E_ion * fmax;
+  double dE = (E_max - E_ion) / npts;
+  int n = (int)((E_max - E_ion) / dE) + 1;
+  if (n <= 1) { n = 1; dE = E_max - E_ion; }
+  if (dE < 0) dE = 0;
+  double n_eff = pow(1 - (chi_ion - E_ion) / chi_ion, -0.5);
+  double s_fac = n_eff / (ion + 1) / (ion + 1);
+#define OR_PX(j) (E_ion + (j) * dE)
+#define OR_PY(j) ((j) < npts ? 6.3e-18 * s_fac * pow(OR_PX(j) / E_ion, -3.0) : 0.0)
+  if (E_ev < OR_PX(0)) return 0;
+  if (E_ev > OR_PX(n - 1)) return 0;
+  if (n == 1) return OR_PY(0);
+  int ind;
+  int bl = 0, bu = n - 1;
+  if (E_ev >= OR_PX(bu)) ind = n;
+  else if (E_ev <= OR_PX(bl)) ind = 0;
+  else {
+    while (bu - bl > 1) {
+      int bm = (bu + bl) / 2;
+      if (OR_PX(bm) <= E_ev) bl = bm;
+      else bu = bm;
+    }
+    ind = bl;
+  }
+  int i1, i2;
+  if (ind < n - 1) { i1 = ind; i2 = ind + 1; }
+  else { i1 = n - 2; i2 = n - 1; ind = n - 1; }   /* E == x[n-1]: the last point */
+  double slope = (OR_PY(i2) - OR_PY(i1)) / (OR_PX(i2) - OR_PX(i1));
+  return OR_PY(ind) + slope * (E_ev - OR_PX(i1));
+#undef OR_PX
+#undef OR_PY
+}
+
+/* AtomicSpecies::calculate_radiative_rates (AtomicSpecies_state.cpp:202-287)
+ * for atom a in a zone with mean intensity J_nu[nn] and gas temperature T,
+ * Doppler width from GasState::solve_state (GasState.cpp:218-220).  Pic, Rci
+ * (global level index) and line_J (global line index) receive this atom's
+ * entries. */
+void or_nlte_radiative_rates(const sgpu_atoms* at, int a, const sgpu_locate* nu, const double* J_nu,
+                             double T, int elem_A, double line_velocity_width, double* Pic, double* Rci,
+                             double* line_J)
+{
+  const int l0 = at->lev_off[a], n = at->lev_off[a + 1] - l0;
+  const int i0 = at->ion_off[a];
+  for (int j = 0; j < n; j++) { Pic[l0 + j] = 0; Rci[l0 + j] = 0; }
+  const double fac1 = 2 / OR_C / OR_C;
+  const int ng = nu->n;
+  for (int i = 1; i < ng; ++i) {
+    double nuc = or_center(nu, i);
+    double E_ergs = OR_H * nuc;
+    double E_ev = E_ergs * 624150975230.2815;
+    double J = J_nu[i];
+    double dnu = or_delta(nu, i);
+    for (int j = 0; j < n; ++j) {
+      int k = l0 + j;
+      if (continuum_level(at, a, k) == -1) continue;
+      double chi_ion = at->ion_chi[i0 + at->lev_ion[k]];
+      double chi = chi_ion - at->lev_E[k];
+      if (E_ev < chi) continue;
+      double sigma = or_photo_cs(chi, chi_ion, at->lev_E[k], at->lev_ion[k], E_ev);
+      double Jterm = sigma * J / E_ergs;
+      Pic[k] += Jterm * dnu;
+      Rci[k] += (sigma * fac1 * nuc * nuc + Jterm) * exp(-1.0 * (E_ev - chi) / OR_K_EV / T) * dnu;
+    }
+  }
+  double lam_t = sqrt(OR_H * OR_H / (2 * OR_PI * OR_M_E * OR_K * T));
+  double saha_fac = lam_t * lam_t * lam_t / 2.0;
+  for (int j = 0; j < n; ++j) {
+    int k = l0 + j;
+    int kc = continuum_level(at, a, k);
+    if (kc == -1) continue;
+    double gl_o_gc = (1.0 * at->lev_g[k]) / (1.0 * at->lev_g[kc]);
+    Pic[k] *= 4 * OR_PI;
+    Rci[k] *= 4 * OR_PI * gl_o_gc * saha_fac;
+  }
+  double vd = sqrt(2 * OR_K * T / OR_M_P / elem_A);
+  if (line_velocity_width > 0) vd = line_velocity_width;
+  double beta = vd / OR_C;
+  const double x_max = 5., dx = 0.05;
+  for (int64_t l = 0; l < at->n_blines; l++) {
+    int ll = at->bline_l[l] - l0;
+    if (ll < 0 || ll >= n) continue;
+    double nu0 = at->bline_nu[l];
+    double sum = 0, J0 = 0;
+    double nu_d = nu0 * beta;
+    double gamma = at->bline_A[l];
+    double a_voigt = gamma / 4 / OR_PI / nu_d;
+    for (double x = -1 * x_max; x <= x_max; x += dx) {
+      double phi = or_voigt_profile(x, a_voigt);
+      double nx = nu0 + x * nu_d;
+      double J1 = J_nu[or_locate_within_bounds(nu, nx)] * phi;   /* locate_array::value_at */
+      sum += 0.5 * (J1 + J0) * dx;
+      J0 = J1;
+    }
+    line_J[l] = sum;
+  }
+}

@@ -68,6 +68,11 @@ def load():
         lib.or_nlte_set_rates.restype = I
         lib.or_nlte_set_rates.argtypes = [P(abi.Atoms), I, D, D, V, V, V, V, I, I, V, V]
         lib.or_nlte_populations.argtypes = [P(abi.Atoms), I, V, V, V, V]
+        lib.or_voigt_profile.restype = D
+        lib.or_voigt_profile.argtypes = [D, D]
+        lib.or_photo_cs.restype = D
+        lib.or_photo_cs.argtypes = [D, D, D, I, D]
+        lib.or_nlte_radiative_rates.argtypes = [P(abi.Atoms), I, P(abi.Locate), V, D, I, D, V, V, V]
         lib.or_solve_eq_temperature.argtypes = [P(abi.Locate), I, V, V, V, D, D, I, V, V]
         lib.or_lu_decomp.restype = I
         lib.or_lu_decomp.argtypes = [I, V, V]

@@ -11,6 +11,9 @@
 //                       The matrices stream to HBM in the CSR layout that
 //                       sgpu_nlte_batched reads.
 //   populations_kernel  the end of AtomicSpecies::solve_nlte (:178-193).
+//   radiative_rates_kernel  AtomicSpecies::calculate_radiative_rates (:202-287):
+//                       photoionisation / recombination integrals over the
+//                       MC J_nu per level and line J-bar by the Voigt profile.
 #include <hip/hip_runtime.h>
 #include "sgpu_internal.h"
 #include "sgpu_device.h"
@@ -170,6 +173,145 @@ __global__ void __launch_bounds__(RATE_THREADS) populations_kernel(PopArgs A)
 
 __global__ void zero_status_kernel(int32_t* st) { st[0] = 0; }
 
+// VoigtProfile::getProfile (src/transport/VoigtProfile.cpp:37-55), with its
+// own constants
+__device__ inline double voigt_profile(double x, double a)
+{
+  const double sqrt_pi = 1.77245385091;
+  const double pi = 3.14159265359;
+  const double xsq = x * x;
+  const double c = (xsq - 0.855) / (xsq + 3.42);
+  double q;
+  if (c < 0) q = 0;
+  else {
+    const double pic = 5.674 * c * c * c * c - 9.207 * c * c * c + 4.421 * c * c + 0.1117 * c;
+    q = (1 + 21 / xsq) * a / pi / (xsq + 1) * pic;
+  }
+  const double H = q * sqrt_pi + exp(-xsq);
+  return H / sqrt_pi;
+}
+
+// The hydrogenic photoionisation cross-section table of a level
+// (AtomicData.cpp:713-746: nodes E_ion + j dE, dE = 9 E_ion / 100, y_j =
+// 6.3e-18 s_fac (x_j / E_ion)^-3 for the first 100 nodes, 0 for a 101st),
+// evaluated as xy_array::value_at_with_zero_edges (xy_array.cpp:42-104) on
+// nodes computed where needed
+__device__ inline double photo_cs(double E_ion, double chi_ion, int ion, double E_ev)
+{
+  const int npts = 100;
+  const double fmax = 10;
+  const double E_max = E_ion * fmax;
+  double dE = (E_max - E_ion) / npts;
+  int n = (int)((E_max - E_ion) / dE) + 1;
+  if (n <= 1) { n = 1; dE = E_max - E_ion; }
+  if (dE < 0) dE = 0;
+  const double n_eff = pow(1 - (chi_ion - E_ion) / chi_ion, -0.5);
+  const double s_fac = n_eff / (ion + 1) / (ion + 1);
+  auto px = [&](int j) { return E_ion + j * dE; };
+  auto py = [&](int j) { return j < npts ? 6.3e-18 * s_fac * pow(px(j) / E_ion, -3.0) : 0.0; };
+  if (E_ev < px(0)) return 0;
+  if (E_ev > px(n - 1)) return 0;
+  if (n == 1) return py(0);
+  int ind;
+  int bl = 0, bu = n - 1;
+  if (E_ev >= px(bu)) ind = n;
+  else if (E_ev <= px(bl)) ind = 0;
+  else {
+    while (bu - bl > 1) {
+      const int bm = (bu + bl) / 2;
+      if (px(bm) <= E_ev) bl = bm;
+      else bu = bm;
+    }
+    ind = bl;
+  }
+  int i1, i2;
+  if (ind < n - 1) { i1 = ind; i2 = ind + 1; }
+  else { i1 = n - 2; i2 = n - 1; ind = n - 1; }   // E == x[n-1]: the last node
+  const double slope = (py(i2) - py(i1)) / (px(i2) - px(i1));
+  return py(ind) + slope * (E_ev - px(i1));
+}
+
+struct RadArgs {
+  sgpu_atoms at;
+  sgpu_locate nu;
+  const int32_t* sys_atom;
+  const int32_t* sys_zone;
+  const double* T;
+  const int32_t* elem_A;
+  double line_velocity_width;
+  const double* J_nu;     // [nz][n_nu]
+  double* Pic;            // [nsys][n_levels]
+  double* Rci;
+  double* line_J;         // [nsys][n_blines]
+};
+
+// AtomicSpecies::calculate_radiative_rates (AtomicSpecies_state.cpp:202-287)
+// for one (zone, atom) system per workgroup: a thread per level sums the
+// bound-free integrand over the bins in order, a thread per line runs the
+// 201-point trapezoid over the Voigt profile
+__global__ void __launch_bounds__(RATE_THREADS) radiative_rates_kernel(RadArgs A)
+{
+  const int s = blockIdx.x;
+  const sgpu_atoms& at = A.at;
+  const int a = A.sys_atom[s];
+  const int l0 = at.lev_off[a], n = at.lev_off[a + 1] - l0;
+  const int i0 = at.ion_off[a];
+  const double T = A.T[s];
+  const int nn = A.nu.n;
+  const double* J = A.J_nu + (size_t)A.sys_zone[s] * nn;
+  double* Pic = A.Pic + (size_t)s * at.n_levels;
+  double* Rci = A.Rci + (size_t)s * at.n_levels;
+  const double fac1 = 2 / C_LIGHT / C_LIGHT;
+  const double lam_t = sqrt(H_PLANCK * H_PLANCK / (2 * PI * M_ELECTRON * K_BOLTZ * T));
+  const double saha_fac = lam_t * lam_t * lam_t / 2.0;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const int k = l0 + j;
+    const int kc = continuum_level(at, a, k);
+    double P = 0, Rr = 0;
+    if (kc != -1) {
+      const int ion = at.lev_ion[k];
+      const double chi_ion = at.ion_chi[i0 + ion];
+      const double chi = chi_ion - at.lev_E[k];
+      for (int i = 1; i < nn; ++i) {
+        const double nuc = loc_center(A.nu, i);
+        const double E_ergs = H_PLANCK * nuc;
+        const double E_ev = E_ergs * 624150975230.2815;
+        if (E_ev < chi) continue;
+        const double sigma = photo_cs(chi, chi_ion, ion, E_ev);
+        const double Jterm = sigma * J[i] / E_ergs;
+        const double dnu = loc_delta(A.nu, i);
+        P += Jterm * dnu;
+        Rr += (sigma * fac1 * nuc * nuc + Jterm) * exp(-1.0 * (E_ev - chi) / K_EV / T) * dnu;
+      }
+      const double gl_o_gc = (1.0 * at.lev_g[k]) / (1.0 * at.lev_g[kc]);
+      P *= 4 * PI;
+      Rr *= 4 * PI * gl_o_gc * saha_fac;
+    }
+    Pic[k] = P;
+    Rci[k] = Rr;
+  }
+  double vd = sqrt(2 * K_BOLTZ * T / M_PROTON / A.elem_A[a]);
+  if (A.line_velocity_width > 0) vd = A.line_velocity_width;
+  const double beta = vd / C_LIGHT;
+  const double x_max = 5., dx = 0.05;
+  double* lJ = A.line_J + (size_t)s * at.n_blines;
+  for (int64_t l = threadIdx.x; l < at.n_blines; l += blockDim.x) {
+    const int ll = at.bline_l[l] - l0;
+    if (ll < 0 || ll >= n) continue;
+    const double nu0 = at.bline_nu[l];
+    const double nu_d = nu0 * beta;
+    const double a_voigt = at.bline_A[l] / 4 / PI / nu_d;
+    double sum = 0, J0 = 0;
+    for (double x = -1 * x_max; x <= x_max; x += dx) {
+      const double phi = voigt_profile(x, a_voigt);
+      const double J1 = J[loc_within(A.nu, nu0 + x * nu_d)] * phi;   // locate_array::value_at
+      sum += 0.5 * (J1 + J0) * dx;
+      J0 = J1;
+    }
+    lJ[l] = sum;
+  }
+}
+
 }  // namespace sgpu
 
 using namespace sgpu;
@@ -222,5 +364,27 @@ extern "C" int sgpu_nlte_populations(sgpu_ctx* ctx, const sgpu_atoms* atoms, int
   PopArgs A{*atoms, sys_atom, vec_off, b, lte, lev_n, ion_frac};
   hipLaunchKernelGGL(populations_kernel, dim3(nsys), dim3(RATE_THREADS), 0, ctx->stream, A);
   SGPU_LAUNCH_CHECK();
+  return SGPU_OK;
+}
+
+extern "C" int sgpu_nlte_radiative_rates(sgpu_ctx* ctx, const sgpu_atoms* atoms, const sgpu_locate* nu_grid,
+                                         int32_t nsys, const int32_t* sys_atom, const int32_t* sys_zone,
+                                         const double* T, const int32_t* elem_A, double line_velocity_width,
+                                         const double* J_nu, double* Pic, double* Rci, double* line_J)
+{
+  SGPU_REQUIRE(ctx && atoms && nu_grid, "null argument");
+  SGPU_REQUIRE(nsys >= 0, "negative system count");
+  if (nsys == 0) return SGPU_OK;
+  SGPU_REQUIRE(sys_atom && sys_zone && T && elem_A && J_nu && Pic && Rci, "missing array");
+  SGPU_REQUIRE(nu_grid->n >= 1 && nu_grid->x, "bad frequency grid");
+  SGPU_REQUIRE(atoms->n_atoms >= 1 && atoms->lev_off && atoms->ion_off && atoms->lev_E && atoms->lev_g &&
+               atoms->lev_ion && atoms->ion_chi, "atomic level data missing");
+  SGPU_REQUIRE(atoms->n_blines == 0 || (line_J && atoms->bline_l && atoms->bline_nu && atoms->bline_A),
+               "level-resolved line arrays (and line_J) missing");
+  RadArgs A{*atoms, *nu_grid, sys_atom, sys_zone, T, elem_A, line_velocity_width, J_nu, Pic, Rci, line_J};
+  timer_begin(ctx, "nlte_radrates");
+  hipLaunchKernelGGL(radiative_rates_kernel, dim3(nsys), dim3(RATE_THREADS), 0, ctx->stream, A);
+  SGPU_LAUNCH_CHECK();
+  timer_end(ctx, "nlte_radrates");
   return SGPU_OK;
 }

@@ -255,3 +255,111 @@ def test_set_rates_solve_and_populations_near_lte_match_oracle(gpu):
         big = lv >= 1e-6 * lv.sum()
         bh, bg = b_h[vo[s]:vo[s + 1]], b_g[vo[s]:vo[s + 1]]
         assert np.max(np.abs(bg[big] - bh[big]) / np.abs(bh[big])) <= 1e-5, s
+
+
+def nlte_pass_gpu(gpu, Pg, sys_atom, sys_zone, T_s, ne_s, lte_in, J_nu):
+    """Radiative rates -> set_rates -> batched FP64 solve -> populations, on
+    the GPU, for the (zone, atom) systems given; returns (Pic, Rci, line_J,
+    b, lte) as numpy."""
+    import ctypes
+    import torch
+    from pubsed_amd import check
+    at = Pg.atoms
+    A = Pg.atoms_np
+    ns = len(sys_atom)
+    n = np.diff(A["lev_off"])[sys_atom].astype(np.int32)
+    rate_off = np.concatenate([[0], np.cumsum(n.astype(np.int64) ** 2)])
+    vec_off = np.concatenate([[0], np.cumsum(n.astype(np.int64))])
+    dev = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to("cuda:0")
+    d = {k: dev(v) for k, v in dict(sys_atom=sys_atom, sys_zone=sys_zone, T=T_s, ne=ne_s, lte_in=lte_in,
+                                      J_nu=J_nu, elem_A=Pg.ej["elem_A"].astype(np.int32), n=n,
+                                      rate_off=rate_off, vec_off=vec_off).items()}
+    z = lambda m: torch.zeros(m, dtype=torch.float64, device="cuda:0")
+    Pic, Rci, lJ = z(ns * at.n_levels), z(ns * at.n_levels), z(ns * at.n_blines)
+    R, l, b = z(int(rate_off[-1])), z(int(vec_off[-1])), z(int(vec_off[-1]))
+    p = abi.ptr
+    check(gpu.lib.sgpu_nlte_radiative_rates(gpu.ctx, ctypes.byref(at), ctypes.byref(Pg.nu), ns, p(d["sys_atom"]),
+                                            p(d["sys_zone"]), p(d["T"]), p(d["elem_A"]), 0.0, p(d["J_nu"]),
+                                            p(Pic), p(Rci), p(lJ)), "sgpu_nlte_radiative_rates")
+    check(gpu.lib.sgpu_nlte_set_rates(gpu.ctx, ctypes.byref(at), ns, p(d["sys_atom"]), p(d["ne"]), p(d["T"]),
+                                      p(lJ), p(Pic), p(Rci), p(d["lte_in"]), 1, 0, p(d["n"]), p(d["rate_off"]),
+                                      p(d["vec_off"]), p(R), p(l)), "sgpu_nlte_set_rates")
+    gpu.nlte_batched(ns, d["n"], d["rate_off"], d["vec_off"], R, l, b, None, abi.NLTE_FP64, 0.0, None)
+    gpu.sync()
+    return [x.cpu().numpy() for x in (Pic, Rci, lJ, b, l)], vec_off
+
+
+def test_radiative_rates_match_oracle(gpu):
+    """calculate_radiative_rates from a seeded J_nu (a diluted blackbody with
+    noise) for every (zone, atom) system: P_ic, R_ci and line J-bar within
+    1e-12 of the oracle (same summation order; device exp / pow within an ulp)."""
+    import ctypes
+    from pubsed_amd import runtime
+    from oracle import oracle_py
+    nz = 4
+    Ph = runtime.Problem(runtime.Space(None), "c2_line", nz=nz, n_lines=200)
+    Pg = runtime.Problem(runtime.Space("cuda:0"), "c2_line", nz=nz, n_lines=200)
+    OracleBackend().opacity_build(Ph)
+    A = Ph.atoms_np
+    na, nl, nb = Ph.atoms.n_atoms, Ph.atoms.n_levels, Ph.atoms.n_blines
+    T = Ph.get("T")
+    nuc = W.grid_center(Ph.nu_np)
+    rng = np.random.default_rng(9)
+    H, KB, C = 6.6260755e-27, 1.380658e-16, 2.99792458e10
+    with np.errstate(over="ignore"):
+        J_nu = np.array([0.5 * 2 * H * nuc ** 3 / C ** 2 / np.expm1(H * nuc / KB / (1.3 * T[z]))
+                         * rng.uniform(0.5, 1.5, len(nuc)) for z in range(nz)])
+    sys_atom = np.array([a for z in range(nz) for a in range(na)], np.int32)
+    sys_zone = np.array([z for z in range(nz) for a in range(na)], np.int32)
+    ns = len(sys_atom)
+    lib = oracle_py.load()
+    Pic_h, Rci_h, lJ_h = np.zeros((ns, nl)), np.zeros((ns, nl)), np.zeros((ns, nb))
+    for s in range(ns):
+        a, z = int(sys_atom[s]), int(sys_zone[s])
+        pi_, rc_, lj_ = np.zeros(nl), np.zeros(nl), np.zeros(nb)
+        lib.or_nlte_radiative_rates(ctypes.byref(Ph.atoms), a, ctypes.byref(Ph.nu),
+                                    np.ascontiguousarray(J_nu[z]).ctypes.data, T[z], int(Ph.ej["elem_A"][a]), 0.0,
+                                    pi_.ctypes.data, rc_.ctypes.data, lj_.ctypes.data)
+        l0, l1 = A["lev_off"][a], A["lev_off"][a + 1]
+        Pic_h[s, l0:l1], Rci_h[s, l0:l1] = pi_[l0:l1], rc_[l0:l1]
+        mine = (A["bline_l"] >= l0) & (A["bline_l"] < l1)
+        lJ_h[s, mine] = lj_[mine]
+    print("case radiative rates", ns, "systems", flush=True)
+    (Pic, Rci, lJ, _, _), _ = nlte_pass_gpu(gpu, Pg, sys_atom, sys_zone, T[sys_zone], Ph.get("n_elec")[sys_zone],
+                                            Ph.get("lev_n").reshape(nz, nl)[sys_zone], J_nu)
+    for name, g, h in (("Pic", Pic.reshape(ns, nl), Pic_h), ("Rci", Rci.reshape(ns, nl), Rci_h),
+                       ("line_J", lJ.reshape(ns, nb), lJ_h)):
+        m = h != 0
+        assert np.array_equal(g != 0, m), name
+        assert m.sum() > 0
+        e = np.max(np.abs(g[m] - h[m]) / np.abs(h[m]))
+        print(name, e)
+        assert e <= 1e-12, (name, e)
+
+
+def test_nlte_pass_in_blackbody_field_returns_lte(gpu):
+    """The whole NLTE pass on the GPU -- radiative rates from J_nu = B_nu(T),
+    set_rates with collisions, the batched solve -- gives the LTE populations
+    back (b = 1 within 2e-2 for levels holding >= 1e-6 of their atom), on a
+    grid that spans every line and photoionisation edge."""
+    from pubsed_amd import runtime
+    nz = 2
+    over = dict(nz=nz, n_lines=200, nu=(1e11, 2e17, 8192))
+    Ph = runtime.Problem(runtime.Space(None), "c2_line", **over)
+    Pg = runtime.Problem(runtime.Space("cuda:0"), "c2_line", **over)
+    OracleBackend().opacity_build(Ph)
+    na, nl = Ph.atoms.n_atoms, Ph.atoms.n_levels
+    T = Ph.get("T")
+    nuc = W.grid_center(Ph.nu_np)
+    H, KB, C = 6.6260755e-27, 1.380658e-16, 2.99792458e10
+    with np.errstate(over="ignore"):
+        J_nu = np.array([2 * H * nuc ** 3 / C ** 2 / np.expm1(H * nuc / KB / T[z]) for z in range(nz)])
+    sys_atom = np.array([a for z in range(nz) for a in range(na)], np.int32)
+    sys_zone = np.array([z for z in range(nz) for a in range(na)], np.int32)
+    print("case NLTE pass in a blackbody field", flush=True)
+    (_, _, _, b, l), vo = nlte_pass_gpu(gpu, Pg, sys_atom, sys_zone, T[sys_zone], Ph.get("n_elec")[sys_zone],
+                                        Ph.get("lev_n").reshape(nz, nl)[sys_zone], J_nu)
+    for s in range(len(sys_atom)):
+        bs, ls = b[vo[s]:vo[s + 1]], l[vo[s]:vo[s + 1]]
+        big = ls >= 1e-6 * ls.sum()
+        assert np.max(np.abs(bs[big] - 1)) < 2e-2, (s, np.max(np.abs(bs[big] - 1)))

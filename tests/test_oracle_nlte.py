@@ -110,10 +110,10 @@ import ctypes
 from pubsed_amd import runtime
 
 
-def lte_problem(nz=3, T=None):
+def lte_problem(nz=3, T=None, **over):
     """A small line-expansion problem after its LTE state solve: per zone the
     temperature, n_e and LTE level fractions (the inputs of set_rates)."""
-    P = runtime.Problem(runtime.Space(None), "c2_line", nz=nz, n_lines=200)
+    P = runtime.Problem(runtime.Space(None), "c2_line", nz=nz, n_lines=200, **over)
     if T is not None:
         P.a["T"][:] = T
     OracleBackend().opacity_build(P)
@@ -279,3 +279,90 @@ def test_populations_sum_to_ion_fractions():
         ions = A["lev_ion"][l0:l1]
         for s in range(ni):
             assert abs(fr[s] - ln[ions == s].sum()) <= 1e-15 * max(fr[s], 1e-300)
+
+
+# ---------------------------------------------------------------------------
+# radiative rates from J_nu (AtomicSpecies::calculate_radiative_rates,
+# AtomicSpecies_state.cpp:202-287; VoigtProfile.cpp:37-55; AtomicData.cpp:713-746)
+# ---------------------------------------------------------------------------
+def test_voigt_profile_limits():
+    lib = oracle_py.load()
+    x = np.linspace(-4, 4, 81)
+    g = np.array([lib.or_voigt_profile(v, 0.0) for v in x])
+    assert np.allclose(g, np.exp(-x * x) / math.sqrt(math.pi), rtol=1e-10, atol=0)   # a = 0: Doppler
+    for xv, a in ((20.0, 0.01), (35.0, 0.1), (-50.0, 0.003)):
+        lor = a / (math.pi * xv * xv)                  # Lorentz wing a / (pi x^2)
+        assert abs(lib.or_voigt_profile(xv, a) / lor - 1) < 0.03
+    # unit area (the truncated quadrature the reference uses sees 1 - erfc(5)/2 ~ 1)
+    xs = np.linspace(-60, 60, 240001)
+    area = np.trapezoid([lib.or_voigt_profile(v, 0.02) for v in xs[::40]], xs[::40])
+    assert abs(area - 1) < 0.02
+
+
+def test_photoionisation_cross_section_table():
+    lib = oracle_py.load()
+    chi_ion, E_lev, ion = 13.6, 3.4, 0
+    E_ion = chi_ion - E_lev
+    s_fac = (1 - E_lev / chi_ion) ** -0.5 / (ion + 1) ** 2
+    assert lib.or_photo_cs(E_ion, chi_ion, E_lev, ion, 0.999 * E_ion) == 0
+    assert lib.or_photo_cs(E_ion, chi_ion, E_lev, ion, 10.01 * E_ion) == 0
+    assert abs(lib.or_photo_cs(E_ion, chi_ion, E_lev, ion, E_ion) / (6.3e-18 * s_fac) - 1) < 1e-12
+    # tabulated E^-3, nodes 0.09 E_ion apart: linear interpolation lies above
+    # the law by at most h^2/8 |f''| = (0.09^2 / 8) 12 / f^2 relative
+    for f in (1.3, 2.0, 4.7, 8.0):
+        got = lib.or_photo_cs(E_ion, chi_ion, E_lev, ion, f * E_ion)
+        rel = got / (6.3e-18 * s_fac * f ** -3) - 1
+        assert -1e-12 <= rel <= 1.05 * 0.09 ** 2 / 8 * 12 / f ** 2, (f, rel)
+
+
+def rates_from_field(P, lev, z, T, ne, J_nu, vwidth=0.0):
+    """P_ic, R_ci and line J-bar of every atom of zone z from J_nu."""
+    lib = oracle_py.load()
+    A = P.atoms_np
+    nl, nb = len(A["lev_E"]), len(A["bline_nu"])
+    Pic, Rci, J = np.zeros(nl), np.zeros(nl), np.zeros(nb)
+    Jz = np.ascontiguousarray(J_nu)
+    for a in range(P.atoms.n_atoms):
+        lib.or_nlte_radiative_rates(ctypes.byref(P.atoms), a, ctypes.byref(P.nu), Jz.ctypes.data, T,
+                                    int(P.ej["elem_A"][a]), vwidth, Pic.ctypes.data, Rci.ctypes.data,
+                                    J.ctypes.data)
+    return Pic, Rci, J
+
+
+def test_milne_relation_and_lte_solution_in_blackbody_field():
+    """J_nu = B_nu(T) with LTE populations: every continuum pair obeys the
+    Milne relation P_ic n_i* = R_ci n_e n_c* (to the bin quadrature and the
+    constants' 1e-5-per-kT consistency), line J-bar = the J of the line's
+    bin (its Doppler core is far narrower than a bin), and the NLTE pass
+    built from these rates --
+    set_rates with collisions, the LU solve -- returns b = 1.  The grid
+    spans every line and photoionisation edge (1e11..2e17 Hz): J-bar of a
+    line off the grid reads the grid's end bin (locate_array::value_at
+    clamps), which is not B_nu(nu_0)."""
+    P, at, T, ne, lev = lte_problem(nz=2, nu=(1e11, 2e17, 8192))
+    A = P.atoms_np
+    nuc = W.grid_center(P.nu_np)
+    for z in range(P.nz):
+        with np.errstate(over="ignore"):
+            Jn = 2 * H * nuc ** 3 / C ** 2 / np.expm1(H * nuc / KB / T[z])
+        Pic, Rci, J = rates_from_field(P, lev, z, T[z], ne[z], Jn)
+        checked = 0
+        for k in range(len(A["lev_E"])):
+            c = continuum_of(P, k)
+            if c < 0 or Pic[k] == 0 or lev[z, k] < 1e-25 or lev[z, c] < 1e-25:
+                continue
+            lhs, rhs = Pic[k] * lev[z, k], Rci[k] * ne[z] * lev[z, c]
+            assert abs(lhs / rhs - 1) < 2e-3, (z, k, lhs, rhs)
+            checked += 1
+        assert checked > 10
+        # the Doppler core (|x| <= 5, nu_d ~ 1e-6 nu_0) sits inside the line's
+        # bin: J-bar is that bin's J times the profile's quadrature (~1)
+        nu0 = A["bline_nu"]
+        jb = np.minimum(np.searchsorted(P.nu_np["x"], nu0, side="right"), P.n_nu - 1)
+        ok = Jn[jb] > 1e-300
+        assert np.median(np.abs(J[ok] / Jn[jb][ok] - 1)) < 1e-4
+        for a in range(at.n_atoms):
+            R, l = set_rates(P, a, ne[z], T[z], J, Pic, Rci, lev[z], coll=1)
+            b = solve(R, l)
+            big = l >= 1e-6 * l.sum()
+            assert np.max(np.abs(b[big] - 1)) < 2e-2, (z, a, np.max(np.abs(b[big] - 1)))
