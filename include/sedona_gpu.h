@@ -187,6 +187,10 @@ typedef struct {
                                       (transport_opacity.cpp:109-120)            */
   int32_t use_line_expansion;      /* opacity_line_expansion: level-resolved lines
                                       (GasState::line_expansion_opacity)          */
+  int32_t use_free_free;           /* opacity_free_free (GasState::free_free_opacity) */
+  int32_t use_bound_free;          /* opacity_bound_free (GasState::bound_free_opacity,
+                                      hydrogenic cross-sections)                  */
+  int32_t no_ground_recomb;        /* opacity_no_ground_recomb                     */
 } sgpu_opacity_params;
 
 /* Opacity-build outputs.  Tables [nz][n_nu]; scalars [nz]; optional

@@ -122,6 +122,7 @@ int      or_nlte_set_rates(const sgpu_atoms* at, int a, double ne, double T, con
 void     or_nlte_populations(const sgpu_atoms* at, int a, const double* b, const double* lte,
                              double* lev_n, double* ion_frac);
 double   or_voigt_profile(double x, double a);
+int      or_continuum_level(const sgpu_atoms* at, int a, int k);
 double   or_photo_cs(double E_ion, double chi_ion, double E_lev, int ion, double E_ev);
 void     or_nlte_radiative_rates(const sgpu_atoms* at, int a, const sgpu_locate* nu, const double* J_nu,
                                  double T, int elem_A, double line_velocity_width, double* Pic,

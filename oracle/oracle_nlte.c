@@ -89,7 +89,7 @@ int or_nlte_solve(int n, const double* R, const double* lte, double* b, double* 
 
 /* continuum level of level k (global index) of atom a: the ground (first)
  * level of the next ion stage, or -1 (AtomicData.cpp:598-602) */
-static int continuum_level(const sgpu_atoms* at, int a, int k)
+int or_continuum_level(const sgpu_atoms* at, int a, int k)
 {
   for (int j = at->lev_off[a]; j < at->lev_off[a + 1]; j++)
     if (at->lev_ion[j] == at->lev_ion[k] + 1) return j;
@@ -139,7 +139,7 @@ int or_nlte_set_rates(const sgpu_atoms* at, int a, double ne, double T, const do
   /* bound-free (:386-423) */
   for (int i = 0; i < n; ++i) {
     int k = i + l0;
-    int kc = continuum_level(at, a, k);
+    int kc = or_continuum_level(at, a, k);
     if (kc == -1) continue;
     int ic = kc - l0;
     int istage = at->lev_ion[k];
@@ -263,7 +263,7 @@ void or_nlte_radiative_rates(const sgpu_atoms* at, int a, const sgpu_locate* nu,
     double dnu = or_delta(nu, i);
     for (int j = 0; j < n; ++j) {
       int k = l0 + j;
-      if (continuum_level(at, a, k) == -1) continue;
+      if (or_continuum_level(at, a, k) == -1) continue;
       double chi_ion = at->ion_chi[i0 + at->lev_ion[k]];
       double chi = chi_ion - at->lev_E[k];
       if (E_ev < chi) continue;
@@ -277,7 +277,7 @@ void or_nlte_radiative_rates(const sgpu_atoms* at, int a, const sgpu_locate* nu,
   double saha_fac = lam_t * lam_t * lam_t / 2.0;
   for (int j = 0; j < n; ++j) {
     int k = l0 + j;
-    int kc = continuum_level(at, a, k);
+    int kc = or_continuum_level(at, a, k);
     if (kc == -1) continue;
     double gl_o_gc = (1.0 * at->lev_g[k]) / (1.0 * at->lev_g[kc]);
     Pic[k] *= 4 * OR_PI;

@@ -291,6 +291,8 @@ int or_opacity_build(const sgpu_locate* nu, const sgpu_zone_state* zs,
   double* opac = (double*)malloc(sizeof(double) * nn);
   double* aopac = (double*)malloc(sizeof(double) * nn);
   double* atom_opac = (double*)malloc(sizeof(double) * nn);
+  double* emis2 = (double*)malloc(sizeof(double) * nn);
+  double* phifac = (double*)calloc(n_lev > 0 ? n_lev : 1, sizeof(double));
   int worst = 0;
 
   for (int i = 0; i < nz; i++) {
@@ -343,6 +345,76 @@ int or_opacity_build(const sgpu_locate* nu, const sgpu_zone_state* zs,
         for (int j = 0; j < nn; j++) {
           scat[j] += es;
           abs[j] += 1e-20 * p->epsilon * es;
+        }
+      }
+      /* GasState::free_free_opacity (GasState_opacities.cpp:150-182), gaunt
+       * factor 1: abs += fac nu^-3 (1 - e^{-h nu/kT}), emis += abs_ff B_nu */
+      if (p->use_free_free) {
+        double fac = 0;
+        for (int a = 0; a < at->n_atoms; a++) {
+          double Z_eff_sq = 0;
+          int i0 = at->ion_off[a], ni = at->ion_off[a + 1] - i0;
+          for (int j = 0; j < ni; j++) Z_eff_sq += ifr[i0 + j] * j * j;
+          double n_ion = mf[a] * dens / (zs->elem_A[a] * OR_M_P);
+          fac += n_ion * Z_eff_sq;
+        }
+        fac *= 3.7e8 * pow(temp, -0.5) * n_elec;
+        for (int j = 0; j < nn; j++) {
+          double nuc = or_center(nu, j);
+          double ezeta = exp(-1.0 * OR_H * nuc / OR_K / temp);
+          double bb = 2.0 * nuc * nuc * nuc * OR_H / OR_C / OR_C / (1.0 / ezeta - 1);
+          double op = fac / nuc / nuc / nuc * (1 - ezeta);
+          abs[j] += op;
+          emis[j] += op * bb;
+        }
+      }
+      /* GasState::bound_free_opacity (:352-372) with
+       * AtomicSpecies::bound_free_opacity_general, coolheat 0
+       * (AtomicSpecies_opacities.cpp:58-140): abs += sum over levels of
+       * sigma (n_l - n_c phi e^{(E_ion - E)/kT} n_e), floored at 0; emis +=
+       * n_e sum 2 h nu^3 / c^2 sigma n_c phi e^{(E_ion - E)/kT} */
+      if (p->use_bound_free) {
+        double kt_ev = OR_K_EV * temp;
+        double lam_t = sqrt(OR_H * OR_H / (2 * OR_PI * OR_M_E * OR_K * temp));
+        for (int j = 0; j < nn; j++) { opac[j] = 0; aopac[j] = 0; }   /* aopac: emissivity */
+        for (int a = 0; a < at->n_atoms; a++) {
+          int l0 = at->lev_off[a], nl = at->lev_off[a + 1] - l0, i0 = at->ion_off[a];
+          double n_dens = dens * mf[a] / (zs->elem_A[a] * OR_M_P);
+          for (int j = 0; j < nn; j++) { atom_opac[j] = 0; emis2[j] = 0; }
+          for (int l = 0; l < nl; l++) {
+            int k = l0 + l;
+            int kc = or_continuum_level(at, a, k);
+            phifac[l] = 0;
+            if (kc == -1) continue;
+            double nc = n_dens * ln[kc];
+            double gl_o_gc = (1.0 * at->lev_g[k]) / (1.0 * at->lev_g[kc]);
+            phifac[l] = nc * gl_o_gc / 2. * lam_t * lam_t * lam_t;
+          }
+          for (int j = 0; j < nn; j++) {
+            double nuc = or_center(nu, j);
+            double E = OR_H * nuc * 624150975230.2815;
+            double emis_fac = 2. * OR_H * nuc * nuc * nuc / OR_C / OR_C;
+            for (int l = 0; l < nl; l++) {
+              int k = l0 + l;
+              if (or_continuum_level(at, a, k) == -1) continue;
+              double chi_ion = at->ion_chi[i0 + at->lev_ion[k]];
+              double Eion = chi_ion - at->lev_E[k];
+              if (E < Eion) continue;
+              double zeta_net = (Eion - E) / kt_ev;
+              double ezeta_net = exp(zeta_net);
+              double sigma = or_photo_cs(Eion, chi_ion, at->lev_E[k], at->lev_ion[k], E);
+              double opac_fac = n_dens * ln[k] - phifac[l] * n_elec * ezeta_net;
+              if (opac_fac < 0) opac_fac = 0.;   /* kill maser */
+              atom_opac[j] += sigma * opac_fac;
+              if (at->lev_E[k] == 0 && p->no_ground_recomb) continue;
+              emis2[j] += emis_fac * sigma * phifac[l] * ezeta_net;
+            }
+          }
+          for (int j = 0; j < nn; j++) { opac[j] += atom_opac[j]; aopac[j] += emis2[j]; }
+        }
+        for (int j = 0; j < nn; j++) {
+          abs[j] += opac[j];
+          emis[j] += aopac[j] * n_elec;
         }
       }
       /* GasState::line_expansion_opacity (GasState_opacities.cpp:424-454) with
@@ -497,6 +569,6 @@ int or_opacity_build(const sgpu_locate* nu, const sgpu_zone_state* zs,
     }
   }
   free(X_now); free(mf); free(ip); free(ifr); free(ln);
-  free(emis); free(opac); free(aopac); free(atom_opac);
+  free(emis); free(opac); free(aopac); free(atom_opac); free(emis2); free(phifac);
   return worst;
 }

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include "sgpu_internal.h"
 #include "sgpu_device.h"
+#include "sgpu_physics.h"
 
 namespace sgpu {
 
@@ -39,15 +40,6 @@ struct RateArgs {
   double* lte;
   int32_t* status;         // set to 1 when n[s] differs from the atom's level count
 };
-
-// ground (first) level of the next ion stage of global level k of atom a, or -1
-__device__ inline int continuum_level(const sgpu_atoms& at, int a, int k)
-{
-  const int ion = at.lev_ion[k] + 1;
-  for (int j = at.lev_off[a]; j < at.lev_off[a + 1]; j++)
-    if (at.lev_ion[j] == ion) return j;
-  return -1;
-}
 
 __global__ void __launch_bounds__(RATE_THREADS) set_rates_kernel(RateArgs A)
 {
@@ -191,46 +183,6 @@ __device__ inline double voigt_profile(double x, double a)
   return H / sqrt_pi;
 }
 
-// The hydrogenic photoionisation cross-section table of a level
-// (AtomicData.cpp:713-746: nodes E_ion + j dE, dE = 9 E_ion / 100, y_j =
-// 6.3e-18 s_fac (x_j / E_ion)^-3 for the first 100 nodes, 0 for a 101st),
-// evaluated as xy_array::value_at_with_zero_edges (xy_array.cpp:42-104) on
-// nodes computed where needed
-__device__ inline double photo_cs(double E_ion, double chi_ion, int ion, double E_ev)
-{
-  const int npts = 100;
-  const double fmax = 10;
-  const double E_max = E_ion * fmax;
-  double dE = (E_max - E_ion) / npts;
-  int n = (int)((E_max - E_ion) / dE) + 1;
-  if (n <= 1) { n = 1; dE = E_max - E_ion; }
-  if (dE < 0) dE = 0;
-  const double n_eff = pow(1 - (chi_ion - E_ion) / chi_ion, -0.5);
-  const double s_fac = n_eff / (ion + 1) / (ion + 1);
-  auto px = [&](int j) { return E_ion + j * dE; };
-  auto py = [&](int j) { return j < npts ? 6.3e-18 * s_fac * pow(px(j) / E_ion, -3.0) : 0.0; };
-  if (E_ev < px(0)) return 0;
-  if (E_ev > px(n - 1)) return 0;
-  if (n == 1) return py(0);
-  int ind;
-  int bl = 0, bu = n - 1;
-  if (E_ev >= px(bu)) ind = n;
-  else if (E_ev <= px(bl)) ind = 0;
-  else {
-    while (bu - bl > 1) {
-      const int bm = (bu + bl) / 2;
-      if (px(bm) <= E_ev) bl = bm;
-      else bu = bm;
-    }
-    ind = bl;
-  }
-  int i1, i2;
-  if (ind < n - 1) { i1 = ind; i2 = ind + 1; }
-  else { i1 = n - 2; i2 = n - 1; ind = n - 1; }   // E == x[n-1]: the last node
-  const double slope = (py(i2) - py(i1)) / (px(i2) - px(i1));
-  return py(ind) + slope * (E_ev - px(i1));
-}
-
 struct RadArgs {
   sgpu_atoms at;
   sgpu_locate nu;
@@ -275,7 +227,7 @@ __global__ void __launch_bounds__(RATE_THREADS) radiative_rates_kernel(RadArgs A
       for (int i = 1; i < nn; ++i) {
         const double nuc = loc_center(A.nu, i);
         const double E_ergs = H_PLANCK * nuc;
-        const double E_ev = E_ergs * 624150975230.2815;
+        const double E_ev = E_ergs * ERGS_TO_EV;
         if (E_ev < chi) continue;
         const double sigma = photo_cs(chi, chi_ion, ion, E_ev);
         const double Jterm = sigma * J[i] / E_ergs;

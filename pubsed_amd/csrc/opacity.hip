@@ -6,10 +6,11 @@
 //                       spread over the lanes, partition functions and the
 //                       net ion fraction are wave reductions.
 //   K2 bins_kernel      one thread per (zone, frequency bin): electron
-//                       scattering, fuzz-line expansion opacity summed over
-//                       the bin's lines (lines sorted by bin, CSR index),
-//                       Planck emissivity, max-extinction clamp; writes abs,
-//                       scat and emis*dnu.
+//                       scattering, free-free, bound-free (every level with
+//                       its edge below the bin), level-resolved and fuzz-line
+//                       expansion opacity summed over the bin's lines (lines
+//                       sorted by bin, CSR index), Planck emissivity,
+//                       max-extinction clamp; writes abs, scat and emis*dnu.
 //   K3 finalize_kernel  one block per zone: emissivity CDF (block scan),
 //                       normalisation, L_thermal, Planck / Rosseland means,
 //                       gamma-ray coefficients, eps_imc.
@@ -43,6 +44,8 @@ struct StateArgs {
   sgpu_opacity_out out;
   double* levn;        // [nz][n_lev]  level populations (fractions of the atom), or NULL
   double* ndens;       // [nz][n_atoms] atom number densities, or NULL
+  double* fffac;       // [nz] free-free prefactor 3.7e8 T^-1/2 n_e sum n_ion Z^2, or NULL
+  double* phifac;      // [nz][n_lev] bound-free n_c (g_l/g_c)/2 lambda_T^3, -1 without a continuum, or NULL
 };
 
 struct ZoneLds {
@@ -142,6 +145,7 @@ __global__ void __launch_bounds__(WAVE) state_kernel(StateArgs A)
     if (lane == 0) {
       A.out.n_elec[z] = 0;
       if (A.out.zone_status) A.out.zone_status[z] = 0;
+      if (A.fffac) A.fffac[z] = 0;
     }
     return;
   }
@@ -228,6 +232,41 @@ __global__ void __launch_bounds__(WAVE) state_kernel(StateArgs A)
   if (A.ndens)
     for (int at_i = lane; at_i < at.n_atoms; at_i += WAVE)
       A.ndens[(size_t)z * at.n_atoms + at_i] = dens * L.mf[at_i] / (zs.elem_A[at_i] * M_PROTON);
+  // AtomicSpecies::bound_free_opacity_general (AtomicSpecies_opacities.cpp:94-104):
+  // n_c g_l / g_c / 2 lambda_T^3 per level, with n_c the (floored) population
+  // of the next ion's ground level
+  if (A.phifac) {
+    const double lam_t = sqrt(H_PLANCK * H_PLANCK / (2 * PI * M_ELECTRON * K_BOLTZ * T));
+    for (int at_i = 0; at_i < at.n_atoms; ++at_i) {
+      const int i0 = at.ion_off[at_i];
+      const double n_dens = dens * L.mf[at_i] / (zs.elem_A[at_i] * M_PROTON);
+      for (int l = at.lev_off[at_i] + lane; l < at.lev_off[at_i + 1]; l += WAVE) {
+        const int kc = continuum_level(at, at_i, l);
+        double ph = -1;
+        if (kc != -1) {
+          const int ion = at.lev_ion[kc];
+          double v = L.frac[i0 + ion] * (double)at.lev_g[kc] * L.bz[kc] / L.U[i0 + ion];
+          v = v < 1e-30 ? 1e-30 : v;
+          const double nc = n_dens * v;
+          const double gl_o_gc = (1.0 * at.lev_g[l]) / (1.0 * at.lev_g[kc]);
+          ph = nc * gl_o_gc / 2. * lam_t * lam_t * lam_t;
+        }
+        A.phifac[(size_t)z * A.n_lev + l] = ph;
+      }
+    }
+  }
+  // GasState::free_free_opacity (GasState_opacities.cpp:160-173)
+  if (A.fffac && lane == 0) {
+    double fac = 0;
+    for (int at_i = 0; at_i < at.n_atoms; ++at_i) {
+      double Z_eff_sq = 0;
+      const int i0 = at.ion_off[at_i], ni = at.ion_off[at_i + 1] - i0;
+      for (int j = 0; j < ni; j++) Z_eff_sq += L.frac[i0 + j] * j * j;
+      const double n_ion = L.mf[at_i] * dens / (zs.elem_A[at_i] * M_PROTON);
+      fac += n_ion * Z_eff_sq;
+    }
+    A.fffac[z] = fac * (3.7e8 * pow(T, -0.5) * root);
+  }
   if (lane == 0) {
     A.out.n_elec[z] = root;
     if (A.out.zone_status) A.out.zone_status[z] = err;
@@ -247,9 +286,11 @@ struct BinArgs {
   double* scat;
   double* ednu;    // emissivity * dnu, scanned in place by K3
   const double* n_elec;
-  const double* levn;    // [nz][n_lev] (line expansion only)
+  const double* levn;    // [nz][n_lev] (line expansion and bound-free only)
   const double* ndens;   // [nz][n_atoms]
   int n_lev;
+  const double* fffac;   // [nz] (free-free only)
+  const double* phifac;  // [nz][n_lev] (bound-free only)
 };
 
 __global__ void __launch_bounds__(256) bins_kernel(BinArgs A)
@@ -277,6 +318,52 @@ __global__ void __launch_bounds__(256) bins_kernel(BinArgs A)
       double es = THOMSON * A.n_elec[z];
       sc += es;
       ab += 1e-20 * eps * es;
+    }
+    // GasState::free_free_opacity (GasState_opacities.cpp:152-182), gaunt factor 1
+    if (A.p.use_free_free) {
+      const double fac = A.fffac[z];
+      const double ezeta = exp(-1.0 * H_PLANCK * nuc / K_BOLTZ / T);
+      const double bb = 2.0 * nuc * nuc * nuc * H_PLANCK / C_LIGHT / C_LIGHT / (1.0 / ezeta - 1);
+      const double op = fac / nuc / nuc / nuc * (1 - ezeta);
+      ab += op;
+      em += op * bb;
+    }
+    // GasState::bound_free_opacity (GasState_opacities.cpp:352-372) over
+    // AtomicSpecies::bound_free_opacity_general, coolheat 0
+    // (AtomicSpecies_opacities.cpp:106-140): every level whose edge lies below
+    // the bin centre; the emissivity sum is multiplied by n_e afterwards
+    if (A.p.use_bound_free && A.at.n_atoms > 0) {
+      const double kt_ev = K_EV * T;
+      const double E = H_PLANCK * nuc * ERGS_TO_EV;
+      const double emis_fac = 2. * H_PLANCK * nuc * nuc * nuc / C_LIGHT / C_LIGHT;
+      const double ne = A.n_elec[z];
+      const double* ln = A.levn + (size_t)z * A.n_lev;
+      const double* ph = A.phifac + (size_t)z * A.n_lev;
+      double opac = 0, emis = 0;
+      for (int a = 0; a < A.at.n_atoms; ++a) {
+        const double n_dens = A.ndens[(size_t)z * A.at.n_atoms + a];
+        const int i0 = A.at.ion_off[a];
+        double atom_opac = 0, atom_emis = 0;
+        for (int k = A.at.lev_off[a]; k < A.at.lev_off[a + 1]; ++k) {
+          const double phk = ph[k];
+          if (phk < 0) continue;             // no continuum level
+          const double E_lev = A.at.lev_E[k];
+          const double chi_ion = A.at.ion_chi[i0 + A.at.lev_ion[k]];
+          const double Eion = chi_ion - E_lev;
+          if (E < Eion) continue;
+          const double ezeta_net = exp((Eion - E) / kt_ev);
+          const double sigma = photo_cs(Eion, chi_ion, A.at.lev_ion[k], E);
+          double opac_fac = n_dens * ln[k] - phk * ne * ezeta_net;
+          if (opac_fac < 0) opac_fac = 0.;   // kill maser
+          atom_opac += sigma * opac_fac;
+          if (E_lev == 0 && A.p.no_ground_recomb) continue;
+          atom_emis += emis_fac * sigma * phk * ezeta_net;
+        }
+        opac += atom_opac;
+        emis += atom_emis;
+      }
+      ab += opac;
+      em += emis * ne;
     }
     // GasState::line_expansion_opacity (GasState_opacities.cpp:424-454),
     // AtomicSpecies::line_expansion_opacity (AtomicSpecies_opacities.cpp:329-375):
@@ -537,23 +624,35 @@ extern "C" int sgpu_opacity_build(sgpu_ctx* ctx, const sgpu_locate* nu_grid,
   int32_t* worst = (int32_t*)scratch(ctx, "op_worst", sizeof(int32_t), &st);
   if (st) return st;
   const bool lexp = params->use_line_expansion && at.n_atoms > 0 && at.n_blines > 0;
+  const bool bf = params->use_bound_free && at.n_atoms > 0;
   double* levn = nullptr;
   double* ndens = nullptr;
-  if (lexp) {
+  double* fffac = nullptr;
+  double* phifac = nullptr;
+  if (lexp)
     SGPU_REQUIRE(at.bline_l && at.bline_u && at.bline_nu && at.bline_f && at.bline_bin && at.bline_bin_off,
                  "line expansion needs the level-resolved line arrays");
+  if (lexp || bf) {
     levn = (double*)scratch(ctx, "op_levn", sizeof(double) * (size_t)nz * n_lev, &st);
     if (st) return st;
     ndens = (double*)scratch(ctx, "op_ndens", sizeof(double) * (size_t)nz * at.n_atoms, &st);
     if (st) return st;
   }
+  if (bf) {
+    phifac = (double*)scratch(ctx, "op_phifac", sizeof(double) * (size_t)nz * n_lev, &st);
+    if (st) return st;
+  }
+  if (params->use_free_free) {
+    fffac = (double*)scratch(ctx, "op_fffac", sizeof(double) * nz, &st);
+    if (st) return st;
+  }
 
   timer_begin(ctx, "opacity");
-  StateArgs sa{*zones, at, *params, n_lev, n_ion, temp_used, nion, upart, *out, levn, ndens};
+  StateArgs sa{*zones, at, *params, n_lev, n_ion, temp_used, nion, upart, *out, levn, ndens, fffac, phifac};
   hipLaunchKernelGGL(state_kernel, dim3(nz), dim3(WAVE), 0, ctx->stream, sa);
   SGPU_LAUNCH_CHECK();
   BinArgs ba{*nu_grid, *zones, at, *params, n_ion, temp_used, nion, upart,
-             out->abs, out->scat, out->emis_cdf, out->n_elec, levn, ndens, n_lev};
+             out->abs, out->scat, out->emis_cdf, out->n_elec, levn, ndens, n_lev, fffac, phifac};
   hipLaunchKernelGGL(bins_kernel, dim3((nu_grid->n + 255) / 256, nz), dim3(256), 0, ctx->stream, ba);
   SGPU_LAUNCH_CHECK();
   FinArgs fa{*nu_grid, *zones, *params, temp_used, *out};

@@ -27,6 +27,7 @@ constexpr double SIGMA_TOT = 0.0265400193567;
 constexpr double ALPHA_FS = 0.007;
 constexpr double MEV_TO_ERGS = 1.60217646e-6;
 constexpr double EV_TO_ERGS = 1.60217646e-12;
+constexpr double ERGS_TO_EV = 624150975230.2815;
 // radioactive decay constants, src/transport/radioactive.h:11-35
 constexpr double TAU_56NI = 757241.8;
 constexpr double TAU_56CO = 9627379.;

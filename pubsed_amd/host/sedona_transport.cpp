@@ -172,6 +172,9 @@ int transport::set_opacity(double dt)
   op_.use_electron_scattering = p_.opacity_electron_scattering;
   op_.use_fuzz_expansion = p_.opacity_fuzz_expansion;
   op_.use_line_expansion = p_.opacity_line_expansion;
+  op_.use_free_free = p_.opacity_free_free;
+  op_.use_bound_free = p_.opacity_bound_free;
+  op_.no_ground_recomb = p_.opacity_no_ground_recomb;
   op_.radiative_eq = p_.transport_radiative_equilibrium;
   op_.first_step = first_step_;
   int rc = sgpu_opacity_build(ctx_, &nu_, &zs_, have_atoms_ ? &at_ : nullptr, &op_, &out_);

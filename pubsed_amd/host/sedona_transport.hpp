@@ -31,6 +31,9 @@ struct Params {
   int opacity_electron_scattering = 0;
   int opacity_fuzz_expansion = 0;
   int opacity_line_expansion = 0;
+  int opacity_free_free = 0;
+  int opacity_bound_free = 0;
+  int opacity_no_ground_recomb = 0;
   double opacity_maximum_opacity = 1e40;
   double limits_temp_min = 1000;
   double limits_temp_max = 1e8;

@@ -88,3 +88,20 @@ def test_level_resolved_line_expansion_matches_oracle(gpu, fuzz):
     P0 = runtime.Problem(runtime.Space(None), "c2_line", nz=32, lexp=0, fuzz=fuzz)
     OracleBackend().opacity_build(P0)
     assert not np.allclose(P0.get("abs"), Ph.get("abs"), rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("ff,bf,lexp,ngr", [(1, 0, 0, 0), (0, 1, 0, 0), (1, 1, 1, 1)])
+def test_free_free_and_bound_free_match_oracle(gpu, ff, bf, lexp, ngr):
+    """GasState::free_free_opacity and bound_free_opacity (hydrogenic
+    cross-sections of every level with a continuum) added in the reference's
+    order, alone and together with the level-resolved lines and
+    opacity_no_ground_recomb, 32 zones."""
+    over = dict(nz=32, ff=ff, bf=bf, lexp=lexp, no_ground_recomb=ngr)
+    Pg, Ph = run_both("c2_line", gpu, **over)
+    for k in KEYS_TABLE + KEYS_ZONE:
+        e = relerr(Pg.get(k), Ph.get(k), floor=1e-300)
+        print(k, e)
+        assert e <= RTOL, (k, e)
+    P0 = runtime.Problem(runtime.Space(None), "c2_line", nz=32, lexp=lexp)
+    OracleBackend().opacity_build(P0)
+    assert not np.allclose(P0.get("abs"), Ph.get("abs"), rtol=1e-6, atol=0)

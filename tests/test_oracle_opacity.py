@@ -39,10 +39,11 @@ def test_sigma_tot_is_classical_line_constant():
 class OneZone:
     """One zone, one atom (hydrogen-like), a chosen fuzz-line list."""
 
-    def __init__(self, lines, T=1.0e4, dens=1e-13, grid=(1e14, 1e16, 400), eps=0.6, escat=0):
+    def __init__(self, lines, T=1.0e4, dens=1e-13, grid=(1e14, 1e16, 400), eps=0.6, escat=0,
+                 atom=HYD, Z=1, mass=1):
         self.nu = W.nu_log_grid(*grid)
         self.L, self.xkeep = runtime.locate_struct(runtime.Space(None), self.nu)
-        self.A, self.akeep = make_atoms([HYD])
+        self.A, self.akeep = make_atoms([atom])
         nn = self.nu["n"]
         lnu = np.array([l[0] for l in lines], float)
         gf = np.array([l[1] for l in lines], float)
@@ -60,7 +61,7 @@ class OneZone:
         A.line_nu, A.line_gf, A.line_El = lnu.ctypes.data, gf.ctypes.data, El.ctypes.data
         A.line_ion, A.line_bin, A.line_bin_off = ion.ctypes.data, bins.ctypes.data, off.ctypes.data
         self.rho = np.array([dens]); self.T = np.array([T]); self.X = np.array([1.0])
-        self.grey = np.zeros(1); self.Z = np.array([1], np.int32); self.Am = np.array([1], np.int32)
+        self.grey = np.zeros(1); self.Z = np.array([Z], np.int32); self.Am = np.array([mass], np.int32)
         self.zs = abi.ZoneState(1, 1, self.rho.ctypes.data, self.T.ctypes.data, self.X.ctypes.data, None,
                                 self.grey.ctypes.data, self.Z.ctypes.data, self.Am.ctypes.data)
         self.t = 20 * 86400.0
@@ -70,8 +71,8 @@ class OneZone:
                   "photoion_opac", "eps_imc"):
             self.o[k] = np.zeros(1)
         self.o["zone_status"] = np.zeros(1, np.int32)
-        self.o["ion_frac"] = np.zeros(A.n_ions); self.o["ion_part"] = np.zeros(A.n_ions)
-        self.o["lev_n"] = np.zeros(A.n_levels)
+        self.o["ion_frac"] = np.zeros(self.A.n_ions); self.o["ion_part"] = np.zeros(self.A.n_ions)
+        self.o["lev_n"] = np.zeros(self.A.n_levels)
         o = self.o
         self.out = abi.OpacityOut(*[o[k].ctypes.data for k in (
             "abs", "scat", "emis_cdf", "L_thermal", "planck_mean", "rosseland_mean", "n_elec",
@@ -295,3 +296,141 @@ def test_level_and_fuzz_lines_add():
     oc = zc.run()
     for k in ("abs", "scat"):
         assert np.allclose(oc[k], oa[k] + ob[k], rtol=1e-13, atol=0), k
+
+
+# ---------------------------------------------------------------------------
+# free-free (GasState::free_free_opacity, GasState_opacities.cpp:152-182) and
+# bound-free (AtomicSpecies::bound_free_opacity_general,
+# AtomicSpecies_opacities.cpp:58-140) continua
+# ---------------------------------------------------------------------------
+HEL = dict(levels=[(0.0, 1, 0), (19.82, 3, 0), (20.62, 1, 0), (0.0, 2, 1), (40.8, 8, 1), (0.0, 1, 2)],
+           chi=[24.587, 54.418, 99999.0])
+
+
+def bin_emissivity(z, o):
+    """Per-bin emissivity recovered from the normalised emission CDF and
+    L_thermal = 4 pi sum_j emis_j dnu_j (transport_opacity.cpp:152-179), with
+    the reference's pi = 3.14159."""
+    x = z.nu["x"]
+    dnu = np.diff(np.concatenate([[z.nu["min"]], x]))
+    ednu = np.diff(np.concatenate([[0.0], o["emis_cdf"]])) * o["L_thermal"][0] / (4 * PI_REF)
+    return ednu / dnu, ednu / ednu.sum()
+
+
+def centers(z):
+    x = z.nu["x"]
+    return 0.5 * (np.concatenate([[z.nu["min"]], x[:-1]]) + x)
+
+
+@pytest.mark.parametrize("T", [6.0e3, 1.5e4, 4.0e4])
+def test_free_free_is_kramers_with_unit_gaunt_factor(T):
+    """Rybicki & Lightman (5.18b) with g_ff = 1: alpha_ff = 3.7e8 T^-1/2 n_e
+    sum_ions n_ion Z^2 nu^-3 (1 - e^{-h nu/kT}); Z is the ion's charge, so
+    neutral helium adds nothing and He III counts four times He II. The
+    emissivity obeys Kirchhoff's law j = alpha B_nu."""
+    dens = 1e-13
+    z = OneZone([], T=T, dens=dens, grid=(1e12, 3e16, 600), atom=HEL, Z=2, mass=4)
+    z.p.use_free_free = 1
+    o = z.run()
+    assert o["n_elec"][0] > 0
+    nu = centers(z)
+    fr = o["ion_frac"]
+    n_he = dens / (4 * MP)
+    want = 3.7e8 / math.sqrt(T) * o["n_elec"][0] * n_he * (fr[1] + 4 * fr[2]) \
+        * nu ** -3 * -np.expm1(-H * nu / KB / T)
+    assert np.allclose(o["abs"], want, rtol=1e-12, atol=0)
+    assert np.all(o["scat"] == 0)
+    emis, w = bin_emissivity(z, o)
+    k = w > 1e-6      # a CDF step of w carries a rounding error of ~1e-16 / w
+    assert k.sum() > 100
+    assert np.allclose(emis[k] / o["abs"][k], planck(nu[k], T), rtol=1e-8, atol=0)
+
+
+@pytest.mark.parametrize("T,dens", [(8.0e3, 1e-12), (1.2e4, 1e-14)])
+def test_bound_free_obeys_kirchhoff_in_lte(T, dens):
+    """With Saha-Boltzmann populations n_l = phi_l n_c n_e e^{E_ion/kT} the
+    stimulated-recombination term reduces the extinction to
+    sum sigma n_l (1 - e^{-h nu/kT}) and the recombination emissivity to
+    sum (2 h nu^3/c^2) sigma n_l e^{-h nu/kT}, so j / alpha = B_nu exactly.
+    Both exponents are taken in eV (h nu ergs_to_ev / k_ev T), and the
+    reference's k_ev differs from k ergs_to_ev by 6.7e-6, so B_nu is written
+    with that exponent; the cgs one misses by up to 2e-4 at x = 30. The
+    populations come from the n_e solver's last evaluation, which sits within
+    its tolerance (~1e-7 here) of the n_e returned and used by the
+    stimulated-recombination term."""
+    z = OneZone([], T=T, dens=dens, grid=(1e14, 1e16, 800))
+    z.p.use_bound_free = 1
+    o = z.run()
+    emis, w = bin_emissivity(z, o)
+    nu = centers(z)
+    k = (w > 1e-6) & (o["abs"] > 0)
+    assert k.sum() > 100
+    x_ev = H * nu[k] * 624150975230.2815 / (KEV * T)
+    B_ev = 2 * H * nu[k] ** 3 / C ** 2 / np.expm1(x_ev)
+    assert np.allclose(emis[k] / o["abs"][k], B_ev, rtol=1e-6, atol=0)
+    # and the cgs exponent fails by far more: the pin sees the constants
+    assert not np.allclose(emis[k] / o["abs"][k], planck(nu[k], T), rtol=1e-5, atol=0)
+
+
+def test_bound_free_thresholds_and_cross_sections():
+    """alpha_bf(nu) = sum_{l: h nu >= E_ion,l} sigma_l(nu) n_l (1 - e^{-h nu/kT}):
+    zero below the lowest edge (n=3 of hydrogen, 1.51 eV), a jump at each
+    edge (3.4 eV, 13.6 eV) and the hydrogenic cross-sections in between."""
+    T, dens = 1.0e4, 1e-13
+    z = OneZone([], T=T, dens=dens, grid=(1e14, 1e16, 800))
+    z.p.use_bound_free = 1
+    o = z.run()
+    nu = centers(z)
+    E = H * nu * 624150975230.2815
+    n_dens = dens / MP
+    lib = oracle_py.load()
+    lev = [(0, 0.0, 13.6), (1, 10.2, 13.6 - 10.2), (2, 12.09, 13.6 - 12.09)]
+    want = np.zeros_like(nu)
+    for l, El, Eion in lev:
+        for j in np.nonzero(E >= Eion)[0]:
+            sig = lib.or_photo_cs(Eion, 13.6, El, 0, float(E[j]))
+            want[j] += sig * n_dens * o["lev_n"][l] * -math.expm1(-E[j] / (KEV * T))
+    assert np.all(o["abs"][E < 13.6 - 12.09] == 0)
+    k = want > 0
+    # 1e-6: n_l versus the n_e the stimulated term uses, as in the Kirchhoff pin
+    assert np.allclose(o["abs"][k], want[k], rtol=1e-6, atol=0)
+    # every edge is a rise: the newly open level adds to the total
+    for edge in (13.6 - 10.2, 13.6):
+        j = int(np.searchsorted(E, edge))
+        assert o["abs"][j] > 1.5 * o["abs"][j - 1], edge
+
+
+def test_bound_free_no_ground_recomb_drops_ground_emission_only():
+    T, dens = 1.0e4, 1e-13
+    za = OneZone([], T=T, dens=dens, grid=(1e14, 1e16, 800))
+    za.p.use_bound_free = 1
+    oa = za.run()
+    zb = OneZone([], T=T, dens=dens, grid=(1e14, 1e16, 800))
+    zb.p.use_bound_free = 1
+    zb.p.no_ground_recomb = 1
+    ob = zb.run()
+    assert np.array_equal(oa["abs"], ob["abs"])
+    ea, _ = bin_emissivity(za, oa)
+    eb, _ = bin_emissivity(zb, ob)
+    E = H * centers(za) * 624150975230.2815
+    lo = (E > 2.0) & (E < 13.0)
+    assert np.allclose(eb[lo], ea[lo], rtol=1e-9, atol=0)
+    hi = E > 13.7
+    assert np.all(eb[hi] < 0.5 * ea[hi])
+
+
+def test_continua_add_to_lines_and_scattering():
+    nu0 = 5.0e14
+    fuzz = [(nu0, 1e-3, 0.2)]
+    za = OneZone(fuzz, escat=1)
+    oa = za.run()
+    zb = OneZone([], escat=0)
+    zb.p.use_free_free = 1
+    zb.p.use_bound_free = 1
+    ob = zb.run()
+    zc = OneZone(fuzz, escat=1)
+    zc.p.use_free_free = 1
+    zc.p.use_bound_free = 1
+    oc = zc.run()
+    assert np.allclose(oc["abs"], oa["abs"] + ob["abs"], rtol=1e-13, atol=0)
+    assert np.array_equal(oc["scat"], oa["scat"])
