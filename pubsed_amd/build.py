@@ -72,6 +72,7 @@ def build_host(verbose=True):
     against the HIP library with an rpath to the package directory."""
     cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-o", DRIVER + ".tmp",
            os.path.join(HOST, "drive_grey.cpp"), os.path.join(HOST, "sedona_transport.cpp"),
+           os.path.join(HOST, "sedona_output.cpp"),
            "-L" + HERE, "-lsedona_gpu", "-Wl,-rpath,$ORIGIN/.."]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

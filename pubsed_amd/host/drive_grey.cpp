@@ -1,12 +1,15 @@
 // drive_grey.cpp -- a host C++ caller of the drop-in transport class: the
 // grey 1D 56Ni ejecta (100 shells to 2e9 cm/s at 20 d, rho ~ exp(-v/3e8)
 // normalised to 1.4 Msun, 56Ni inside 0.6 Msun, grey kappa 0.1 + Thomson),
-// one steady-state step, results printed as one JSON line.
+// one steady-state step, results printed as one JSON line; with an output
+// directory, the spectra (spectrum_0.dat, gamma_0.dat) and the zone file
+// plt_00000.dat are written there in the reference's text formats.
 //
-// usage: drive_grey [n_packets] [precision 0|1]
+// usage: drive_grey [n_packets] [precision 0|1] [output_dir]
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "sedona_transport.hpp"
@@ -16,6 +19,7 @@ int main(int argc, char** argv)
   using namespace sedona_gpu;
   const int64_t n_packets = argc > 1 ? atoll(argv[1]) : 100000;
   const int precision = argc > 2 ? atoi(argv[2]) : SGPU_PREC_MIXED;
+  const std::string out_dir = argc > 3 ? argv[3] : "";
   const double PI_REF = 3.14159, M_SUN = 1.98892e33, DAY = 86400.0;
   const double kappa_T = 0.66523e-24 * 0.5 / 1.67262158e-24;
   const int nz = 100;
@@ -54,10 +58,16 @@ int main(int argc, char** argv)
   p.particles_n_emit_radioactive = n_packets;
   p.particles_max_total = n_packets;
   p.precision = precision;
+  p.spectrum_name = out_dir.empty() ? "" : out_dir + "/spectrum";
+  p.gamma_name = out_dir.empty() ? "" : out_dir + "/gamma";
 
   transport tr;
   if (tr.init(p, g, nullptr) != SGPU_OK) { fprintf(stderr, "init: %s\n", tr.error().c_str()); return 1; }
   if (tr.step(1.0) != SGPU_OK) { fprintf(stderr, "step: %s\n", tr.error().c_str()); return 1; }
+  if (!out_dir.empty() && (tr.output_spectrum(0) != SGPU_OK || tr.write_plotfile(0, t, 1, out_dir) != SGPU_OK)) {
+    fprintf(stderr, "output: %s\n", tr.error().c_str());
+    return 1;
+  }
   double dep = 0, erad = 0, eabs = 0, spec = 0;
   auto L = tr.L_radio_dep();
   auto E = tr.e_rad();

@@ -3,6 +3,7 @@
 #include "sedona_transport.hpp"
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 
 namespace sedona_gpu {
@@ -67,6 +68,7 @@ std::vector<double> transport::download(const double* d, size_t n) const
 int transport::init(const Params& p, const Grid1D& g, const AtomicData* atoms, int device)
 {
   p_ = p;
+  g_ = g;
   ctx_ = sgpu_create(device);
   if (!ctx_) return fail(SGPU_ERR_HIP, "sgpu_create");
   nz_ = g.n_zones();
@@ -264,5 +266,40 @@ std::vector<double> transport::e_abs() const { return download(ta_.e_abs, nz_); 
 std::vector<double> transport::L_radio_dep() const { return download(ta_.L_radio_dep, nz_); }
 std::vector<double> transport::optical_spectrum() const { return download(ta_.optical.flux, nn_); }
 std::vector<double> transport::abs_opacity() const { return download(out_.abs, (size_t)nz_ * nn_); }
+std::vector<double> transport::T_gas() const { return download(zs_.T_gas, nz_); }
+std::vector<double> transport::n_elec() const { return download(out_.n_elec, nz_); }
+
+int transport::output_spectrum(int it)
+{
+  // the spectrum grids set up in init: one time bin {0, 1}, the transport nu
+  // grid, one mu bin on [-1, 1], one phi bin on [0, 2 pi]
+  HostGrid tg{0.0, {1.0}}, mg{-1.0, {1.0}}, pg{0.0, {2 * PI_REF}};
+  HostGrid ng{p_.transport_nu_grid[0], nu_edges_};
+  const std::string base = it < 0 ? std::string("_final") : "_" + std::to_string(it);
+  if (p_.spectrum_suppress_txt) return SGPU_OK;
+  if (!p_.spectrum_name.empty() &&
+      write_spectrum_dat(p_.spectrum_name + base, tg, ng, mg, pg, download(ta_.optical.flux, nn_),
+                         download(ta_.optical.click, nn_)))
+    return fail(SGPU_ERR_ARG, "output_spectrum: cannot write the optical spectrum");
+  if (!p_.gamma_name.empty() &&
+      write_spectrum_dat(p_.gamma_name + base, tg, ng, mg, pg, download(ta_.gamma.flux, nn_),
+                         download(ta_.gamma.click, nn_)))
+    return fail(SGPU_ERR_ARG, "output_spectrum: cannot write the gamma spectrum");
+  return SGPU_OK;
+}
+
+int transport::write_plotfile(int iw, double tt, int write_mass_fracs, const std::string& dir)
+{
+  char name[64];
+  snprintf(name, sizeof(name), "plt_%05d.dat", iw);
+  // homologous zones: v = r_out / t (the velocity a homologous model file holds)
+  std::vector<double> v(nz_);
+  for (int i = 0; i < nz_; ++i) v[i] = g_.r_out[i] / grid_.t_now;
+  int rc = write_plotfile_1d(dir + "/" + name, tt, g_.r_min, g_.r_out, download(d_rho_, nz_), v, T_gas(),
+                             e_rad(), n_elec(),
+                             L_radio_dep(), download(ta_.L_radio_emit, nz_),
+                             write_mass_fracs ? &g_.X : nullptr, g_.elems_Z, g_.elems_A);
+  return rc ? fail(SGPU_ERR_ARG, "write_plotfile: cannot write the zone file") : SGPU_OK;
+}
 
 }  // namespace sedona_gpu

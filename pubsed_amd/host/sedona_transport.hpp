@@ -41,7 +41,34 @@ struct Params {
   int64_t particles_max_total = 10000000;
   uint64_t transport_rng_seed = 20261203;
   int precision = SGPU_PREC_MIXED;
+  std::string spectrum_name = "spectrum";   // output file stem; "" writes nothing
+  std::string gamma_name = "";
+  int spectrum_suppress_txt = 0;
 };
+
+// a locate_array on the host: right edges x and the left edge of bin 0
+struct HostGrid {
+  double min = 0;
+  std::vector<double> x;
+  int size() const { return (int)x.size(); }
+  double center(int i) const { return 0.5 * ((i == 0 ? min : x[i - 1]) + x[i]); }
+  double delta(int i) const { return x[i] - (i == 0 ? min : x[i - 1]); }
+};
+
+// spectrum_array::print's text file <name>.dat (spectrum_array.cpp:253-311);
+// flux / click are [n_t][n_nu][n_mu][n_phi].  Returns 0, or -1 when the file
+// cannot be written.
+int write_spectrum_dat(const std::string& name, const HostGrid& time, const HostGrid& nu,
+                       const HostGrid& mu, const HostGrid& phi, const std::vector<double>& flux,
+                       const std::vector<double>& click, int n_procs = 1);
+
+// grid_1D_sphere::write_plotfile's text file (grid_1D_sphere.cpp:598-636)
+int write_plotfile_1d(const std::string& path, double t, double r_min, const std::vector<double>& r_out,
+                      const std::vector<double>& rho, const std::vector<double>& v,
+                      const std::vector<double>& T_gas, const std::vector<double>& e_rad,
+                      const std::vector<double>& n_elec, const std::vector<double>& L_dep,
+                      const std::vector<double>& L_emit, const std::vector<double>* X,
+                      const std::vector<int32_t>& elems_Z, const std::vector<int32_t>& elems_A);
 
 // 1D spherical homologous grid, host side (grid_1D_sphere's fields)
 struct Grid1D {
@@ -82,6 +109,14 @@ class transport {
   std::vector<double> L_radio_dep() const;
   std::vector<double> optical_spectrum() const; // flux per bin (raw counts of energy)
   std::vector<double> abs_opacity() const;
+  std::vector<double> T_gas() const;
+  std::vector<double> n_elec() const;
+  // transport::output_spectrum (transport_output.cpp:17-40): the optical and
+  // gamma spectra as <name>_<it>.dat (<name>_final.dat for it < 0); the
+  // HDF5 twin is not written
+  int output_spectrum(int it);
+  // grid_1D_sphere::write_plotfile's plt_<iw>.dat, under dir
+  int write_plotfile(int iw, double tt, int write_mass_fracs = 0, const std::string& dir = ".");
   int64_t n_particles() const { return n_particles_; }
   int64_t n_escaped_last() const { return n_escaped_; }
   const std::string& error() const { return err_; }
@@ -109,6 +144,7 @@ class transport {
   std::string err_;
   std::vector<void*> owned_;
   std::vector<double> nu_edges_;
+  Grid1D g_;
   sgpu_locate nu_{};
   sgpu_grid grid_{};
   sgpu_zone_state zs_{};

@@ -38,3 +38,41 @@ def test_host_driver_matches_python_path(gpu, precision):
     assert got["alive"] == 0
     assert abs(got["L_dep"] / L_dep - 1) < 1e-9
     assert abs(got["optical_flux"] / flux - 1) < 1e-9
+
+
+def test_host_driver_writes_reference_text_files(gpu, tmp_path):
+    """transport::output_spectrum and write_plotfile through the host class:
+    spectrum_0.dat / gamma_0.dat hold the Python path's escaping flux per unit
+    frequency (1024 rows: the single time, mu and phi bins add no column), and
+    plt_00000.dat the zone state."""
+    from pubsed_amd import output as O
+    n = 20000
+    print("case host driver output", flush=True)
+    r = subprocess.run([DRIVER, str(n), str(abi.PREC_FP64), str(tmp_path)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    P = runtime.Problem(runtime.Space("cuda:0"), "c1_grey")
+    P.tparams.precision = abi.PREC_FP64
+    P.alloc_packets(n)
+    gpu.opacity_build(P)
+    gpu.wipe(P)
+    gpu.emit(P, n)
+    gpu.propagate(P, n)
+    gpu.sync()
+    g = P.spec_grids["nu"]
+    x = np.asarray(g["x"]); dnu = x - np.concatenate([[g["min"]], x[:-1]])
+    for stem, key in (("spectrum_0", "opt"), ("gamma_0", "gam")):
+        counts, rows = O.read_spectrum_dat(str(tmp_path / (stem + ".dat")))
+        assert counts == (1, 1024, 1, 1) and rows.shape == (1024, 3)
+        want = P.get(key + "_flux") / dnu
+        assert np.array_equal(rows[:, 2], P.get(key + "_click"))
+        if key == "gam" and want.max() == 0:
+            assert np.all(rows[:, 1] == 0)
+            continue
+        big = want > 1e-3 * want.max()
+        assert big.sum() > 10
+        assert np.allclose(rows[big, 1], want[big], rtol=2e-5, atol=0), stem
+    plt = np.loadtxt(str(tmp_path / "plt_00000.dat"))
+    assert plt.shape == (100, 8 + 4)
+    assert np.allclose(plt[:, 3], P.get("T"), rtol=1e-8)
+    assert np.allclose(plt[:, 1], P.get("rho"), rtol=1e-8)
