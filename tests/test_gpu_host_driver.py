@@ -66,12 +66,11 @@ def test_host_driver_writes_reference_text_files(gpu, tmp_path):
         assert counts == (1, 1024, 1, 1) and rows.shape == (1024, 3)
         want = P.get(key + "_flux") / dnu
         assert np.array_equal(rows[:, 2], P.get(key + "_click"))
-        if key == "gam" and want.max() == 0:
-            assert np.all(rows[:, 1] == 0)
-            continue
-        big = want > 1e-3 * want.max()
-        assert big.sum() > 10
-        assert np.allclose(rows[big, 1], want[big], rtol=2e-5, atol=0), stem
+        hit = want > 0
+        # escaping gamma-rays carry nu in MeV, off the optical grid: bin 0
+        assert hit.sum() > (10 if key == "opt" else 0), stem
+        assert np.all(rows[~hit, 1] == 0)
+        assert np.allclose(rows[hit, 1], want[hit], rtol=2e-5, atol=0), stem
     plt = np.loadtxt(str(tmp_path / "plt_00000.dat"))
     assert plt.shape == (100, 8 + 4)
     assert np.allclose(plt[:, 3], P.get("T"), rtol=1e-8)
