@@ -278,6 +278,10 @@ typedef struct {
   int32_t precision;           /* SGPU_PREC_*                               */
   int32_t max_events;          /* 0 = unbounded; else packets still moving after
                                   this many events are left 'moving'          */
+  int32_t use_ddmc;            /* transport_use_ddmc: 0 off, 2 discrete diffusion
+                                  Monte Carlo (1D sphere, FP64 form only)     */
+  double  ddmc_tau_threshold;  /* transport_ddmc_tau_threshold: zone optical
+                                  depth above which a photon diffuses        */
 } sgpu_transport_params;
 
 /* Device counters of one propagation call (roofline inputs). */

@@ -307,18 +307,264 @@ static int do_scatter(tctx_t* c, pkt_t* p, double eps)
   return fate;
 }
 
-/* transport::propagate_monte_carlo, transport.cpp:260-484 (no DDMC) */
+
+/* ---------------------------------------------------------------------------
+ * Discrete diffusion Monte Carlo (transport_use_ddmc = 2), 1D sphere
+ * ------------------------------------------------------------------------- */
+#define DDMC_LAMBDA 0.7104     /* discrete_diffusion.cpp:118 */
+#define DDMC_SML_PUSH 1.0e-8   /* :119 */
+#define DDMC_SCATTER_CAP 100000
+
+static void sample_in_zone(const sgpu_grid* g, int i, const double* ran, double* r);
+
+/* grid_1D_sphere::get_zone_size (grid_1D_sphere.h:65-68): r_out.delta(i) */
+static double zone_size(const sgpu_grid* g, int i)
+{
+  return i == 0 ? g->r_out.x[0] - g->r_out.min : g->r_out.x[i] - g->r_out.x[i - 1];
+}
+
+static double radius(const double* x) { return sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]); }
+
+/* photon in a zone whose optical depth sigma dr exceeds the threshold
+ * (transport.cpp:219-231, 275-286, 298-309) */
+static int ddmc_zone(tctx_t* c, pkt_t* p, int zone, double dshift, double* sigma, double* dr)
+{
+  int old = p->ind;
+  double eps;
+  p->ind = zone;
+  get_opacity(c, p, dshift, sigma, &eps);
+  p->ind = old;
+  *dr = zone_size(c->g, zone);
+  return (*sigma * *dr > c->p->ddmc_tau_threshold) && (p->type == SGPU_PHOTON);
+}
+
+/* transport::sample_dir_from_blackbody_surface, discrete_diffusion.cpp:760-794 */
+static void sample_dir_bb_surface(tctx_t* c, pkt_t* p)
+{
+  double mu = sqrt(urand(c, p));
+  double phi = 2.0 * OR_PI * urand(c, p);
+  double smu = sqrt(1.0 - mu * mu);
+  double v1 = smu * cos(phi), v2 = smu * sin(phi), v3 = mu;
+  double r1 = p->x[0] / radius(p->x), r2 = p->x[1] / radius(p->x), r3 = p->x[2] / radius(p->x);
+  double costheta = r3, sintheta = sqrt(1 - costheta * costheta);
+  double v1p = v1 * costheta + r1 * v3 * sintheta - r2 * (1.0 - costheta) * (-r2 * v1 + r1 * v2);
+  double v2p = v2 * costheta + r2 * v3 * sintheta - r1 * (1.0 - costheta) * (-r2 * v1 + r1 * v2);
+  double v3p = v3 * costheta - (r1 * v1 + r2 * v2) * sintheta;
+  double norm = sqrt(v1p * v1p + v2p * v2p + v3p * v3p);
+  p->D[0] = v1p / norm; p->D[1] = v2p / norm; p->D[2] = v3p / norm;
+}
+
+/* transport::discrete_diffuse_DDMC, discrete_diffusion.cpp:111-391: leakage
+ * between zones with the spherical geometry factors, effective inelastic
+ * scattering on a frequency grid, adiabatic loss and advection.  The
+ * inelastic-scattering loop, unbounded in the reference, stops after
+ * DDMC_SCATTER_CAP redistributions. */
+static int ddmc_walk(tctx_t* c, pkt_t* p, double tstop)
+{
+  const sgpu_grid* g = c->g;
+  const int nz = g->n_zones, nn = c->nu->n;
+  const double tau_thr = c->p->ddmc_tau_threshold;
+  int fate = SGPU_MOVING;
+  double dt_remaining = tstop - p->t;
+  while (fate == SGPU_MOVING) {
+    int ii = p->ind, ip = ii + 1, im = ii - 1;
+    if (ip == nz) ip = ii;
+    if (im < 0) im = 0;
+    double dx = zone_size(g, ii), dxp1 = zone_size(g, ip), dxm1 = zone_size(g, im);
+    double sigma_i, sigma_ip1, sigma_im1, eps_i, eps_ip1, eps_im1;
+    double dshift = do_dshift(c, p, 0);
+    int i_nu = get_opacity(c, p, dshift, &sigma_i, &eps_i);
+    p->ind = ip;
+    get_opacity(c, p, dshift, &sigma_ip1, &eps_ip1);
+    p->ind = im;
+    get_opacity(c, p, dshift, &sigma_im1, &eps_im1);
+    p->ind = ii;
+    if (sigma_i * dx <= tau_thr) return SGPU_MOVING;
+    double r_p = g->r_out.x[ii];
+    double r_m = g->r_out.x[im];
+    if (ii == 0) r_m = g->r_out.min;
+    double r_0 = 0.5 * (r_p + r_m);
+    double Xi = 1.0 + dx * dx / (12.0 * r_0 * r_0);
+    double geo = r_m * r_m / r_0 / r_0 / Xi;
+    int on_im1 = sigma_im1 * dxm1 > tau_thr;
+    double leak_left = on_im1 ? (2.0 / 3.0 / dx) * (1.0 / (sigma_i * dx + sigma_im1 * dxm1)) * geo
+                              : (2.0 / 3.0 / dx) * (1.0 / (sigma_i * dx + 2.0 * DDMC_LAMBDA)) * geo;
+    geo = r_p * r_p / r_0 / r_0 / Xi;
+    int on_ip1 = sigma_ip1 * dxp1 > tau_thr;
+    double leak_right = on_ip1 ? (2.0 / 3.0 / dx) * (1.0 / (sigma_i * dx + sigma_ip1 * dxp1)) * geo
+                               : (2.0 / 3.0 / dx) * (1.0 / (sigma_i * dx + 2.0 * DDMC_LAMBDA)) * geo;
+    if (ip == ii) leak_right = (2.0 / 3.0 / dx) * (1.0 / (sigma_i * dx + 2.0 * DDMC_LAMBDA)) * geo;
+    if (ii == 0) leak_left = 0.0;
+    double leak_tot = leak_left + leak_right;
+    double xi = urand(c, p);
+    double d_stay = OR_C * dt_remaining;
+    double d_leak = -log(xi) / (leak_left + leak_right);
+    double d_sc = INFINITY;
+    if (nn > 1) {
+      double tau_r = -1.0 * log(urand(c, p));
+      if (eps_i > 0.0) {
+        double k_inel = sigma_i * eps_i;
+        dshift = do_dshift(c, p, 0);
+        i_nu = get_opacity(c, p, dshift, &sigma_i, &eps_i);
+        double elastic = c->tab->emis_cdf[(size_t)p->ind * nn + i_nu];
+        k_inel *= 1.0 - elastic;
+        d_sc = tau_r / k_inel;
+      }
+    }
+    int event;
+    double this_d;
+    if ((d_sc < d_leak) && (d_sc < d_stay)) { event = 0; this_d = d_sc; }
+    else if (d_leak < d_stay) { event = 1; this_d = d_leak; }
+    else { event = 2; this_d = d_stay; }
+    double this_dt = this_d / OR_C;
+    p->t += this_dt;
+    dt_remaining -= this_dt;
+    atomic_add(&c->ta->J_nu[(size_t)p->ind * (c->p->store_Jnu ? nn : 1)], p->e * this_d);
+    atomic_add(&c->ta->e_abs[p->ind], (p->e * dshift) * this_d * sigma_i * eps_i);
+    int leaked2imc = 0;
+    if (event == 0) {
+      int now_i_nu = i_nu;
+      for (int k = 0; now_i_nu == i_nu && k < DDMC_SCATTER_CAP; k++) {
+        fate = do_scatter(c, p, 1.0);
+        dshift = do_dshift(c, p, 0);
+        now_i_nu = get_opacity(c, p, dshift, &sigma_i, &eps_i);
+      }
+    } else if (event == 2) {
+      dt_remaining = -1.0;
+      fate = SGPU_STOPPED;
+    } else {
+      double P_left = leak_left / leak_tot;
+      double xi2 = urand(c, p);
+      if (xi2 <= P_left) {
+        p->ind--;
+        double rr = radius(p->x);
+        double dr = rr - r_m;
+        if (on_im1) dr += urand(c, p) * dxm1;
+        else dr *= (1.0 + DDMC_SML_PUSH);
+        p->x[0] -= p->x[0] / rr * dr;
+        p->x[1] -= p->x[1] / rr * dr;
+        p->x[2] -= p->x[2] / rr * dr;
+        if (!on_im1) {
+          leaked2imc = 1;
+          lorentz_transform(c, p, 0);
+          sample_dir_bb_surface(c, p);
+          double mu = (p->x[0] * p->D[0] + p->x[1] * p->D[1] + p->x[2] * p->D[2]) / radius(p->x);
+          if (mu > 0.0) { p->D[0] = -p->D[0]; p->D[1] = -p->D[1]; p->D[2] = -p->D[2]; }
+          lorentz_transform(c, p, 1);
+        }
+      } else {
+        p->ind++;
+        double rr = radius(p->x);
+        double dr = r_p - rr;
+        if (on_ip1) dr += urand(c, p) * dxp1;
+        else dr *= (1.0 + DDMC_SML_PUSH);
+        p->x[0] += p->x[0] / rr * dr;
+        p->x[1] += p->x[1] / rr * dr;
+        p->x[2] += p->x[2] / rr * dr;
+        if (!on_ip1) {
+          leaked2imc = 1;
+          lorentz_transform(c, p, 0);
+          sample_dir_bb_surface(c, p);
+          double mu = (p->x[0] * p->D[0] + p->x[1] * p->D[1] + p->x[2] * p->D[2]) / radius(p->x);
+          if (mu < 0.0) { p->D[0] = -p->D[0]; p->D[1] = -p->D[1]; p->D[2] = -p->D[2]; }
+          lorentz_transform(c, p, 1);
+        }
+      }
+    }
+    /* homologous zone velocity and dv/ds (grid_1D_sphere.cpp:707-716) */
+    double tn = g->t_now;
+    double zv[3] = {p->x[0] / tn, p->x[1] / tn, p->x[2] / tn};
+    double dvds = 1.0 / tn;
+    double vel = sqrt(zv[0] * zv[0] + zv[1] * zv[1] + zv[2] * zv[2]);
+    double v_over_r = vel / radius(p->x);
+    double dlnrhodt = dvds + 2.0 * v_over_r;
+    double shift = (1.0 - dlnrhodt * this_dt / 3.0);
+    lorentz_transform(c, p, 0);
+    p->e *= shift;
+    p->nu *= shift;
+    lorentz_transform(c, p, 1);
+    if (!leaked2imc) {
+      p->x[0] += zv[0] * this_dt;
+      p->x[1] += zv[1] * this_dt;
+      p->x[2] += zv[2] * this_dt;
+    }
+    p->ind = or_get_zone(g, p->x);
+    if (p->ind == -1) fate = SGPU_ABSORBED;
+    if (p->ind == -2) fate = SGPU_ESCAPED;
+  }
+  return fate;
+}
+
+/* transport::move_across_DDMC_interface, discrete_diffusion.cpp:398-509:
+ * converts the photon into the diffusing neighbour with the asymptotic
+ * diffusion-limit probability, else re-emits it from the interface back into
+ * its own zone.  Returns 1 when converted (the packet is then left in the
+ * comoving frame, as the reference leaves it). */
+static int ddmc_interface(tctx_t* c, pkt_t* p, int new_ind, double sigma_i, double dr)
+{
+  const sgpu_grid* g = c->g;
+  int nz = g->n_zones;
+  int ip = p->ind + 1, im = p->ind - 1;
+  if (ip == nz) ip = p->ind;
+  if (im < 0) im = 0;
+  double r_p = g->r_out.x[p->ind];
+  double r_m = g->r_out.x[im];
+  if (p->ind == 0) r_m = g->r_out.min;
+  double r_interface = new_ind == ip ? r_p : r_m;
+  lorentz_transform(c, p, 0);
+  double rr = radius(p->x);
+  double mu = fabs((p->x[0] * p->D[0] + p->x[1] * p->D[1] + p->x[2] * p->D[2]) / rr);
+  double p_convert = 4.0 * (1.0 + 1.5 * mu) / (3.0 * sigma_i * dr + 6.0 * 0.7104);
+  double xi = urand(c, p);
+  double ran[3];
+  ran[0] = urand(c, p); ran[1] = urand(c, p); ran[2] = urand(c, p);
+  double new_r[3];
+  if (xi <= p_convert) {
+    p->ind = new_ind;
+    sample_in_zone(g, p->ind, ran, new_r);
+    p->x[0] = new_r[0]; p->x[1] = new_r[1]; p->x[2] = new_r[2];
+    return 1;
+  }
+  sample_in_zone(g, p->ind, ran, new_r);
+  p->x[0] = new_r[0]; p->x[1] = new_r[1]; p->x[2] = new_r[2];
+  /* the reference re-evaluates |x| after each component moves */
+  if (new_ind == ip) {
+    double d = (r_interface - radius(p->x)) * (1.0 - DDMC_SML_PUSH);
+    p->x[0] += p->x[0] / radius(p->x) * d;
+    p->x[1] += p->x[1] / radius(p->x) * d;
+    p->x[2] += p->x[2] / radius(p->x) * d;
+  } else if (new_ind == im) {
+    double d = (radius(p->x) - r_interface) * (1.0 - DDMC_SML_PUSH);
+    p->x[0] -= p->x[0] / radius(p->x) * d;
+    p->x[1] -= p->x[1] / radius(p->x) * d;
+    p->x[2] -= p->x[2] / radius(p->x) * d;
+  }
+  sample_dir_bb_surface(c, p);
+  double mu2 = (p->x[0] * p->D[0] + p->x[1] * p->D[1] + p->x[2] * p->D[2]) / radius(p->x);
+  if (new_ind == ip && mu2 > 0.0) { p->D[0] = -p->D[0]; p->D[1] = -p->D[1]; p->D[2] = -p->D[2]; }
+  else if (new_ind == im && mu2 < 0.0) { p->D[0] = -p->D[0]; p->D[1] = -p->D[1]; p->D[2] = -p->D[2]; }
+  lorentz_transform(c, p, 1);
+  return 0;
+}
+
+/* transport::propagate_monte_carlo, transport.cpp:260-484, with the DDMC
+ * checks of :272-309 and :441-450 when use_ddmc = 2 */
 static int propagate_mc(tctx_t* c, pkt_t* p, double tstop, int max_events, int* n_ev)
 {
   int fate = SGPU_MOVING;
   int nn = c->nu->n;
+  const int ddmc = c->p->use_ddmc == 2;
   while (fate == SGPU_MOVING) {
     if (max_events > 0 && *n_ev >= max_events) break;
+    double sig_n = 0, dr_n = 0;
+    if (ddmc && ddmc_zone(c, p, p->ind, do_dshift(c, p, 0), &sig_n, &dr_n)) return SGPU_MOVING;
     (*n_ev)++;
     c->n_events++;
     double d_bn = 0;
     int new_ind = or_get_next_zone(c->g, p->x, p->D, p->ind, c->g->r_core, &d_bn);
     double dshift = do_dshift(c, p, 0);
+    int new_cell_ddmc = 0;
+    if (ddmc && new_ind >= 0) new_cell_ddmc = ddmc_zone(c, p, new_ind, dshift, &sig_n, &dr_n);
     double kappa, eps;
     int i_nu = get_opacity(c, p, dshift, &kappa, &eps);
     double d_nu = or_delta(c->nu, i_nu) / p->nu * OR_C / p->dvds;
@@ -363,6 +609,8 @@ static int propagate_mc(tctx_t* c, pkt_t* p, double tstop, int max_events, int* 
       } else if (new_ind == -2) {
         if (c->p->boundary_out_reflect) { p->D[0] *= -1; p->D[1] *= -1; p->D[2] *= -1; }
         else { p->ind = new_ind; fate = SGPU_ESCAPED; }
+      } else if (new_cell_ddmc && new_ind != p->ind) {
+        if (ddmc_interface(c, p, new_ind, sig_n, dr_n)) return SGPU_MOVING;
       } else {
         p->ind = new_ind;
       }
@@ -446,7 +694,12 @@ int or_propagate(const sgpu_grid* grid, const sgpu_locate* nu_grid,
         double tstop = prm->t_now + prm->dt;
         fate = SGPU_MOVING;
         while (fate == SGPU_MOVING) {
-          fate = propagate_mc(&c, &p, tstop, prm->max_events, &n_ev);
+          double sig, dr;
+          /* transport::propagate, transport.cpp:216-248 */
+          if (prm->use_ddmc == 2 && ddmc_zone(&c, &p, p.ind, do_dshift(&c, &p, 0), &sig, &dr))
+            fate = ddmc_walk(&c, &p, tstop);
+          else
+            fate = propagate_mc(&c, &p, tstop, prm->max_events, &n_ev);
           if (prm->max_events > 0 && n_ev >= prm->max_events) break;
         }
       }

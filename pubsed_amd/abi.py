@@ -94,7 +94,8 @@ class TransportParams(C.Structure):
     _fields_ = [("t_now", f64), ("dt", f64), ("seed", u64), ("steady_state", i32),
                 ("radiative_eq", i32), ("boundary_in_reflect", i32),
                 ("boundary_out_reflect", i32), ("store_Jnu", i32), ("omit_scattering", i32),
-                ("precision", i32), ("max_events", i32)]
+                ("precision", i32), ("max_events", i32), ("use_ddmc", i32),
+                ("ddmc_tau_threshold", f64)]
 
 
 class Counters(C.Structure):

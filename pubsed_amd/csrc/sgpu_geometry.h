@@ -35,6 +35,8 @@ __device__ __forceinline__ int ax_locate(const Axis<double>& g, double xv)
   else if (g.type == SGPU_LOC_LOG) ind = (int)floor(log(xv / g.min) / log(1 + g.del));
   else if (g.type == SGPU_LOC_NONE) ind = 0;
   else ind = upper_bound_dev(g.x, n, xv);
+  if (ind < 0) ind = 0;        // a NaN coordinate stays inside the table
+  if (ind > n) ind = n;
   if (g.type != SGPU_LOC_NONE && ind != 0 && ind != n && (ax_left(g, ind) > xv || g.x[ind] <= xv))
     ind = upper_bound_dev(g.x, n, xv);
   return ind;

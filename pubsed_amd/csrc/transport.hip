@@ -286,6 +286,33 @@ __device__ __forceinline__ void sample_photon_frequency(const PropArgs<float>& A
   }
 }
 
+// grid_1D_sphere::sample_in_zone (grid_1D_sphere.cpp:681-700),
+// grid_2D_cyln::sample_in_zone (grid_2D_cyln.cpp:528-541)
+__device__ __forceinline__ void sample_in_zone(const GridView<double>& g, int i, const double* ran, double* r)
+{
+  if (g.kind == SGPU_GRID_1D_SPHERE) {
+    double r_0 = (i == 0) ? g.r_out.min : g.r_out.x[i - 1];
+    double ro = g.r_out.x[i];
+    double r_samp = pow(r_0 * r_0 * r_0 + ran[0] * (ro * ro * ro - r_0 * r_0 * r_0), 1.0 / 3.0);
+    double mu = 1 - 2.0 * ran[1];
+    double phi = 2.0 * PI * ran[2];
+    double sin_theta = sqrt(1 - mu * mu);
+    r[0] = r_samp * sin_theta * cos(phi);
+    r[1] = r_samp * sin_theta * sin(phi);
+    r[2] = r_samp * mu;
+  } else {
+    const int nz = g.z_out.n;
+    const int ix = i / nz, iz = i - ix * nz;
+    double phi = 2 * PI * ran[1];
+    double p_in = ax_left(g.x_out, ix);
+    double p_out = g.x_out.x[ix];
+    double p_samp = sqrt(p_in * p_in + ran[0] * (p_out * p_out - p_in * p_in));
+    r[0] = p_samp * cos(phi);
+    r[1] = p_samp * sin(phi);
+    r[2] = ax_left(g.z_out, iz) + ran[2] * ax_delta(g.z_out, iz);
+  }
+}
+
 // transport::isotropic_scatter (scatter.cpp:303-351); D divided by |D|
 template <class R>
 __device__ __forceinline__ void isotropic_scatter(const PropArgs<R>& A, Pkt<R>& p, int redist, Counts& c)
@@ -366,8 +393,253 @@ __device__ __forceinline__ int do_scatter(const PropArgs<R>& A, Pkt<R>& p, R eps
   return fate;
 }
 
+// ---------------------------------------------------------------------------
+// Discrete diffusion Monte Carlo (transport_use_ddmc = 2): 1D sphere, FP64
+// form.  A photon in a zone of optical depth sigma dr above the threshold
+// leaves the event loop and random-walks between zones with the leakage
+// opacities of discrete_diffuse_DDMC; a Monte Carlo photon about to enter
+// such a zone is converted at the interface with the asymptotic
+// diffusion-limit probability or re-emitted into its own zone.
+// ---------------------------------------------------------------------------
+constexpr double DDMC_LAMBDA = 0.7104;     // discrete_diffusion.cpp:118
+constexpr double DDMC_SML_PUSH = 1.0e-8;   // :119
+// the inelastic-scattering loop of the walk, unbounded in the reference
+// (:256-264), stops after this many redistributions
+constexpr int DDMC_SCATTER_CAP = 100000;
+
+// grid_1D_sphere::get_zone_size (grid_1D_sphere.h:65-68)
+__device__ __forceinline__ double zone_size(const GridView<double>& g, int i)
+{
+  return i == 0 ? g.r_out.x[0] - g.r_out.min : g.r_out.x[i] - g.r_out.x[i - 1];
+}
+
+__device__ __forceinline__ double radius(const double* x) { return sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]); }
+
+// the diffusion test of transport.cpp:219-231, 275-286, 298-309 for `zone`
+__device__ __forceinline__ bool ddmc_zone(const PropArgs<double>& A, Pkt<double>& p, int zone, double dshift,
+                                          double* sigma, double* dr)
+{
+  const int old = p.ind;
+  double eps;
+  p.ind = zone;
+  get_opacity(A, p, dshift, sigma, &eps);
+  p.ind = old;
+  *dr = zone_size(A.g, zone);
+  return (*sigma * *dr > A.p.ddmc_tau_threshold) && (p.type == SGPU_PHOTON);
+}
+
+// transport::sample_dir_from_blackbody_surface (discrete_diffusion.cpp:760-794)
+__device__ __forceinline__ void sample_dir_bb_surface(Pkt<double>& p)
+{
+  const double mu = sqrt(p.rng.uniform());
+  const double phi = 2.0 * PI * p.rng.uniform();
+  const double smu = sqrt(1.0 - mu * mu);
+  const double v1 = smu * cos(phi), v2 = smu * sin(phi), v3 = mu;
+  const double r = radius(p.x);
+  const double r1 = p.x[0] / r, r2 = p.x[1] / r, r3 = p.x[2] / r;
+  const double costheta = r3, sintheta = sqrt(1 - costheta * costheta);
+  const double v1p = v1 * costheta + r1 * v3 * sintheta - r2 * (1.0 - costheta) * (-r2 * v1 + r1 * v2);
+  const double v2p = v2 * costheta + r2 * v3 * sintheta - r1 * (1.0 - costheta) * (-r2 * v1 + r1 * v2);
+  const double v3p = v3 * costheta - (r1 * v1 + r2 * v2) * sintheta;
+  const double norm = sqrt(v1p * v1p + v2p * v2p + v3p * v3p);
+  p.D[0] = v1p / norm; p.D[1] = v2p / norm; p.D[2] = v3p / norm;
+}
+
+__device__ __forceinline__ void flip(Pkt<double>& p) { p.D[0] = -p.D[0]; p.D[1] = -p.D[1]; p.D[2] = -p.D[2]; }
+
+// transport::discrete_diffuse_DDMC (discrete_diffusion.cpp:111-391)
+__device__ __forceinline__ int ddmc_walk(const PropArgs<double>& A, Pkt<double>& p, double tstop, ZoneSink& zs,
+                                      Counts& c)
+{
+  const GridView<double>& g = A.g;
+  const int nz = g.n_zones, nn = A.nu.n;
+  const double tau_thr = A.p.ddmc_tau_threshold;
+  int fate = SGPU_MOVING;
+  double dt_remaining = tstop - p.t;
+  while (fate == SGPU_MOVING) {
+    const int ii = p.ind;
+    int ip = ii + 1, im = ii - 1;
+    if (ip == nz) ip = ii;
+    if (im < 0) im = 0;
+    const double dx = zone_size(g, ii), dxp1 = zone_size(g, ip), dxm1 = zone_size(g, im);
+    double sigma_i, sigma_ip1, sigma_im1, eps_i, eps_n;
+    double dshift = dshift_lab_to_comoving(g, p);
+    int i_nu = get_opacity(A, p, dshift, &sigma_i, &eps_i);
+    p.ind = ip;
+    get_opacity(A, p, dshift, &sigma_ip1, &eps_n);
+    p.ind = im;
+    get_opacity(A, p, dshift, &sigma_im1, &eps_n);
+    p.ind = ii;
+    if (sigma_i * dx <= tau_thr) return SGPU_MOVING;
+    const double r_p = g.r_out.x[ii];
+    const double r_m = ii == 0 ? g.r_out.min : g.r_out.x[im];
+    const double r_0 = 0.5 * (r_p + r_m);
+    const double Xi = 1.0 + dx * dx / (12.0 * r_0 * r_0);
+    double geo = r_m * r_m / r_0 / r_0 / Xi;
+    const bool on_im1 = sigma_im1 * dxm1 > tau_thr;
+    double leak_left = on_im1 ? (2.0 / 3.0 / dx) * (1.0 / (sigma_i * dx + sigma_im1 * dxm1)) * geo
+                              : (2.0 / 3.0 / dx) * (1.0 / (sigma_i * dx + 2.0 * DDMC_LAMBDA)) * geo;
+    geo = r_p * r_p / r_0 / r_0 / Xi;
+    const bool on_ip1 = sigma_ip1 * dxp1 > tau_thr;
+    double leak_right = on_ip1 ? (2.0 / 3.0 / dx) * (1.0 / (sigma_i * dx + sigma_ip1 * dxp1)) * geo
+                               : (2.0 / 3.0 / dx) * (1.0 / (sigma_i * dx + 2.0 * DDMC_LAMBDA)) * geo;
+    if (ip == ii) leak_right = (2.0 / 3.0 / dx) * (1.0 / (sigma_i * dx + 2.0 * DDMC_LAMBDA)) * geo;
+    if (ii == 0) leak_left = 0.0;
+    const double leak_tot = leak_left + leak_right;
+    const double xi = p.rng.uniform();
+    const double d_stay = C_LIGHT * dt_remaining;
+    const double d_leak = -log(xi) / (leak_left + leak_right);
+    double d_sc = INFINITY;
+    if (nn > 1) {
+      const double tau_r = -1.0 * log(p.rng.uniform());
+      if (eps_i > 0.0) {
+        double k_inel = sigma_i * eps_i;
+        dshift = dshift_lab_to_comoving(g, p);
+        i_nu = get_opacity(A, p, dshift, &sigma_i, &eps_i);
+        const double elastic = A.tab.cdf[(size_t)p.ind * nn + i_nu];
+        k_inel *= 1.0 - elastic;
+        d_sc = tau_r / k_inel;
+      }
+    }
+    int event;
+    double this_d;
+    if ((d_sc < d_leak) && (d_sc < d_stay)) { event = 0; this_d = d_sc; }
+    else if (d_leak < d_stay) { event = 1; this_d = d_leak; }
+    else { event = 2; this_d = d_stay; }
+    const double this_dt = this_d / C_LIGHT;
+    p.t += this_dt;
+    dt_remaining -= this_dt;
+    // bin 0 of the zone's J row, and e_abs without the eps_imc factor, as
+    // the reference tallies them (:240-244)
+    atomic_add_f64(&A.ta.J_nu[(size_t)p.ind * (A.p.store_Jnu ? nn : 1)], p.e * this_d);
+    atomic_add_f64(&A.ta.e_abs[p.ind], (p.e * dshift) * this_d * sigma_i * eps_i);
+    bool leaked2imc = false;
+    if (event == 0) {
+      int now_i_nu = i_nu;
+      for (int k = 0; now_i_nu == i_nu && k < DDMC_SCATTER_CAP; k++) {
+        fate = do_scatter(A, p, 1.0, zs, c);
+        dshift = dshift_lab_to_comoving(g, p);
+        now_i_nu = get_opacity(A, p, dshift, &sigma_i, &eps_i);
+      }
+    } else if (event == 2) {
+      dt_remaining = -1.0;
+      fate = SGPU_STOPPED;
+    } else {
+      const double P_left = leak_left / leak_tot;
+      const double xi2 = p.rng.uniform();
+      if (xi2 <= P_left) {
+        p.ind--;
+        const double rr = radius(p.x);
+        double dr = rr - r_m;
+        if (on_im1) dr += p.rng.uniform() * dxm1;
+        else dr *= (1.0 + DDMC_SML_PUSH);
+        p.x[0] -= p.x[0] / rr * dr;
+        p.x[1] -= p.x[1] / rr * dr;
+        p.x[2] -= p.x[2] / rr * dr;
+        if (!on_im1) {
+          leaked2imc = true;
+          lorentz_transform(g, p, 0);
+          sample_dir_bb_surface(p);
+          const double mu = (p.x[0] * p.D[0] + p.x[1] * p.D[1] + p.x[2] * p.D[2]) / radius(p.x);
+          if (mu > 0.0) flip(p);
+          lorentz_transform(g, p, 1);
+        }
+      } else {
+        p.ind++;
+        const double rr = radius(p.x);
+        double dr = r_p - rr;
+        if (on_ip1) dr += p.rng.uniform() * dxp1;
+        else dr *= (1.0 + DDMC_SML_PUSH);
+        p.x[0] += p.x[0] / rr * dr;
+        p.x[1] += p.x[1] / rr * dr;
+        p.x[2] += p.x[2] / rr * dr;
+        if (!on_ip1) {
+          leaked2imc = true;
+          lorentz_transform(g, p, 0);
+          sample_dir_bb_surface(p);
+          const double mu = (p.x[0] * p.D[0] + p.x[1] * p.D[1] + p.x[2] * p.D[2]) / radius(p.x);
+          if (mu < 0.0) flip(p);
+          lorentz_transform(g, p, 1);
+        }
+      }
+    }
+    // homologous zone velocity and dv/ds (grid_1D_sphere.cpp:707-716); the
+    // adiabatic loss and the advection of :342-368
+    const double tn = g.t_now;
+    const double zv[3] = {p.x[0] / tn, p.x[1] / tn, p.x[2] / tn};
+    const double dvds = 1.0 / tn;
+    const double vel = sqrt(zv[0] * zv[0] + zv[1] * zv[1] + zv[2] * zv[2]);
+    const double v_over_r = vel / radius(p.x);
+    const double dlnrhodt = dvds + 2.0 * v_over_r;
+    const double shift = (1.0 - dlnrhodt * this_dt / 3.0);
+    lorentz_transform(g, p, 0);
+    p.e *= shift;
+    p.nu *= shift;
+    lorentz_transform(g, p, 1);
+    if (!leaked2imc) {
+      p.x[0] += zv[0] * this_dt;
+      p.x[1] += zv[1] * this_dt;
+      p.x[2] += zv[2] * this_dt;
+    }
+    p.ind = get_zone(g, p.x);
+    if (p.ind == -1) fate = SGPU_ABSORBED;
+    if (p.ind == -2) fate = SGPU_ESCAPED;
+  }
+  return fate;
+}
+
+// transport::move_across_DDMC_interface (discrete_diffusion.cpp:398-509);
+// true when converted (the packet is then left in the comoving frame, as
+// the reference leaves it)
+__device__ __forceinline__ bool ddmc_interface(const PropArgs<double>& A, Pkt<double>& p, int new_ind,
+                                            double sigma_i, double dr)
+{
+  const GridView<double>& g = A.g;
+  const int nz = g.n_zones;
+  int ip = p.ind + 1, im = p.ind - 1;
+  if (ip == nz) ip = p.ind;
+  if (im < 0) im = 0;
+  const double r_p = g.r_out.x[p.ind];
+  const double r_m = p.ind == 0 ? g.r_out.min : g.r_out.x[im];
+  const double r_interface = new_ind == ip ? r_p : r_m;
+  lorentz_transform(g, p, 0);
+  const double rr = radius(p.x);
+  const double mu = fabs((p.x[0] * p.D[0] + p.x[1] * p.D[1] + p.x[2] * p.D[2]) / rr);
+  const double p_convert = 4.0 * (1.0 + 1.5 * mu) / (3.0 * sigma_i * dr + 6.0 * 0.7104);
+  const double xi = p.rng.uniform();
+  double ran[3];
+  ran[0] = p.rng.uniform(); ran[1] = p.rng.uniform(); ran[2] = p.rng.uniform();
+  double new_r[3];
+  if (xi <= p_convert) {
+    p.ind = new_ind;
+    sample_in_zone(g, p.ind, ran, new_r);
+    p.x[0] = new_r[0]; p.x[1] = new_r[1]; p.x[2] = new_r[2];
+    return true;
+  }
+  sample_in_zone(g, p.ind, ran, new_r);
+  p.x[0] = new_r[0]; p.x[1] = new_r[1]; p.x[2] = new_r[2];
+  // |x| is re-evaluated after each component moves, as in the reference
+  if (new_ind == ip) {
+    const double d = (r_interface - radius(p.x)) * (1.0 - DDMC_SML_PUSH);
+    p.x[0] += p.x[0] / radius(p.x) * d;
+    p.x[1] += p.x[1] / radius(p.x) * d;
+    p.x[2] += p.x[2] / radius(p.x) * d;
+  } else if (new_ind == im) {
+    const double d = (radius(p.x) - r_interface) * (1.0 - DDMC_SML_PUSH);
+    p.x[0] -= p.x[0] / radius(p.x) * d;
+    p.x[1] -= p.x[1] / radius(p.x) * d;
+    p.x[2] -= p.x[2] / radius(p.x) * d;
+  }
+  sample_dir_bb_surface(p);
+  const double mu2 = (p.x[0] * p.D[0] + p.x[1] * p.D[1] + p.x[2] * p.D[2]) / radius(p.x);
+  if (new_ind == ip && mu2 > 0.0) flip(p);
+  else if (new_ind == im && mu2 < 0.0) flip(p);
+  lorentz_transform(g, p, 1);
+  return false;
+}
+
 // one iteration of transport::propagate_monte_carlo (transport.cpp:266-481)
-template <class R>
+template <class R, bool DDMC = false>
 __device__ __forceinline__ int mc_event(const PropArgs<R>& A, Pkt<R>& p, double tstop,
                                         ZoneSink& zs, Counts& c)
 {
@@ -379,6 +651,10 @@ __device__ __forceinline__ int mc_event(const PropArgs<R>& A, Pkt<R>& p, double 
   int new_ind = get_next_zone(A.g, p.x, p.D, p.ind, &d_bn);
   d_bn = floor_boundary_step(p, d_bn);
   R dshift = dshift_lab_to_comoving(A.g, p);
+  bool new_cell_ddmc = false;
+  R sig_n = 0, dr_n = 0;
+  if constexpr (DDMC)
+    if (new_ind >= 0) new_cell_ddmc = ddmc_zone(A, p, new_ind, dshift, &sig_n, &dr_n);
   R kappa, eps;
   int i_nu = get_opacity(A, p, dshift, &kappa, &eps);
   R d_nu = qdiv(qdiv(ax_delta(A.nu, i_nu), p.nu) * cl, p.dvds);
@@ -421,6 +697,9 @@ __device__ __forceinline__ int mc_event(const PropArgs<R>& A, Pkt<R>& p, double 
     } else if (new_ind == -2) {
       if (A.p.boundary_out_reflect) { p.D[0] *= -1; p.D[1] *= -1; p.D[2] *= -1; }
       else { p.ind = new_ind; fate = SGPU_ESCAPED; }
+    } else if (DDMC && new_cell_ddmc && new_ind != p.ind) {
+      if constexpr (DDMC)
+        if (ddmc_interface(A, p, new_ind, sig_n, dr_n)) return SGPU_MOVING;
     } else {
       p.ind = new_ind;
     }
@@ -536,7 +815,7 @@ __device__ __forceinline__ void stage_axis(Axis<float>& dst, const Axis<float>& 
 // OCC: minimum waves per SIMD the register allocation must allow (1: no bound).
 // KIND: the grid kind when known at compile time (0: read from the arguments);
 // a fixed kind removes the other geometry from the loop body.
-template <class R, bool PERSIST, bool ACCUM, int OCC = 1, int KIND = 0>
+template <class R, bool PERSIST, bool ACCUM, int OCC = 1, int KIND = 0, bool DDMC = false>
 __global__ void __launch_bounds__(PROP_THREADS) __attribute__((amdgpu_waves_per_eu(OCC)))
 propagate_kernel(PropArgs<R> A0)
 {
@@ -580,7 +859,16 @@ propagate_kernel(PropArgs<R> A0)
       }
       have = true;
     }
-    int fate = mc_event(A, p, tstop, zs, c);
+    int fate;
+    if constexpr (DDMC) {
+      // transport::propagate (transport.cpp:216-248): diffuse while the
+      // photon sits in an optically thick zone
+      R sig, dr;
+      if (ddmc_zone(A, p, p.ind, dshift_lab_to_comoving(A.g, p), &sig, &dr)) fate = ddmc_walk(A, p, tstop, zs, c);
+      else fate = mc_event<R, true>(A, p, tstop, zs, c);
+    } else {
+      fate = mc_event(A, p, tstop, zs, c);
+    }
     n_ev++;
     if (fate != SGPU_MOVING || (A.p.max_events > 0 && n_ev >= A.p.max_events)) {
       finish_packet(A, i, p, fate, c);
@@ -737,30 +1025,6 @@ struct EmitArgs {
   int lds_dep;
 };
 
-__device__ __forceinline__ void sample_in_zone(const GridView<double>& g, int i, const double* ran, double* r)
-{
-  if (g.kind == SGPU_GRID_1D_SPHERE) {
-    double r_0 = (i == 0) ? g.r_out.min : g.r_out.x[i - 1];
-    double ro = g.r_out.x[i];
-    double r_samp = pow(r_0 * r_0 * r_0 + ran[0] * (ro * ro * ro - r_0 * r_0 * r_0), 1.0 / 3.0);
-    double mu = 1 - 2.0 * ran[1];
-    double phi = 2.0 * PI * ran[2];
-    double sin_theta = sqrt(1 - mu * mu);
-    r[0] = r_samp * sin_theta * cos(phi);
-    r[1] = r_samp * sin_theta * sin(phi);
-    r[2] = r_samp * mu;
-  } else {
-    const int nz = g.z_out.n;
-    const int ix = i / nz, iz = i - ix * nz;
-    double phi = 2 * PI * ran[1];
-    double p_in = ax_left(g.x_out, ix);
-    double p_out = g.x_out.x[ix];
-    double p_samp = sqrt(p_in * p_in + ran[0] * (p_out * p_out - p_in * p_in));
-    r[0] = p_samp * cos(phi);
-    r[1] = p_samp * sin(phi);
-    r[2] = ax_left(g.z_out, iz) + ran[2] * ax_delta(g.z_out, iz);
-  }
-}
 
 __global__ void __launch_bounds__(256) emit_kernel(EmitArgs A)
 {
@@ -1042,6 +1306,12 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
                tallies->fr_rad && tallies->L_radio_dep, "tally arrays missing");
   SGPU_REQUIRE(params->precision == SGPU_PREC_FP64 || params->precision == SGPU_PREC_MIXED,
                "unknown precision %d", params->precision);
+  SGPU_REQUIRE(params->use_ddmc == 0 || params->use_ddmc == 2,
+               "transport_use_ddmc = %d: only 0 (off) and 2 (DDMC) are supported", params->use_ddmc);
+  if (params->use_ddmc == 2)
+    SGPU_REQUIRE(grid->kind == SGPU_GRID_1D_SPHERE && params->precision == SGPU_PREC_FP64 &&
+                 params->ddmc_tau_threshold >= 0,
+                 "DDMC runs on 1D spherical grids in the FP64 form with a threshold >= 0");
   if (n == 0) return SGPU_OK;
   const int nz = grid->n_zones, nn = nu_grid->n;
   if (params->precision == SGPU_PREC_FP64) {
@@ -1052,8 +1322,12 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
     A.eps_imc = tables->eps_imc; A.compton = tables->compton_opac; A.photoion = tables->photoion_opac;
     A.p = *params; A.pk = *packets; A.n = n; A.ta = *tallies; A.cnt = counters; A.lds_tallies = 0;
     timer_begin(ctx, "propagate");
-    hipLaunchKernelGGL((propagate_kernel<double, false, false>), dim3(grid_blocks(n, PROP_THREADS)),
-                       dim3(PROP_THREADS), 0, ctx->stream, A);
+    if (params->use_ddmc == 2)
+      hipLaunchKernelGGL((propagate_kernel<double, false, false, 1, 0, true>), dim3(grid_blocks(n, PROP_THREADS)),
+                         dim3(PROP_THREADS), 0, ctx->stream, A);
+    else
+      hipLaunchKernelGGL((propagate_kernel<double, false, false>), dim3(grid_blocks(n, PROP_THREADS)),
+                         dim3(PROP_THREADS), 0, ctx->stream, A);
     SGPU_LAUNCH_CHECK();
   } else {
     const size_t ntab = (size_t)nz * nn;

@@ -237,6 +237,8 @@ int transport::step(double dt)
   tp_.boundary_out_reflect = p_.transport_boundary_out_reflect;
   tp_.store_Jnu = p_.transport_store_Jnu;
   tp_.precision = p_.precision;
+  tp_.use_ddmc = p_.transport_use_ddmc;
+  tp_.ddmc_tau_threshold = p_.transport_ddmc_tau_threshold;
   int rc;
   if ((rc = set_opacity(dt))) return rc;
   if ((rc = wipe_radiation())) return rc;

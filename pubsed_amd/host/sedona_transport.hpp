@@ -26,6 +26,8 @@ struct Params {
   int transport_boundary_in_reflect = 0;
   int transport_boundary_out_reflect = 0;
   double transport_fleck_alpha = 0;
+  int transport_use_ddmc = 0;                 // 0 off, 2 DDMC (1D, FP64 form)
+  double transport_ddmc_tau_threshold = 100;
   double opacity_grey_opacity = 0;
   double opacity_epsilon = 1.0;
   int opacity_electron_scattering = 0;
