@@ -77,7 +77,8 @@ def load_library(path=None):
     global _lib
     if _lib is not None and path is None:
         return _lib
-    p = path or LIB_PATH
+    # SGPU_LIBRARY: load another build of the library (A/B timing of builds)
+    p = path or os.environ.get("SGPU_LIBRARY") or LIB_PATH
     if not os.path.exists(p):
         raise SgpuError("libsedona_gpu.so not built (%s): run python -m pubsed_amd.build" % p)
     lib = _declare(ctypes.CDLL(p))

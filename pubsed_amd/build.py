@@ -32,7 +32,10 @@ def sources():
 
 def _compile(src):
     obj = os.path.join(OBJ, os.path.basename(src)[:-4] + ".o")
-    cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
+    # SGPU_HIPCC_EXTRA: extra hipcc flags for a build (e.g. -gline-tables-only
+    # for a profile that maps samples to source lines)
+    extra = os.environ.get("SGPU_HIPCC_EXTRA", "").split()
+    cmd = [HIPCC] + FLAGS + extra + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed on %s:\n%s\n%s" % (src, r.stdout, r.stderr))

@@ -1372,8 +1372,9 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
     //                    2 one packet per thread, per-event atomics
     //   SGPU_PROP_WG_PER_CU  persistent workgroups per CU (default 12)
     //   SGPU_PROP_LDS    0: zone tallies as global atomics instead of LDS
-    //   SGPU_PROP_OCC    1: no register bound (2 waves per SIMD); default 3
-    //                    (168 VGPRs, no extra spill)
+    //   SGPU_PROP_OCC    waves per SIMD the register allocation must allow:
+    //                    4 (default: 128 VGPRs, 43 spilled, faster than 3
+    //                    without spills), 3 (168 VGPRs), 1 (no bound)
     const char* es = getenv("SGPU_PROP_SHAPE");
     const int shape = es ? atoi(es) : 0;
     const char* ew = getenv("SGPU_PROP_WG_PER_CU");
@@ -1386,27 +1387,26 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
     int64_t need = grid_blocks(n, PROP_THREADS);
     unsigned blocks = (unsigned)(want < need ? want : need);
     const char* eo = getenv("SGPU_PROP_OCC");
-    const bool occ3 = !(eo && atoi(eo) == 1);
+    const int occ = eo ? atoi(eo) : 4;
     const bool d1 = grid->kind == SGPU_GRID_1D_SPHERE;
     timer_begin(ctx, "propagate");
+    const dim3 gb(blocks), tb(PROP_THREADS);
     if (shape == 1)
-      hipLaunchKernelGGL((propagate_kernel<float, false, true>), dim3((unsigned)need), dim3(PROP_THREADS), lds,
-                         ctx->stream, A);
+      hipLaunchKernelGGL((propagate_kernel<float, false, true>), dim3((unsigned)need), tb, lds, ctx->stream, A);
     else if (shape == 2)
-      hipLaunchKernelGGL((propagate_kernel<float, false, false>), dim3((unsigned)need), dim3(PROP_THREADS), lds,
-                         ctx->stream, A);
-    else if (occ3 && d1)
-      hipLaunchKernelGGL((propagate_kernel<float, true, true, 3, SGPU_GRID_1D_SPHERE>), dim3(blocks),
-                         dim3(PROP_THREADS), lds, ctx->stream, A);
-    else if (occ3)
-      hipLaunchKernelGGL((propagate_kernel<float, true, true, 3, SGPU_GRID_2D_CYLN>), dim3(blocks),
-                         dim3(PROP_THREADS), lds, ctx->stream, A);
+      hipLaunchKernelGGL((propagate_kernel<float, false, false>), dim3((unsigned)need), tb, lds, ctx->stream, A);
+    else if (occ == 4 && d1)
+      hipLaunchKernelGGL((propagate_kernel<float, true, true, 4, SGPU_GRID_1D_SPHERE>), gb, tb, lds, ctx->stream, A);
+    else if (occ == 4)
+      hipLaunchKernelGGL((propagate_kernel<float, true, true, 4, SGPU_GRID_2D_CYLN>), gb, tb, lds, ctx->stream, A);
+    else if (occ == 3 && d1)
+      hipLaunchKernelGGL((propagate_kernel<float, true, true, 3, SGPU_GRID_1D_SPHERE>), gb, tb, lds, ctx->stream, A);
+    else if (occ == 3)
+      hipLaunchKernelGGL((propagate_kernel<float, true, true, 3, SGPU_GRID_2D_CYLN>), gb, tb, lds, ctx->stream, A);
     else if (d1)
-      hipLaunchKernelGGL((propagate_kernel<float, true, true, 1, SGPU_GRID_1D_SPHERE>), dim3(blocks),
-                         dim3(PROP_THREADS), lds, ctx->stream, A);
+      hipLaunchKernelGGL((propagate_kernel<float, true, true, 1, SGPU_GRID_1D_SPHERE>), gb, tb, lds, ctx->stream, A);
     else
-      hipLaunchKernelGGL((propagate_kernel<float, true, true, 1, SGPU_GRID_2D_CYLN>), dim3(blocks),
-                         dim3(PROP_THREADS), lds, ctx->stream, A);
+      hipLaunchKernelGGL((propagate_kernel<float, true, true, 1, SGPU_GRID_2D_CYLN>), gb, tb, lds, ctx->stream, A);
     SGPU_LAUNCH_CHECK();
   }
   timer_end(ctx, "propagate");
