@@ -29,6 +29,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# the mixed-form propagation kernel the default launch runs (4-wave register
+# bound; the last template argument is the grid kind: 1 sphere, 2 cylinder)
+PROP_KERNEL = "propagate_kernel<float, true, true, 4, %d, false>"
 
 
 def parse():
@@ -235,7 +238,7 @@ def main():
         "per_packet": per_pkt,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "propagate_kernel<float, true, true, 3, 1>", "kernel_ms": prop_avg,
+                     "kernel": PROP_KERNEL % 1, "kernel_ms": prop_avg,
                      "bytes_per_packet": bpk, "traffic_source": traffic_src},
         "opacity_build": opacity,
         "nlte": nlte,
@@ -311,7 +314,7 @@ def bench_time_dependent(a, gpu):
         "mixed_over_plain": value / plain, "per_packet": per,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "propagate_kernel<float, true, true, 3, 1>", "kernel_ms": prop_avg,
+                     "kernel": PROP_KERNEL % P.grid.kind, "kernel_ms": prop_avg,
                      "bytes_per_packet": bpk},
         "opacity_build": {"kernel_ms": float(np.mean(op_ms)), "zones": P.nz,
                           "lines": P.cfg.get("n_lines", 0)},

@@ -446,6 +446,26 @@ int sgpu_nlte_set_rates(sgpu_ctx* ctx, const sgpu_atoms* atoms, int32_t nsys,
                         const int32_t* n, const int64_t* rate_off, const int64_t* vec_off,
                         double* rates, double* lte);
 
+/* transport::set_opacity with the NLTE gas state (transport_opacity.cpp:
+ * 133-136 -> GasState::solve_state(J_nu) under opacity_use_nlte,
+ * GasState.cpp:208-263): the opacity build of sgpu_opacity_build, except that
+ * at every evaluation of the n_e Brent iteration the atoms flagged in
+ * atom_nlte (HOST array [n_atoms], opacity_atoms_in_nlte) are solved by
+ * AtomicSpecies::solve_nlte (AtomicSpecies_state.cpp:131-196: LTE at n_e,
+ * set_rates with the radiative rates computed once per zone from J_nu
+ * [nz][n_nu] (device), LU, populations) and the others by solve_lte.  The
+ * zone keeps the state of its last evaluation, as the reference's atoms do.
+ * lu_precision: SGPU_NLTE_FP64 or SGPU_NLTE_FP32_REFINE for the batched
+ * solves; use_collisions: opacity_use_collisions_nlte; the line Doppler
+ * width is params->line_velocity_width as for sgpu_nlte_radiative_rates.
+ * On the first step (params->first_step)
+ * or with no atom flagged it is sgpu_opacity_build.  *n_evals (may be NULL)
+ * receives the number of n_e evaluation passes. */
+int sgpu_opacity_build_nlte(sgpu_ctx* ctx, const sgpu_locate* nu_grid, const sgpu_zone_state* zones,
+                            const sgpu_atoms* atoms, const sgpu_opacity_params* params,
+                            const double* J_nu, const int32_t* atom_nlte, int32_t use_collisions,
+                            int32_t lu_precision, sgpu_opacity_out* out, int32_t* n_evals);
+
 /* AtomicSpecies::calculate_radiative_rates (AtomicSpecies_state.cpp:202-287)
  * for a batch of (zone, atom) systems: from the zone's mean intensity
  * J_nu[sys_zone[s]][n_nu] (the transport tally) and gas temperature T[s],

@@ -85,6 +85,9 @@ int      or_solve_state(const sgpu_atoms* at, int n_el, const int32_t* elem_A,
 int      or_opacity_build(const sgpu_locate* nu_grid, const sgpu_zone_state* zs,
                           const sgpu_atoms* at, const sgpu_opacity_params* p,
                           sgpu_opacity_out* out);
+int      or_opacity_build_nlte(const sgpu_locate* nu, const sgpu_zone_state* zs, const sgpu_atoms* at,
+                               const sgpu_opacity_params* p, const double* J_nu, const int32_t* atom_nlte,
+                               int use_collisions, sgpu_opacity_out* out);
 double   or_klein_nishina(double x);
 double   or_blackbody_nu(double T, double nu);
 void     or_solve_eq_temperature(const sgpu_locate* nu, int nz, const double* abs_opacity,

@@ -162,6 +162,12 @@ typedef struct {
   double dens, T;
   double* ion_part; double* ion_frac; double* lev_n;
   int n_evals;
+  /* NLTE atoms (atom_nlte NULL: every atom in LTE): this zone's radiative
+   * rates (global level / line index), flags and scratch */
+  const int32_t* atom_nlte;
+  const double* Pic; const double* Rci; const double* line_J;
+  int use_collisions, no_ground_recomb;
+  double* lev_lte; double* rates; double* lte_v; double* bvec;
 } zone_gas;
 
 /* GasState::charge_conservation, src/opacity/GasState.cpp:242-263 */
@@ -173,6 +179,15 @@ static double charge_conservation(zone_gas* z, double ne)
     int l0 = z->at->lev_off[i], i0 = z->at->ion_off[i];
     int nl = z->at->lev_off[i + 1] - l0;
     or_solve_lte(z->at, i, ne, z->T, z->ion_part + i0, z->ion_frac + i0, z->lev_n + l0);
+    if (z->atom_nlte && z->atom_nlte[i]) {
+      /* AtomicSpecies::solve_nlte (AtomicSpecies_state.cpp:131-196): the LTE
+       * populations just computed are lev_lte; rates, LU, populations */
+      for (int l = 0; l < nl; ++l) z->lev_lte[l0 + l] = z->lev_n[l0 + l];
+      int n = or_nlte_set_rates(z->at, i, ne, z->T, z->line_J, z->Pic, z->Rci, z->lev_lte, z->use_collisions,
+                                z->no_ground_recomb, z->rates, z->lte_v);
+      or_nlte_solve(n, z->rates, z->lte_v, z->bvec, NULL);
+      or_nlte_populations(z->at, i, z->bvec, z->lte_v, z->lev_n + l0, z->ion_frac + i0);
+    }
     double x = 0;   /* AtomicSpecies::get_net_ion_fraction, AtomicSpecies.h:125-131 */
     for (int l = 0; l < nl; ++l) x += z->lev_n[l0 + l] * z->at->lev_ion[l0 + l];
     f += z->dens * z->mf[i] / (z->A[i] * OR_M_P) * x;
@@ -243,12 +258,53 @@ int or_solve_state(const sgpu_atoms* at, int n_el, const int32_t* elem_A,
                    double* n_elec, double* ion_part, double* ion_frac, double* lev_n,
                    int* n_evals)
 {
-  zone_gas z = {at, n_el, elem_A, mass_frac, dens, T, ion_part, ion_frac, lev_n, 0};
+  zone_gas z = {at, n_el, elem_A, mass_frac, dens, T, ion_part, ion_frac, lev_n, 0,
+                NULL, NULL, NULL, NULL, 0, 0, NULL, NULL, NULL, NULL};
   double max_ne = 100 * dens / (mu_I * OR_M_P);
   double min_ne = 1e-10 * dens / (mu_I * OR_M_P);
   int err = 0;
   *n_elec = ne_brent(&z, min_ne, max_ne, 1e-3, &err);
   if (n_evals) *n_evals = z.n_evals;
+  return err;
+}
+
+/* NLTE inputs of the opacity build (NULL: LTE) */
+typedef struct {
+  const double* J_nu;        /* [nz][n_nu] */
+  const int32_t* atom_nlte;  /* [n_atoms] */
+  double line_velocity_width;
+  int use_collisions;
+} nlte_in;
+
+/* GasState::solve_state(J_nu) with use_nlte_ (GasState.cpp:208-233): the
+ * radiative rates of every NLTE atom from this zone's J_nu once, then the
+ * n_e Brent iteration with solve_nlte for those atoms. */
+static int solve_state_nlte(const sgpu_atoms* at, const sgpu_locate* nu, int n_el, const int32_t* elem_A,
+                            const double* mass_frac, double mu_I, double dens, double T, const nlte_in* nl,
+                            const double* J, int no_ground_recomb, double* n_elec, double* ion_part,
+                            double* ion_frac, double* lev_n)
+{
+  int n_lev = at->lev_off[at->n_atoms];
+  size_t nbl = at->n_blines > 0 ? (size_t)at->n_blines : 1;
+  int ml = at->max_levels > 0 ? at->max_levels : 1;
+  double* Pic = (double*)calloc((size_t)n_lev, sizeof(double));
+  double* Rci = (double*)calloc((size_t)n_lev, sizeof(double));
+  double* line_J = (double*)calloc(nbl, sizeof(double));
+  double* lev_lte = (double*)calloc((size_t)n_lev, sizeof(double));
+  double* rates = (double*)malloc(sizeof(double) * (size_t)ml * ml);
+  double* lte_v = (double*)malloc(sizeof(double) * (size_t)ml);
+  double* bvec = (double*)malloc(sizeof(double) * (size_t)ml);
+  for (int a = 0; a < n_el; ++a)
+    if (nl->atom_nlte[a])
+      or_nlte_radiative_rates(at, a, nu, J, T, elem_A[a], nl->line_velocity_width, Pic, Rci, line_J);
+  zone_gas z = {at, n_el, elem_A, mass_frac, dens, T, ion_part, ion_frac, lev_n, 0,
+                nl->atom_nlte, Pic, Rci, line_J, nl->use_collisions, no_ground_recomb,
+                lev_lte, rates, lte_v, bvec};
+  double max_ne = 100 * dens / (mu_I * OR_M_P);
+  double min_ne = 1e-10 * dens / (mu_I * OR_M_P);
+  int err = 0;
+  *n_elec = ne_brent(&z, min_ne, max_ne, 1e-3, &err);
+  free(Pic); free(Rci); free(line_J); free(lev_lte); free(rates); free(lte_v); free(bvec);
   return err;
 }
 
@@ -275,9 +331,8 @@ double or_blackbody_nu(double T, double nu)
  * fuzz_expansion_opacity (:464-494), AtomicSpecies::fuzzline_expansion_opacity
  * (AtomicSpecies_opacities.cpp:292-324), get_planck_mean (:564-591),
  * get_rosseland_mean (:638-666) and the Fleck factor (:231-252). */
-int or_opacity_build(const sgpu_locate* nu, const sgpu_zone_state* zs,
-                     const sgpu_atoms* at, const sgpu_opacity_params* p,
-                     sgpu_opacity_out* out)
+static int opacity_build(const sgpu_locate* nu, const sgpu_zone_state* zs, const sgpu_atoms* at,
+                         const sgpu_opacity_params* p, const nlte_in* nlte, sgpu_opacity_out* out)
 {
   int nz = zs->n_zones, ne_ = zs->n_elems, nn = nu->n;
   int n_lev = at && at->n_atoms ? at->lev_off[at->n_atoms] : 0;
@@ -318,7 +373,12 @@ int or_opacity_build(const sgpu_locate* nu, const sgpu_zone_state* zs,
     int status = 0;
     double n_elec = 0;
     if (grey == 0) {
-      status = or_solve_state(at, ne_, zs->elem_A, mf, mu_I, dens, temp, &n_elec, ip, ifr, ln, NULL);
+      /* transport_opacity.cpp:30-36, 114-136: LTE on the first step */
+      if (nlte && !p->first_step)
+        status = solve_state_nlte(at, nu, ne_, zs->elem_A, mf, mu_I, dens, temp, nlte,
+                                  nlte->J_nu + (size_t)i * nn, p->no_ground_recomb, &n_elec, ip, ifr, ln);
+      else
+        status = or_solve_state(at, ne_, zs->elem_A, mf, mu_I, dens, temp, &n_elec, ip, ifr, ln, NULL);
     }
     if (status > worst) worst = status;
     if (out->zone_status) out->zone_status[i] = status;
@@ -571,4 +631,22 @@ int or_opacity_build(const sgpu_locate* nu, const sgpu_zone_state* zs,
   free(X_now); free(mf); free(ip); free(ifr); free(ln);
   free(emis); free(opac); free(aopac); free(atom_opac); free(emis2); free(phifac);
   return worst;
+}
+
+/* transport::set_opacity zone loop, src/transport/transport_opacity.cpp:83-204 */
+int or_opacity_build(const sgpu_locate* nu, const sgpu_zone_state* zs, const sgpu_atoms* at,
+                     const sgpu_opacity_params* p, sgpu_opacity_out* out)
+{
+  return opacity_build(nu, zs, at, p, NULL, out);
+}
+
+/* the same with GasState::solve_state(J_nu) under use_nlte_ (GasState.cpp:208-263,
+ * transport_opacity.cpp:133-136): atoms flagged in atom_nlte are solved by
+ * solve_nlte at every n_e evaluation */
+int or_opacity_build_nlte(const sgpu_locate* nu, const sgpu_zone_state* zs, const sgpu_atoms* at,
+                          const sgpu_opacity_params* p, const double* J_nu, const int32_t* atom_nlte,
+                          int use_collisions, sgpu_opacity_out* out)
+{
+  nlte_in nl = {J_nu, atom_nlte, p->line_velocity_width, use_collisions};
+  return opacity_build(nu, zs, at, p, &nl, out);
 }

@@ -42,6 +42,10 @@ def load():
         lib.or_locate.argtypes = [P(abi.Locate), D]
         lib.or_locate_within_bounds.restype = I
         lib.or_locate_within_bounds.argtypes = [P(abi.Locate), D]
+        lib.or_opacity_build_nlte.restype = I
+        lib.or_opacity_build_nlte.argtypes = [P(abi.Locate), P(abi.ZoneState), P(abi.Atoms),
+                                              P(abi.OpacityParams), ctypes.c_void_p, ctypes.c_void_p,
+                                              I, P(abi.OpacityOut)]
         lib.or_opacity_build.restype = I
         lib.or_opacity_build.argtypes = [P(abi.Locate), P(abi.ZoneState), P(abi.Atoms),
                                          P(abi.OpacityParams), P(abi.OpacityOut)]
@@ -116,6 +120,17 @@ class OracleBackend:
         return self.lib.or_opacity_build(ctypes.byref(P.nu), ctypes.byref(P.zs),
                                          ctypes.byref(P.atoms) if P.atoms else None,
                                          ctypes.byref(P.op_params), ctypes.byref(P.op_out))
+
+    def opacity_build_nlte(self, P, J_nu, atom_nlte, use_collisions=1):
+        """The opacity build with the atoms flagged in atom_nlte solved in NLTE
+        from J_nu [nz][n_nu] (numpy) at every n_e evaluation."""
+        import numpy as np
+        J = np.ascontiguousarray(J_nu, dtype=np.float64)
+        fl = np.ascontiguousarray(atom_nlte, dtype=np.int32)
+        return self.lib.or_opacity_build_nlte(ctypes.byref(P.nu), ctypes.byref(P.zs), ctypes.byref(P.atoms),
+                                              ctypes.byref(P.op_params), J.ctypes.data_as(ctypes.c_void_p),
+                                              fl.ctypes.data_as(ctypes.c_void_p), use_collisions,
+                                              ctypes.byref(P.op_out))
 
     def emit(self, P, n_emit, id_base=0, offset=0, force_rproc=0):
         nout = ctypes.c_int64(0)

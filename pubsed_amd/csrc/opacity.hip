@@ -23,6 +23,7 @@
 // (src/opacity/AtomicSpecies_opacities.cpp:292-324), get_planck_mean /
 // get_rosseland_mean (GasState_opacities.cpp:564-666).
 #include <cfloat>
+#include <vector>
 #include "sgpu_internal.h"
 #include "sgpu_physics.h"
 
@@ -106,6 +107,52 @@ __device__ double charge_conservation(ZoneLds& L, const StateArgs& A, double ne,
   return f - ne;
 }
 
+// The temperature-only part of a zone's gas state, shared by the LTE and
+// NLTE solves: temperature clamp (transport_opacity.cpp:92-113), composition
+// decay, mass fractions and mean ion mass (GasState::set_mass_fractions,
+// GasState.cpp:137-156); for a gas zone also the Boltzmann factors and
+// partition functions, which depend on T only and so serve every n_e
+// evaluation (AtomicSpecies_state.cpp:59-67).  Returns false for a grey
+// zone or a problem without atoms.  Executed by the whole wave.
+__device__ bool zone_setup(ZoneLds& L, const sgpu_zone_state& zs, const sgpu_atoms& at,
+                           const sgpu_opacity_params& p, int n_lev, int z, int lane, double* T_out)
+{
+  const int ne_ = zs.n_elems;
+  double T = zs.T_gas[z];
+  if (T < p.temp_min) T = p.temp_min;
+  if (T > p.temp_max) T = p.temp_max;
+  if (p.first_step) T = zs.T_gas[z];   // transport_opacity.cpp:111-113
+  *T_out = T;
+  if (lane == 0) {
+    for (int j = 0; j < ne_; j++) L.X[j] = zs.X[(size_t)z * ne_ + j];
+    if (!p.omit_composition_decay) decay_composition(ne_, zs.elem_Z, zs.elem_A, L.X, p.t_now);
+    double norm = 0.0;
+    for (int j = 0; j < ne_; ++j) { L.mf[j] = L.X[j]; norm += L.X[j]; }
+    for (int j = 0; j < ne_; ++j) L.mf[j] /= norm;
+    double inv = 0.;
+    for (int j = 0; j < ne_; ++j) inv += L.mf[j] / zs.elem_A[j];
+    L.mu_I = 1. / inv;
+  }
+  __syncthreads();
+  double grey = zs.grey_opacity ? zs.grey_opacity[z] : 0.0;
+  if (grey != 0 || at.n_atoms == 0) return false;
+  for (int l = lane; l < n_lev; l += WAVE) L.bz[l] = exp(-at.lev_E[l] / K_EV / T);
+  __syncthreads();
+  for (int a = 0; a < at.n_atoms; ++a) {
+    int l0 = at.lev_off[a], l1 = at.lev_off[a + 1];
+    int i0 = at.ion_off[a], ni = at.ion_off[a + 1] - i0;
+    for (int s = 0; s < ni; ++s) {
+      double acc = 0;
+      for (int l = l0 + lane; l < l1; l += WAVE)
+        if (at.lev_ion[l] == s) acc += (double)at.lev_g[l] * L.bz[l];
+      double u = wave_sum(acc);
+      if (lane == 0) L.U[i0 + s] = u;
+    }
+  }
+  __syncthreads();
+  return true;
+}
+
 __global__ void __launch_bounds__(WAVE) state_kernel(StateArgs A)
 {
   __shared__ ZoneLds L;
@@ -113,27 +160,11 @@ __global__ void __launch_bounds__(WAVE) state_kernel(StateArgs A)
   const int lane = threadIdx.x;
   const sgpu_zone_state& zs = A.zs;
   const sgpu_atoms& at = A.at;
-  const int ne_ = zs.n_elems;
   double dens = zs.rho[z];
-  double T = zs.T_gas[z];
-  if (T < A.p.temp_min) T = A.p.temp_min;
-  if (T > A.p.temp_max) T = A.p.temp_max;
-  if (A.p.first_step) T = zs.T_gas[z];   // transport_opacity.cpp:111-113
-  if (lane == 0) {
-    for (int j = 0; j < ne_; j++) L.X[j] = zs.X[(size_t)z * ne_ + j];
-    if (!A.p.omit_composition_decay) decay_composition(ne_, zs.elem_Z, zs.elem_A, L.X, A.p.t_now);
-    // GasState::set_mass_fractions, GasState.cpp:137-156
-    double norm = 0.0;
-    for (int j = 0; j < ne_; ++j) { L.mf[j] = L.X[j]; norm += L.X[j]; }
-    for (int j = 0; j < ne_; ++j) L.mf[j] /= norm;
-    double inv = 0.;
-    for (int j = 0; j < ne_; ++j) inv += L.mf[j] / zs.elem_A[j];
-    L.mu_I = 1. / inv;
-    A.temp_used[z] = T;
-  }
-  __syncthreads();
-  double grey = zs.grey_opacity ? zs.grey_opacity[z] : 0.0;
-  if (grey != 0 || at.n_atoms == 0) {
+  double T;
+  const bool gas = zone_setup(L, zs, at, A.p, A.n_lev, z, lane, &T);
+  if (lane == 0) A.temp_used[z] = T;
+  if (!gas) {
     for (int k = lane; k < A.n_ion; k += WAVE) {
       A.nion[(size_t)z * A.n_ion + k] = 0;
       A.upart[(size_t)z * A.n_ion + k] = 1;
@@ -149,22 +180,6 @@ __global__ void __launch_bounds__(WAVE) state_kernel(StateArgs A)
     }
     return;
   }
-  // Boltzmann factors and partition functions depend on T only, so they are
-  // computed once for all Brent evaluations (AtomicSpecies_state.cpp:59-67)
-  for (int l = lane; l < A.n_lev; l += WAVE) L.bz[l] = exp(-at.lev_E[l] / K_EV / T);
-  __syncthreads();
-  for (int a = 0; a < at.n_atoms; ++a) {
-    int l0 = at.lev_off[a], l1 = at.lev_off[a + 1];
-    int i0 = at.ion_off[a], ni = at.ion_off[a + 1] - i0;
-    for (int s = 0; s < ni; ++s) {
-      double acc = 0;
-      for (int l = l0 + lane; l < l1; l += WAVE)
-        if (at.lev_ion[l] == s) acc += (double)at.lev_g[l] * L.bz[l];
-      double u = wave_sum(acc);
-      if (lane == 0) L.U[i0 + s] = u;
-    }
-  }
-  __syncthreads();
 
   // GasState::ne_brent_method (GasState.cpp:271-337), wave-uniform
   const double x1 = 1e-10 * dens / (L.mu_I * M_PROTON);
@@ -577,6 +592,289 @@ __global__ void max_status_kernel(const int32_t* st, int n, int32_t* worst)
   if (i < n && st[i] > 0) atomicMax(worst, st[i]);
 }
 
+
+// ------------------------------------------------------------------
+// NLTE-coupled gas state (GasState::solve_state(J_nu) with use_nlte_,
+// GasState.cpp:208-263): the n_e Brent iteration of every zone advances
+// one evaluation per pass; an evaluation is the LTE state of every atom at
+// the zone's trial n_e (K_a), set_rates + batched LU + populations for the
+// NLTE atoms of all zones at once, and the charge balance (K_b).
+// ------------------------------------------------------------------
+struct BrentState {
+  double a, b, c, d, e, fa, fb, fc;
+  double x;       // the trial n_e of the next evaluation
+  double root;
+  int iter, phase, err, done;
+};
+
+struct NlteArgs {
+  sgpu_zone_state zs;
+  sgpu_atoms at;
+  sgpu_opacity_params p;
+  int n_lev, n_ion;
+  const int32_t* sys_of;    // [nz][n_atoms]: system of an NLTE atom, -1 for LTE
+  double* temp_used;        // [nz]
+  BrentState* br;           // [nz]
+  double* sys_ne;           // [nsys]
+  double* lev_lte;          // [nsys][n_lev]   (global level index)
+  const double* sys_levn;   // [nsys][n_lev]   NLTE populations of the last solve
+  const double* sys_frac;   // [nsys][n_ion]
+  double* levn;             // [nz][n_lev]     state of the zone's last evaluation
+  double* frac;             // [nz][n_ion]
+  double* nion;             // [nz][n_ion]
+  double* upart;            // [nz][n_ion]
+  double* ndens;            // [nz][n_atoms]
+  double* x_lte;            // [nz][n_atoms]   net ion fraction of the LTE atoms
+  double* fval;             // [nz]
+  int32_t* active;          // zones still iterating
+  sgpu_opacity_out out;
+  double* fffac;
+  double* phifac;
+};
+
+// set-up, the Brent bracket, and the outputs of grey zones
+__global__ void __launch_bounds__(WAVE) nlte_init_kernel(NlteArgs A)
+{
+  __shared__ ZoneLds L;
+  const int z = blockIdx.x, lane = threadIdx.x;
+  const sgpu_atoms& at = A.at;
+  double T;
+  const bool gas = zone_setup(L, A.zs, at, A.p, A.n_lev, z, lane, &T);
+  BrentState& B = A.br[z];
+  if (lane == 0) {
+    A.temp_used[z] = T;
+    B = BrentState{};
+    B.done = !gas;
+  }
+  if (!gas) {
+    for (int k = lane; k < A.n_ion; k += WAVE) {
+      A.nion[(size_t)z * A.n_ion + k] = 0;
+      A.upart[(size_t)z * A.n_ion + k] = 1;
+      A.frac[(size_t)z * A.n_ion + k] = 0;
+    }
+    for (int k = lane; k < A.n_lev; k += WAVE) A.levn[(size_t)z * A.n_lev + k] = 0;
+    // the zone's NLTE systems are assembled and solved with the others but
+    // never read: give them finite inputs
+    for (int a = 0; a < at.n_atoms; ++a) {
+      const int s = A.sys_of[(size_t)z * at.n_atoms + a];
+      if (s < 0) continue;
+      if (lane == 0) A.sys_ne[s] = 1.0;
+      for (int l = at.lev_off[a] + lane; l < at.lev_off[a + 1]; l += WAVE) A.lev_lte[(size_t)s * A.n_lev + l] = 1.0;
+    }
+    return;
+  }
+  const double dens = A.zs.rho[z];
+  for (int a = lane; a < at.n_atoms; a += WAVE)
+    A.ndens[(size_t)z * at.n_atoms + a] = dens * L.mf[a] / (A.zs.elem_A[a] * M_PROTON);
+  if (lane == 0) {
+    const double x1 = 1e-10 * dens / (L.mu_I * M_PROTON);
+    const double x2 = 100 * dens / (L.mu_I * M_PROTON);
+    B.a = x1; B.b = x2; B.c = x2;
+    B.x = x1;
+  }
+}
+
+// K_a: the LTE state of every atom at the trial n_e (AtomicSpecies::solve_lte,
+// AtomicSpecies_state.cpp:56-123); NLTE atoms hand their LTE populations
+// and n_e to the rate assembly
+__global__ void __launch_bounds__(WAVE) nlte_lte_kernel(NlteArgs A)
+{
+  __shared__ ZoneLds L;
+  const int z = blockIdx.x, lane = threadIdx.x;
+  if (A.br[z].done) return;
+  const sgpu_atoms& at = A.at;
+  double T;
+  zone_setup(L, A.zs, at, A.p, A.n_lev, z, lane, &T);
+  const double ne = A.br[z].x;
+  for (int a = 0; a < at.n_atoms; ++a) {
+    if (lane == 0) saha_chain(L, at, a, ne, T);
+    __syncthreads();
+    const int l0 = at.lev_off[a], l1 = at.lev_off[a + 1];
+    const int i0 = at.ion_off[a], i1 = at.ion_off[a + 1];
+    const int s = A.sys_of[(size_t)z * at.n_atoms + a];
+    double acc = 0;
+    for (int l = l0 + lane; l < l1; l += WAVE) {
+      const int ion = at.lev_ion[l];
+      double v = L.frac[i0 + ion] * (double)at.lev_g[l] * L.bz[l] / L.U[i0 + ion];
+      v = v < 1e-30 ? 1e-30 : v;
+      acc += v * ion;
+      if (s >= 0) A.lev_lte[(size_t)s * A.n_lev + l] = v;
+      else A.levn[(size_t)z * A.n_lev + l] = v;
+    }
+    const double x = wave_sum(acc);
+    const double nd = A.ndens[(size_t)z * at.n_atoms + a];
+    for (int k = i0 + lane; k < i1; k += WAVE) {
+      A.upart[(size_t)z * A.n_ion + k] = L.U[k];
+      if (s < 0) {
+        A.frac[(size_t)z * A.n_ion + k] = L.frac[k];
+        A.nion[(size_t)z * A.n_ion + k] = nd * L.frac[k];
+      }
+    }
+    if (lane == 0) {
+      A.x_lte[(size_t)z * at.n_atoms + a] = x;
+      if (s >= 0) A.sys_ne[s] = ne;
+    }
+    __syncthreads();
+  }
+}
+
+// K_b: GasState::charge_conservation (GasState.cpp:242-263) with the NLTE
+// atoms' populations, which become the zone's state
+__global__ void __launch_bounds__(WAVE) nlte_charge_kernel(NlteArgs A)
+{
+  const int z = blockIdx.x, lane = threadIdx.x;
+  if (A.br[z].done) return;
+  const sgpu_atoms& at = A.at;
+  const double ne = A.br[z].x;
+  double f = 0;
+  for (int a = 0; a < at.n_atoms; ++a) {
+    const int s = A.sys_of[(size_t)z * at.n_atoms + a];
+    const double nd = A.ndens[(size_t)z * at.n_atoms + a];
+    double x;
+    if (s < 0) {
+      x = A.x_lte[(size_t)z * at.n_atoms + a];
+    } else {
+      double acc = 0;
+      for (int l = at.lev_off[a] + lane; l < at.lev_off[a + 1]; l += WAVE) {
+        const double v = A.sys_levn[(size_t)s * A.n_lev + l];
+        acc += v * at.lev_ion[l];
+        A.levn[(size_t)z * A.n_lev + l] = v;
+      }
+      x = wave_sum(acc);
+      for (int k = at.ion_off[a] + lane; k < at.ion_off[a + 1]; k += WAVE) {
+        const double fr = A.sys_frac[(size_t)s * A.n_ion + k];
+        A.frac[(size_t)z * A.n_ion + k] = fr;
+        A.nion[(size_t)z * A.n_ion + k] = nd * fr;
+      }
+    }
+    f += nd * x;
+  }
+  if (lane == 0) A.fval[z] = f - ne;
+}
+
+// GasState::ne_brent_method (GasState.cpp:271-337) as a state machine:
+// phase 0 has f(x1), phase 1 f(x2), phase 2 f at the new b of an iteration
+__device__ void brent_head(BrentState& B)
+{
+  const double tol = 1e-3, EPS = 3.0e-8;
+  if ((B.fb > 0.0 && B.fc > 0.0) || (B.fb < 0.0 && B.fc < 0.0)) {
+    B.c = B.a; B.fc = B.fa; B.e = B.d = B.b - B.a;
+  }
+  if (fabs(B.fc) < fabs(B.fb)) {
+    B.a = B.b; B.b = B.c; B.c = B.a;
+    B.fa = B.fb; B.fb = B.fc; B.fc = B.fa;
+  }
+  const double tol1 = 2.0 * EPS * fabs(B.b) + 0.5 * tol;
+  const double xm = 0.5 * (B.c - B.b);
+  if (fabs(xm) <= tol1 || B.fb == 0.0) { B.root = B.b; B.done = 1; return; }
+  if (fabs(B.e) >= tol1 && fabs(B.fa) > fabs(B.fb)) {
+    double pp, q, r;
+    const double s = B.fb / B.fa;
+    if (B.a == B.c) { pp = 2.0 * xm * s; q = 1.0 - s; }
+    else {
+      q = B.fa / B.fc; r = B.fb / B.fc;
+      pp = s * (2.0 * xm * q * (q - r) - (B.b - B.a) * (r - 1.0));
+      q = (q - 1.0) * (r - 1.0) * (s - 1.0);
+    }
+    if (pp > 0.0) q = -q;
+    pp = fabs(pp);
+    const double min1 = 3.0 * xm * q - fabs(tol1 * q);
+    const double min2 = fabs(B.e * q);
+    if (2.0 * pp < (min1 < min2 ? min1 : min2)) { B.e = B.d; B.d = pp / q; }
+    else { B.d = xm; B.e = B.d; }
+  } else { B.d = xm; B.e = B.d; }
+  B.a = B.b; B.fa = B.fb;
+  if (fabs(B.d) > tol1) B.b += B.d;
+  else B.b += (xm >= 0.0 ? fabs(tol1) : -fabs(tol1));
+  B.x = B.b;
+  B.phase = 2;
+}
+
+__global__ void nlte_brent_kernel(BrentState* br, const double* fval, int nz, int32_t* active)
+{
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= nz) return;
+  BrentState& B = br[z];
+  if (B.done) return;
+  const double f = fval[z];
+  if (B.phase == 0) {
+    B.fa = f;
+    B.x = B.b;
+    B.phase = 1;
+  } else if (B.phase == 1) {
+    B.fb = f;
+    if ((B.fa > 0.0 && B.fb > 0.0) || (B.fa < 0.0 && B.fb < 0.0)) B.err = 1;
+    B.fc = B.fb;
+    B.iter = 1;
+    brent_head(B);
+  } else {
+    B.fb = f;
+    if (++B.iter > 100) { B.err = 2; B.root = 0.0; B.done = 1; }
+    else brent_head(B);
+  }
+  if (!B.done) atomicAdd(active, 1);
+}
+
+// outputs of the converged state, and the free-free and bound-free factors
+// of the bins kernel from it (the zone's last evaluation)
+__global__ void __launch_bounds__(WAVE) nlte_finish_kernel(NlteArgs A)
+{
+  const int z = blockIdx.x, lane = threadIdx.x;
+  const sgpu_atoms& at = A.at;
+  const BrentState& B = A.br[z];
+  const bool gas = !(A.zs.grey_opacity && A.zs.grey_opacity[z] != 0) && at.n_atoms > 0;
+  for (int k = lane; k < A.n_ion; k += WAVE) {
+    if (A.out.ion_frac) A.out.ion_frac[(size_t)z * A.n_ion + k] = gas ? A.frac[(size_t)z * A.n_ion + k] : 0;
+    if (A.out.ion_part) A.out.ion_part[(size_t)z * A.n_ion + k] = gas ? A.upart[(size_t)z * A.n_ion + k] : 0;
+  }
+  if (A.out.lev_n)
+    for (int k = lane; k < A.n_lev; k += WAVE) A.out.lev_n[(size_t)z * A.n_lev + k] = gas ? A.levn[(size_t)z * A.n_lev + k] : 0;
+  const double T = A.temp_used[z];
+  const double ne = gas ? B.root : 0.0;
+  if (lane == 0) {
+    A.out.n_elec[z] = ne;
+    if (A.out.zone_status) A.out.zone_status[z] = gas ? B.err : 0;
+  }
+  if (!gas) {
+    if (lane == 0 && A.fffac) A.fffac[z] = 0;
+    return;
+  }
+  if (A.phifac) {
+    const double lam_t = sqrt(H_PLANCK * H_PLANCK / (2 * PI * M_ELECTRON * K_BOLTZ * T));
+    for (int a = 0; a < at.n_atoms; ++a) {
+      const double n_dens = A.ndens[(size_t)z * at.n_atoms + a];
+      for (int l = at.lev_off[a] + lane; l < at.lev_off[a + 1]; l += WAVE) {
+        const int kc = continuum_level(at, a, l);
+        double ph = -1;
+        if (kc != -1) {
+          const double nc = n_dens * A.levn[(size_t)z * A.n_lev + kc];
+          const double gl_o_gc = (1.0 * at.lev_g[l]) / (1.0 * at.lev_g[kc]);
+          ph = nc * gl_o_gc / 2. * lam_t * lam_t * lam_t;
+        }
+        A.phifac[(size_t)z * A.n_lev + l] = ph;
+      }
+    }
+  }
+  if (A.fffac && lane == 0) {
+    double fac = 0;
+    for (int a = 0; a < at.n_atoms; ++a) {
+      double Z_eff_sq = 0;
+      const int i0 = at.ion_off[a], ni = at.ion_off[a + 1] - i0;
+      for (int j = 0; j < ni; j++) Z_eff_sq += A.frac[(size_t)z * A.n_ion + i0 + j] * j * j;
+      fac += A.ndens[(size_t)z * at.n_atoms + a] * Z_eff_sq;
+    }
+    A.fffac[z] = fac * (3.7e8 * pow(T, -0.5) * ne);
+  }
+}
+
+__global__ void gather_zone_kernel(const double* src, const int32_t* idx, int n, double* dst)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) dst[s] = src[idx[s]];
+}
+
+__global__ void zero_i32_kernel(int32_t* p) { p[0] = 0; }
+
 }  // namespace sgpu
 
 using namespace sgpu;
@@ -661,6 +959,161 @@ extern "C" int sgpu_opacity_build(sgpu_ctx* ctx, const sgpu_locate* nu_grid,
   timer_end(ctx, "opacity");
   if (!out->zone_status) return SGPU_OK;
   SGPU_HIP(hipMemsetAsync(worst, 0, sizeof(int32_t), ctx->stream));
+  hipLaunchKernelGGL(max_status_kernel, dim3((nz + 255) / 256), dim3(256), 0, ctx->stream,
+                     out->zone_status, nz, worst);
+  SGPU_LAUNCH_CHECK();
+  int32_t hw = 0;
+  SGPU_HIP(hipMemcpyAsync(&hw, worst, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  SGPU_HIP(hipStreamSynchronize(ctx->stream));
+  return hw;
+}
+
+extern "C" int sgpu_opacity_build_nlte(sgpu_ctx* ctx, const sgpu_locate* nu_grid, const sgpu_zone_state* zones,
+                                       const sgpu_atoms* atoms, const sgpu_opacity_params* params,
+                                       const double* J_nu, const int32_t* atom_nlte, int32_t use_collisions,
+                                       int32_t lu_precision, sgpu_opacity_out* out, int32_t* n_evals)
+{
+  SGPU_REQUIRE(ctx && nu_grid && zones && params && out && atoms && atom_nlte, "null argument");
+  SGPU_REQUIRE(lu_precision == SGPU_NLTE_FP64 || lu_precision == SGPU_NLTE_FP32_REFINE,
+               "unknown LU precision %d", lu_precision);
+  const sgpu_atoms& at = *atoms;
+  int n_nlte = 0;
+  if (at.n_atoms > 0)
+    for (int a = 0; a < at.n_atoms; ++a) n_nlte += atom_nlte[a] != 0;
+  if (n_evals) *n_evals = 0;
+  // transport_opacity.cpp:30-36: LTE on the first step; no NLTE atom: LTE
+  if (params->first_step || n_nlte == 0) return sgpu_opacity_build(ctx, nu_grid, zones, atoms, params, out);
+  SGPU_REQUIRE(J_nu, "J_nu missing");
+  SGPU_REQUIRE(out->abs && out->scat && out->emis_cdf && out->L_thermal && out->planck_mean &&
+               out->rosseland_mean && out->n_elec && out->compton_opac && out->photoion_opac &&
+               out->eps_imc, "missing opacity output array");
+  SGPU_REQUIRE(nu_grid->n >= 1 && nu_grid->x, "bad frequency grid");
+  const int nz = zones->n_zones;
+  SGPU_REQUIRE(nz >= 0, "negative zone count");
+  if (nz == 0) return SGPU_OK;
+  SGPU_REQUIRE(zones->n_elems >= 1 && zones->n_elems <= MAX_ELEMS, "n_elems %d outside [1, %d]",
+               zones->n_elems, MAX_ELEMS);
+  SGPU_REQUIRE(zones->rho && zones->T_gas && zones->X && zones->elem_Z && zones->elem_A,
+               "missing zone arrays");
+  SGPU_REQUIRE(params->fleck_alpha == 0 || params->radiative_eq || zones->e_gas,
+               "fleck_alpha != 0 needs e_gas");
+  SGPU_REQUIRE(at.n_atoms == zones->n_elems, "one atom per element required (%d atoms, %d elements)",
+               at.n_atoms, zones->n_elems);
+  SGPU_REQUIRE(at.max_levels >= 1, "max_levels must be set");
+  const int n_lev = at.n_levels, n_ion = at.n_ions;
+  SGPU_REQUIRE(n_lev >= at.n_atoms && n_ion >= at.n_atoms, "n_levels / n_ions not set");
+  SGPU_REQUIRE(n_lev <= MAX_LEV && n_ion <= MAX_ION, "atomic data too large: %d levels (max %d), %d ions (max %d)",
+               n_lev, MAX_LEV, n_ion, MAX_ION);
+  const bool lexp = params->use_line_expansion && at.n_blines > 0;
+  if (lexp)
+    SGPU_REQUIRE(at.bline_l && at.bline_u && at.bline_nu && at.bline_f && at.bline_bin && at.bline_bin_off,
+                 "line expansion needs the level-resolved line arrays");
+
+  // (zone, NLTE atom) systems, zone-major
+  std::vector<int32_t> h_atom, h_zone, h_n, h_sys_of((size_t)nz * at.n_atoms, -1);
+  std::vector<int64_t> h_roff(1, 0), h_voff(1, 0);
+  std::vector<int32_t> lev_off(at.n_atoms + 1);
+  SGPU_HIP(hipMemcpy(lev_off.data(), at.lev_off, sizeof(int32_t) * (at.n_atoms + 1), hipMemcpyDeviceToHost));
+  for (int z = 0; z < nz; ++z)
+    for (int a = 0; a < at.n_atoms; ++a) {
+      if (!atom_nlte[a]) continue;
+      const int n = lev_off[a + 1] - lev_off[a];
+      h_sys_of[(size_t)z * at.n_atoms + a] = (int32_t)h_atom.size();
+      h_atom.push_back(a);
+      h_zone.push_back(z);
+      h_n.push_back(n);
+      h_roff.push_back(h_roff.back() + (int64_t)n * n);
+      h_voff.push_back(h_voff.back() + n);
+    }
+  const int nsys = (int)h_atom.size();
+  const size_t nbl = at.n_blines > 0 ? (size_t)at.n_blines : 1;
+  int st = 0;
+  auto dbl = [&](const char* name, size_t n) { return (double*)scratch(ctx, name, sizeof(double) * (n ? n : 1), &st); };
+  auto i32 = [&](const char* name, size_t n) { return (int32_t*)scratch(ctx, name, sizeof(int32_t) * (n ? n : 1), &st); };
+  int32_t* d_atom = i32("opn_sys_atom", nsys);
+  int32_t* d_zone = i32("opn_sys_zone", nsys);
+  int32_t* d_n = i32("opn_sys_n", nsys);
+  int32_t* d_sys_of = i32("opn_sys_of", (size_t)nz * at.n_atoms);
+  int64_t* d_roff = (int64_t*)scratch(ctx, "opn_roff", sizeof(int64_t) * (nsys + 1), &st);
+  int64_t* d_voff = (int64_t*)scratch(ctx, "opn_voff", sizeof(int64_t) * (nsys + 1), &st);
+  double* temp_used = dbl("opn_temp", nz);
+  double* nion = dbl("opn_nion", (size_t)nz * n_ion);
+  double* upart = dbl("opn_upart", (size_t)nz * n_ion);
+  double* frac = dbl("opn_frac", (size_t)nz * n_ion);
+  double* levn = dbl("opn_levn", (size_t)nz * n_lev);
+  double* ndens = dbl("opn_ndens", (size_t)nz * at.n_atoms);
+  double* x_lte = dbl("opn_xlte", (size_t)nz * at.n_atoms);
+  double* fval = dbl("opn_fval", nz);
+  BrentState* br = (BrentState*)scratch(ctx, "opn_brent", sizeof(BrentState) * nz, &st);
+  double* sys_T = dbl("opn_sys_T", nsys);
+  double* sys_ne = dbl("opn_sys_ne", nsys);
+  double* lev_lte = dbl("opn_lev_lte", (size_t)nsys * n_lev);
+  double* Pic = dbl("opn_pic", (size_t)nsys * n_lev);
+  double* Rci = dbl("opn_rci", (size_t)nsys * n_lev);
+  double* line_J = dbl("opn_line_J", (size_t)nsys * nbl);
+  double* rates = dbl("opn_rates", (size_t)h_roff.back());
+  double* lte = dbl("opn_lte", (size_t)h_voff.back());
+  double* bvec = dbl("opn_b", (size_t)h_voff.back());
+  double* sys_levn = dbl("opn_sys_levn", (size_t)nsys * n_lev);
+  double* sys_frac = dbl("opn_sys_frac", (size_t)nsys * n_ion);
+  int32_t* active = i32("opn_active", 1);
+  int32_t* worst = i32("opn_worst", 1);
+  double* fffac = params->use_free_free ? dbl("opn_fffac", nz) : nullptr;
+  double* phifac = params->use_bound_free ? dbl("opn_phifac", (size_t)nz * n_lev) : nullptr;
+  if (st) return st;
+  SGPU_HIP(hipMemcpyAsync(d_atom, h_atom.data(), sizeof(int32_t) * nsys, hipMemcpyHostToDevice, ctx->stream));
+  SGPU_HIP(hipMemcpyAsync(d_zone, h_zone.data(), sizeof(int32_t) * nsys, hipMemcpyHostToDevice, ctx->stream));
+  SGPU_HIP(hipMemcpyAsync(d_n, h_n.data(), sizeof(int32_t) * nsys, hipMemcpyHostToDevice, ctx->stream));
+  SGPU_HIP(hipMemcpyAsync(d_sys_of, h_sys_of.data(), sizeof(int32_t) * h_sys_of.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+  SGPU_HIP(hipMemcpyAsync(d_roff, h_roff.data(), sizeof(int64_t) * (nsys + 1), hipMemcpyHostToDevice, ctx->stream));
+  SGPU_HIP(hipMemcpyAsync(d_voff, h_voff.data(), sizeof(int64_t) * (nsys + 1), hipMemcpyHostToDevice, ctx->stream));
+  SGPU_HIP(hipStreamSynchronize(ctx->stream));
+
+  timer_begin(ctx, "opacity");
+  NlteArgs A{*zones, at, *params, n_lev, n_ion, d_sys_of, temp_used, br, sys_ne, lev_lte, sys_levn, sys_frac,
+             levn, frac, nion, upart, ndens, x_lte, fval, active, *out, fffac, phifac};
+  hipLaunchKernelGGL(nlte_init_kernel, dim3(nz), dim3(WAVE), 0, ctx->stream, A);
+  hipLaunchKernelGGL(gather_zone_kernel, dim3((nsys + 255) / 256), dim3(256), 0, ctx->stream, temp_used, d_zone,
+                     nsys, sys_T);
+  SGPU_LAUNCH_CHECK();
+  // AtomicSpecies::calculate_radiative_rates once per zone (GasState.cpp:223)
+  st = sgpu_nlte_radiative_rates(ctx, atoms, nu_grid, nsys, d_atom, d_zone, sys_T, zones->elem_A,
+                                 params->line_velocity_width, J_nu, Pic, Rci, line_J);
+  if (st < 0) return st;
+  int evals = 0;
+  for (; evals < 102; ++evals) {
+    hipLaunchKernelGGL(nlte_lte_kernel, dim3(nz), dim3(WAVE), 0, ctx->stream, A);
+    SGPU_LAUNCH_CHECK();
+    st = sgpu_nlte_set_rates(ctx, atoms, nsys, d_atom, sys_ne, sys_T, line_J, Pic, Rci, lev_lte, use_collisions,
+                             params->no_ground_recomb, d_n, d_roff, d_voff, rates, lte);
+    if (st < 0) return st;
+    st = sgpu_nlte_batched(ctx, nsys, d_n, d_roff, d_voff, rates, lte, bvec, nullptr, lu_precision, 1e-13, nullptr);
+    if (st < 0) return st;
+    st = sgpu_nlte_populations(ctx, atoms, nsys, d_atom, d_voff, bvec, lte, sys_levn, sys_frac);
+    if (st < 0) return st;
+    hipLaunchKernelGGL(nlte_charge_kernel, dim3(nz), dim3(WAVE), 0, ctx->stream, A);
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, ctx->stream, active);
+    hipLaunchKernelGGL(nlte_brent_kernel, dim3((nz + 255) / 256), dim3(256), 0, ctx->stream, br, fval, nz, active);
+    SGPU_LAUNCH_CHECK();
+    int32_t h_active = 0;
+    SGPU_HIP(hipMemcpyAsync(&h_active, active, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    SGPU_HIP(hipStreamSynchronize(ctx->stream));
+    if (h_active == 0) { ++evals; break; }
+  }
+  if (n_evals) *n_evals = evals;
+  hipLaunchKernelGGL(nlte_finish_kernel, dim3(nz), dim3(WAVE), 0, ctx->stream, A);
+  SGPU_LAUNCH_CHECK();
+  BinArgs ba{*nu_grid, *zones, at, *params, n_ion, temp_used, nion, upart,
+             out->abs, out->scat, out->emis_cdf, out->n_elec, levn, ndens, n_lev, fffac, phifac};
+  hipLaunchKernelGGL(bins_kernel, dim3((nu_grid->n + 255) / 256, nz), dim3(256), 0, ctx->stream, ba);
+  SGPU_LAUNCH_CHECK();
+  FinArgs fa{*nu_grid, *zones, *params, temp_used, *out};
+  hipLaunchKernelGGL(finalize_kernel, dim3(nz), dim3(FIN_THREADS), 0, ctx->stream, fa);
+  SGPU_LAUNCH_CHECK();
+  timer_end(ctx, "opacity");
+  if (!out->zone_status) return SGPU_OK;
+  hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(1), 0, ctx->stream, worst);
   hipLaunchKernelGGL(max_status_kernel, dim3((nz + 255) / 256), dim3(256), 0, ctx->stream,
                      out->zone_status, nz, worst);
   SGPU_LAUNCH_CHECK();

@@ -119,6 +119,18 @@ int transport::init(const Params& p, const Grid1D& g, const AtomicData* atoms, i
     at_.line_ion = upload(atoms->line_ion);
     at_.line_bin = upload(atoms->line_bin);
     at_.line_bin_off = upload(atoms->line_bin_off);
+    at_.n_blines = (int64_t)atoms->bline_nu.size();
+    if (at_.n_blines > 0) {
+      at_.bline_l = upload(atoms->bline_l);
+      at_.bline_u = upload(atoms->bline_u);
+      at_.bline_nu = upload(atoms->bline_nu);
+      at_.bline_f = upload(atoms->bline_f);
+      at_.bline_A = upload(atoms->bline_A);
+      at_.bline_Bul = upload(atoms->bline_Bul);
+      at_.bline_Blu = upload(atoms->bline_Blu);
+      at_.bline_bin = upload(atoms->bline_bin);
+      at_.bline_bin_off = upload(atoms->bline_bin_off);
+    }
   }
   const size_t T = (size_t)nz_ * nn_;
   auto z = [&](size_t n) { return (double*)dalloc(sizeof(double) * n); };
@@ -179,7 +191,16 @@ int transport::set_opacity(double dt)
   op_.no_ground_recomb = p_.opacity_no_ground_recomb;
   op_.radiative_eq = p_.transport_radiative_equilibrium;
   op_.first_step = first_step_;
-  int rc = sgpu_opacity_build(ctx_, &nu_, &zs_, have_atoms_ ? &at_ : nullptr, &op_, &out_);
+  op_.line_velocity_width = p_.line_velocity_width;
+  int rc;
+  // transport_opacity.cpp:30-36, 133-136: the NLTE gas state from the J_nu of
+  // the previous step (LTE on the first step, inside the call)
+  if (p_.opacity_use_nlte && have_atoms_ && p_.transport_store_Jnu &&
+      (int)p_.opacity_atoms_in_nlte.size() == at_.n_atoms)
+    rc = sgpu_opacity_build_nlte(ctx_, &nu_, &zs_, &at_, &op_, ta_.J_nu, p_.opacity_atoms_in_nlte.data(),
+                                 p_.opacity_use_collisions_nlte, SGPU_NLTE_FP32_REFINE, &out_, nullptr);
+  else
+    rc = sgpu_opacity_build(ctx_, &nu_, &zs_, have_atoms_ ? &at_ : nullptr, &op_, &out_);
   if (rc < 0) return fail(rc, "set_opacity");
   return SGPU_OK;   // positive codes are the reference's solve warnings
 }

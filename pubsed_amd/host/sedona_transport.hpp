@@ -36,6 +36,10 @@ struct Params {
   int opacity_free_free = 0;
   int opacity_bound_free = 0;
   int opacity_no_ground_recomb = 0;
+  int opacity_use_nlte = 0;                   // NLTE gas state from the last step's J_nu
+  std::vector<int32_t> opacity_atoms_in_nlte; // per-atom flags (element order)
+  int opacity_use_collisions_nlte = 1;
+  double line_velocity_width = 0;
   double opacity_maximum_opacity = 1e40;
   double limits_temp_min = 1000;
   double limits_temp_max = 1e8;
@@ -88,6 +92,11 @@ struct AtomicData {
   std::vector<int32_t> lev_off, ion_off, lev_g, lev_ion, atom_zero_eps, line_ion, line_bin;
   std::vector<double> lev_E, ion_chi, line_nu, line_gf, line_El;
   std::vector<int64_t> line_bin_off;
+  // level-resolved lines (AtomicData.h:12-56 lines_): lower / upper level,
+  // frequency, f_lu, A_ul, B_ul, B_lu and their bin CSR
+  std::vector<int32_t> bline_l, bline_u, bline_bin;
+  std::vector<double> bline_nu, bline_f, bline_A, bline_Bul, bline_Blu;
+  std::vector<int64_t> bline_bin_off;
   int max_levels = 0;
 };
 

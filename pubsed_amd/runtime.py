@@ -369,6 +369,22 @@ class GpuBackend:
                                              ctypes.byref(nout)), "sgpu_emit_radioactive")
         return nout.value
 
+    def opacity_build_nlte(self, P, J_nu, atom_nlte, use_collisions=1, lu_precision=None):
+        """sgpu_opacity_build_nlte: J_nu a device tensor [nz][n_nu], atom_nlte a
+        host sequence of per-atom flags.  Returns (status, n_e evaluation passes)."""
+        from . import check
+        import numpy as np
+        fl = np.ascontiguousarray(atom_nlte, dtype=np.int32)
+        ev = ctypes.c_int32(0)
+        prec = abi.NLTE_FP64 if lu_precision is None else lu_precision
+        st = check(self.lib.sgpu_opacity_build_nlte(self.ctx, ctypes.byref(P.nu), ctypes.byref(P.zs),
+                                                     ctypes.byref(P.atoms), ctypes.byref(P.op_params),
+                                                     abi.ptr(J_nu), fl.ctypes.data_as(ctypes.c_void_p),
+                                                     use_collisions, prec,
+                                                     ctypes.byref(P.op_out), ctypes.byref(ev)),
+                   "sgpu_opacity_build_nlte")
+        return st, ev.value
+
     def propagate(self, P, n, counters=None):
         """counters: a torch int64 tensor of 10 entries on the device (or None)."""
         from . import check
