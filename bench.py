@@ -211,6 +211,7 @@ def main():
     nlte = None
     if not a.no_nlte:
         nlte = bench_nlte(gpu, torch, W, abi, np)
+        nlte["opacity_build_nlte"] = bench_nlte_state(gpu, P, torch, abi, np)
 
     cpu = None
     if not a.no_cpu:
@@ -245,6 +246,28 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def bench_nlte_state(gpu, P, torch, abi, np):
+    """The opacity build with every atom in NLTE (sgpu_opacity_build_nlte) on
+    the mixed step's normalised J_nu: its time, the n_e evaluation passes,
+    and (zone, atom) NLTE solves per second (systems x passes / time), with
+    the FP32 + refinement and the FP64 batched LU."""
+    J = P.a["J_nu"].clone()
+    flags = [1] * P.atoms.n_atoms
+    nsys = P.nz * P.atoms.n_atoms
+    P.op_params.first_step = 0
+    out = {"zones": P.nz, "systems_per_pass": nsys}
+    for name, prec in (("fp32_refine", abi.NLTE_FP32_REFINE), ("fp64", abi.NLTE_FP64)):
+        gpu.opacity_build_nlte(P, J, flags, lu_precision=prec)     # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, ev = gpu.opacity_build_nlte(P, J, flags, lu_precision=prec)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        out[name] = {"ms": 1e3 * el, "passes": ev, "solves_per_s": nsys * ev / el,
+                     "n_elec_mean": float(P.get("n_elec").mean())}
+    return out
 
 
 def bench_time_dependent(a, gpu):
