@@ -196,9 +196,15 @@ def main():
             os.path.relpath(a.traffic_from, ROOT), tinfo["packets"], N)
     op_avg = float(np.mean(op_ms))
     n_lines = P.cfg.get("n_lines", 0) if P.cfg.get("fuzz") else 0
+    # K1-K3 algorithmic bytes per zone: abs, scat and emissivity CDF written
+    # (3 nu 8 B), the B_nu / means pass over them (nu 8 B), and the zone's
+    # full line list streamed once (32 B per line: nu, gf, E_l, ion + bin)
+    b_zone = 3 * P.n_nu * 8 + P.n_nu * 8 + 32 * n_lines
     opacity = {"kernel_ms": op_avg, "zones": P.nz, "lines": n_lines,
                "zone_builds_per_s": P.nz / (op_avg * 1e-3) if op_avg > 0 else None,
-               "line_exp_per_s": 3.0 * n_lines * P.nz / (op_avg * 1e-3) if op_avg > 0 else None}
+               "line_exp_per_s": 3.0 * n_lines * P.nz / (op_avg * 1e-3) if op_avg > 0 else None,
+               "bytes_per_zone": b_zone,
+               "achieved_GBps": b_zone * P.nz / (op_avg * 1e-3) / 1e9 if op_avg > 0 else None}
 
     # plain FP64 drop-in (its rate is linear in the packet count at these sizes)
     Np = min(a.plain_packets or N, N)

@@ -14,6 +14,7 @@
 //   radiative_rates_kernel  AtomicSpecies::calculate_radiative_rates (:202-287):
 //                       photoionisation / recombination integrals over the
 //                       MC J_nu per level and line J-bar by the Voigt profile.
+#include <climits>
 #include <hip/hip_runtime.h>
 #include "sgpu_internal.h"
 #include "sgpu_device.h"
@@ -21,6 +22,9 @@
 
 namespace sgpu {
 
+constexpr int RATE_CHUNK = 1024;   // lines (levels) staged in LDS per chunk
+constexpr int RATE_MAX_STAGES = 256;   // ion stages of one atom held in LDS
+constexpr int RATE_LDS_DOUBLES = 6400;  // matrices up to 80 x 80 are assembled in LDS
 constexpr int RATE_THREADS = 256;
 
 struct RateArgs {
@@ -53,7 +57,13 @@ __global__ void __launch_bounds__(RATE_THREADS) set_rates_kernel(RateArgs A)
     return;
   }
   const double ne = A.ne[s], T = A.T[s];
-  double* R = A.rates + A.rate_off[s];
+  // the matrix is accumulated in LDS when it fits (a row's terms are
+  // read-modify-writes by its owner thread: in LDS they do not wait on
+  // global-memory round trips), else in place in global memory
+  __shared__ double R_lds[RATE_LDS_DOUBLES];
+  double* const R_out = A.rates + A.rate_off[s];
+  const bool in_lds = n * n <= RATE_LDS_DOUBLES;
+  double* R = in_lds ? R_lds : R_out;
   const double* J = A.line_J + (size_t)s * at.n_blines;
   const double* Pic = A.Pic + (size_t)s * at.n_levels;
   const double* Rci = A.Rci + (size_t)s * at.n_levels;
@@ -61,64 +71,114 @@ __global__ void __launch_bounds__(RATE_THREADS) set_rates_kernel(RateArgs A)
   for (int r = t; r < n; r += NT)
     for (int j = 0; j < n; j++) R[(size_t)r * n + j] = 0;
 
-  // radiative bound-bound (:303-321)
-  for (int64_t l = 0; l < at.n_blines; l++) {
-    const int ll = at.bline_l[l] - l0, lu = at.bline_u[l] - l0;
-    if (ll < 0 || ll >= n) continue;
-    const bool own_l = ll % NT == t, own_u = lu % NT == t;
-    if (!own_l && !own_u) continue;
-    double R_ul = at.bline_Bul[l] * J[l] + at.bline_A[l];
-    double R_lu = at.bline_Blu[l] * J[l];
-    if (at.bline_nu[l] == 0) { R_ul = 0; R_lu = 0; }
-    if (own_l) R[(size_t)ll * n + lu] += R_lu;
-    if (own_u) R[(size_t)lu * n + ll] += R_ul;
-  }
-  // collisional bound-bound, van Regemorter (:326-380)
-  if (A.use_collisions) {
-    for (int64_t l = 0; l < at.n_blines; l++) {
-      const int ll = at.bline_l[l] - l0, lu = at.bline_u[l] - l0;
-      if (ll < 0 || ll >= n) continue;
-      const bool own_l = ll % NT == t, own_u = lu % NT == t;
-      if (!own_l && !own_u) continue;
-      const int gl = at.lev_g[ll + l0], gu = at.lev_g[lu + l0];
-      const double dE = (at.lev_E[lu + l0] - at.lev_E[ll + l0]) * EV_TO_ERGS;
-      const double zeta = dE / K_BOLTZ / T;
-      const double ezeta = exp(zeta);
-      const double f_lu = at.bline_f[l];
-      const double eff_f = f_lu < 1.e-3 ? 1.e-3 : f_lu;
-      if (own_l) {
-        double C_up = 3.9 * pow(zeta, -1.) * pow(T, -1.5) / ezeta * ne * eff_f;
-        if (zeta > 700) C_up = 0;
-        R[(size_t)ll * n + lu] += C_up;
+  // The line list is read in chunks staged in LDS by the whole workgroup:
+  // one thread per line computes the line's two terms (their transcendentals
+  // run in parallel over the lines), then each row's owner walks the chunk
+  // in stored order and adds the terms of its row, so every matrix element
+  // still sums its terms in the reference's order: all radiative terms
+  // (:303-321), then all collisional ones (van Regemorter, :326-380).
+  __shared__ int32_t ch_l[RATE_CHUNK], ch_u[RATE_CHUNK];
+  __shared__ double ch_lu[RATE_CHUNK], ch_ul[RATE_CHUNK];   // l -> u and u -> l terms
+  __shared__ double bf_up[RATE_CHUNK], bf_down[RATE_CHUNK];
+  for (int pass = 0; pass < (A.use_collisions ? 2 : 1); ++pass) {
+    for (int64_t c0 = 0; c0 < at.n_blines; c0 += RATE_CHUNK) {
+      const int m = (int)min((int64_t)RATE_CHUNK, at.n_blines - c0);
+      __syncthreads();
+      for (int q = t; q < m; q += NT) {
+        const int64_t l = c0 + q;
+        const int ll = at.bline_l[l] - l0, lu = at.bline_u[l] - l0;
+        ch_l[q] = ll;
+        ch_u[q] = lu;
+        double t_lu = 0, t_ul = 0;
+        if (ll >= 0 && ll < n) {
+          if (pass == 0) {
+            t_ul = at.bline_Bul[l] * J[l] + at.bline_A[l];
+            t_lu = at.bline_Blu[l] * J[l];
+            if (at.bline_nu[l] == 0) { t_ul = 0; t_lu = 0; }
+          } else {
+            const int gl = at.lev_g[ll + l0], gu = at.lev_g[lu + l0];
+            const double dE = (at.lev_E[lu + l0] - at.lev_E[ll + l0]) * EV_TO_ERGS;
+            const double zeta = dE / K_BOLTZ / T;
+            const double ezeta = exp(zeta);
+            const double f_lu = at.bline_f[l];
+            const double eff_f = f_lu < 1.e-3 ? 1.e-3 : f_lu;
+            t_lu = 3.9 * pow(zeta, -1.) * pow(T, -1.5) / ezeta * ne * eff_f;
+            if (zeta > 700) t_lu = 0;
+            t_ul = 3.9 * pow(zeta, -1.) * pow(T, -1.5) * ne * eff_f * gl / gu;
+          }
+        }
+        ch_lu[q] = t_lu;
+        ch_ul[q] = t_ul;
       }
-      if (own_u) {
-        const double C_down = 3.9 * pow(zeta, -1.) * pow(T, -1.5) * ne * eff_f * gl / gu;
-        R[(size_t)lu * n + ll] += C_down;
-      }
-    }
-  }
-  // bound-free (:386-423)
-  for (int i = 0; i < n; ++i) {
-    const int k = i + l0;
-    const bool own_i = i % NT == t;
-    const int kc = continuum_level(at, a, k);
-    if (kc == -1) continue;
-    const int ic = kc - l0;
-    const bool own_c = ic % NT == t;
-    if (!own_i && !own_c) continue;
-    const double chi = at.ion_chi[at.ion_off[a] + at.lev_ion[k]] - at.lev_E[k];
-    const double zeta = chi / K_EV / T;
-    if (A.use_collisions) {
-      if (own_i) R[(size_t)i * n + ic] += 2.7 / zeta / zeta * pow(T, -1.5) * exp(-zeta) * ne;
-      if (own_c) {
-        const int gi = at.lev_g[k], gc = at.lev_g[kc];
-        R[(size_t)ic * n + i] += 5.59080e-16 / zeta / zeta * pow(T, -3) * gi / gc * ne * ne;
+      __syncthreads();
+      for (int q = 0; q < m; q++) {
+        const int ll = ch_l[q], lu = ch_u[q];
+        if (ll < 0 || ll >= n) continue;
+        if (ll % NT == t) R[(size_t)ll * n + lu] += ch_lu[q];
+        if (lu % NT == t) R[(size_t)lu * n + ll] += ch_ul[q];
       }
     }
-    double rci = Rci[k];
-    if (A.no_ground_recomb && at.lev_E[k] == 0) rci = 0.;
-    if (own_c) R[(size_t)ic * n + i] += rci * ne;
-    if (own_i) R[(size_t)i * n + ic] += Pic[k];
+  }
+  // bound-free (:386-423).  The continuum level of level k is the first
+  // level of the next ion stage (continuum_level): the first level of every
+  // stage is found once, in LDS.  One thread per level computes the level's
+  // terms; the owners of the two rows then add them in level order.
+  __shared__ int first_lev[RATE_MAX_STAGES];
+  const int i0a = at.ion_off[a], n_st = at.ion_off[a + 1] - i0a;
+  __syncthreads();
+  for (int q = t; q < n_st && q < RATE_MAX_STAGES; q += NT) first_lev[q] = INT_MAX;
+  __syncthreads();
+  for (int i = t; i < n; i += NT) {
+    const int st = at.lev_ion[i + l0];
+    if (st >= 0 && st < n_st && st < RATE_MAX_STAGES) atomicMin(&first_lev[st], i + l0);
+  }
+  __syncthreads();
+  for (int c0 = 0; c0 < n; c0 += RATE_CHUNK) {
+    const int m = min(RATE_CHUNK, n - c0);
+    __syncthreads();
+    for (int q = t; q < m; q += NT) {
+      const int i = c0 + q, k = i + l0;
+      const int nxt = at.lev_ion[k] + 1;
+      const int kc = (nxt < n_st && nxt < RATE_MAX_STAGES && first_lev[nxt] != INT_MAX) ? first_lev[nxt]
+                     : (nxt < RATE_MAX_STAGES ? -1 : continuum_level(at, a, k));
+      ch_l[q] = kc == -1 ? -1 : kc - l0;
+      double up = 0, down = 0;   // i -> c (collisional ionisation + Pic), c -> i
+      if (kc != -1) {
+        const double chi = at.ion_chi[i0a + at.lev_ion[k]] - at.lev_E[k];
+        const double zeta = chi / K_EV / T;
+        double c_ion = 0, c_rec = 0;
+        if (A.use_collisions) {
+          c_ion = 2.7 / zeta / zeta * pow(T, -1.5) * exp(-zeta) * ne;
+          const int gi = at.lev_g[k], gc = at.lev_g[kc];
+          c_rec = 5.59080e-16 / zeta / zeta * pow(T, -3) * gi / gc * ne * ne;
+        }
+        double rci = Rci[k];
+        if (A.no_ground_recomb && at.lev_E[k] == 0) rci = 0.;
+        // two terms per element in the reference's order: collisional
+        // first, then the radiative one
+        ch_lu[q] = c_ion;
+        ch_ul[q] = c_rec;
+        up = Pic[k];
+        down = rci * ne;
+      } else {
+        ch_lu[q] = 0;
+        ch_ul[q] = 0;
+      }
+      bf_up[q] = up;
+      bf_down[q] = down;
+    }
+    __syncthreads();
+    for (int q = 0; q < m; q++) {
+      const int ic = ch_l[q];
+      if (ic < 0) continue;
+      const int i = c0 + q;
+      if (A.use_collisions) {
+        if (i % NT == t) R[(size_t)i * n + ic] += ch_lu[q];
+        if (ic % NT == t) R[(size_t)ic * n + i] += ch_ul[q];
+      }
+      if (ic % NT == t) R[(size_t)ic * n + i] += bf_down[q];
+      if (i % NT == t) R[(size_t)i * n + ic] += bf_up[q];
+    }
   }
   // times the LTE population of the source level (:429-431)
   double* lo = A.lte + A.vec_off[s];
@@ -126,6 +186,10 @@ __global__ void __launch_bounds__(RATE_THREADS) set_rates_kernel(RateArgs A)
     const double w = lte[r + l0];
     for (int j = 0; j < n; j++) R[(size_t)r * n + j] *= w;
     lo[r] = w;
+  }
+  if (in_lds) {
+    __syncthreads();
+    for (int q = t; q < n * n; q += NT) R_out[q] = R_lds[q];
   }
 }
 

@@ -198,7 +198,7 @@ int transport::set_opacity(double dt)
   if (p_.opacity_use_nlte && have_atoms_ && p_.transport_store_Jnu &&
       (int)p_.opacity_atoms_in_nlte.size() == at_.n_atoms)
     rc = sgpu_opacity_build_nlte(ctx_, &nu_, &zs_, &at_, &op_, ta_.J_nu, p_.opacity_atoms_in_nlte.data(),
-                                 p_.opacity_use_collisions_nlte, SGPU_NLTE_FP32_REFINE, &out_, nullptr);
+                                 p_.opacity_use_collisions_nlte, SGPU_NLTE_FP64, &out_, nullptr);
   else
     rc = sgpu_opacity_build(ctx_, &nu_, &zs_, have_atoms_ ? &at_ : nullptr, &op_, &out_);
   if (rc < 0) return fail(rc, "set_opacity");
