@@ -502,6 +502,21 @@ int sgpu_nlte_autotune(sgpu_ctx* ctx, int32_t nsys, const int32_t* n,
 /* ------------------------------------------------------------------ */
 /* radiative-equilibrium gas temperature                               */
 /* ------------------------------------------------------------------ */
+/* Walker alias tables of the zones' emissivity distributions -- the
+ * frequency sampler of the FP32 propagation form, standing in for the
+ * binary search of transport::sample_photon_frequency over emissivity_[i]
+ * (emission.cpp:27-46).  Column k of zone z keeps bin k with probability
+ * q and otherwise samples bin alias; the distribution is p_k = cdf_k -
+ * cdf_{k-1} over the normalised CDF emis_cdf [n_zones][n_nu] (device), with
+ * the last bin taking the remainder below 1.  table [n_zones][n_nu] (device).
+ * n_nu <= 4096. */
+typedef struct {
+  float q;
+  int32_t alias;
+} sgpu_alias_entry;
+int sgpu_alias_tables(sgpu_ctx* ctx, const double* emis_cdf, int32_t n_zones, int32_t n_nu,
+                      sgpu_alias_entry* table);
+
 /* transport::solve_eq_temperature (solve_equilibrium.cpp:66-94): per zone,
  * Brent on [temp_min, temp_max] (tolerance 1e-2 K, EPS 3e-8, at most 100
  * iterations, then 0.0 as the reference returns) for the T at which the

@@ -60,6 +60,7 @@ def _declare(lib):
         "sgpu_nlte_populations": (I, [V, P(abi.Atoms), I32, V, V, V, V, V, V]),
         "sgpu_solve_eq_temperature": (I, [V, P(abi.Locate), I32, V, V, V, D, D, I32, V, V]),
         "sgpu_nlte_radiative_rates": (I, [V, P(abi.Atoms), P(abi.Locate), I32, V, V, V, V, D, V, V, V, V]),
+        "sgpu_alias_tables": (I, [V, V, I32, I32, V]),
         "sgpu_opacity_build_nlte": (I, [V, P(abi.Locate), P(abi.ZoneState), P(abi.Atoms),
                                         P(abi.OpacityParams), V, V, I32, I32, P(abi.OpacityOut), V]),
         "sgpu_adjoint_forward": (I, [V, P(abi.Grid), P(abi.Locate), P(abi.Locate), V, V, V,

@@ -124,7 +124,7 @@ EXPORTED_SYMBOLS = [
     "sgpu_normalize_tallies", "sgpu_nlte_solve", "sgpu_nlte_batched", "sgpu_nlte_autotune",
     "sgpu_homologous_step", "sgpu_adjoint_forward", "sgpu_adjoint_backward",
     "sgpu_nlte_set_rates", "sgpu_nlte_populations", "sgpu_solve_eq_temperature",
-    "sgpu_nlte_radiative_rates", "sgpu_opacity_build_nlte",
+    "sgpu_nlte_radiative_rates", "sgpu_opacity_build_nlte", "sgpu_alias_tables",
 ]
 
 

@@ -898,26 +898,33 @@ __global__ void pack_tables_kernel(const double* abs, const double* scat, int om
   ke[k] = make_float2((float)op, op == 0 ? 0.f : (float)(a / op));
 }
 
-// Alias table of one zone's emissivity distribution (Vose 1991), in FP64,
-// one workgroup per zone with the scaled probabilities and the two work
-// stacks in LDS (nn <= ALIAS_LDS_MAX); lane 0 pairs the columns.  The
+// Alias table of one zone's emissivity distribution (Walker 1977), in FP64,
+// one wave per zone with the scaled probabilities in LDS (nn <= ALIAS_LDS_MAX).
+// The wave classifies the columns and compacts the small (p n < 1) and large
+// ones into two index-ordered lists (ballot prefix sums); lane 0 then sweeps
+// both lists once (the stack-free form of Vose's pairing, as in
+// Huebschle-Schneider & Sanders 2019): the current large column fills the
+// next small one, r <- (r + p_small) - 1 as Vose updates it, and when r
+// drops below 1 the large column is itself filled from the next large one.
+// Every column ends with its own threshold and alias, so the table samples
+// the distribution exactly; leftovers (rounding) get threshold 1.  The
 // distribution is the one upper_bound over the normalised CDF samples,
 // including the last bin taking the remainder when cdf[nn-1] < 1 (the
 // min(upper_bound, nn - 1) clamp); a zone with no emission sends every
 // column to the last bin, as upper_bound on an all-zero CDF does.
-constexpr int ALIAS_LDS_MAX = 4096;
+constexpr int ALIAS_LDS_MAX = 4096;   // columns (the lists hold 16-bit indices)
 
 __global__ void __launch_bounds__(64) alias_kernel(const double* cdf, int nn, AliasEntry* tab)
 {
   extern __shared__ double alds[];
   double* sc = alds;                              // scaled probabilities
-  int32_t* small = (int32_t*)(alds + nn);
-  int32_t* large = small + nn;
-  const int z = blockIdx.x;
+  uint16_t* small = (uint16_t*)(alds + nn);       // small columns, ascending
+  uint16_t* large = small + nn;                   // large columns, ascending
+  const int z = blockIdx.x, lane = threadIdx.x;
   const double* c = cdf + (size_t)z * nn;
   AliasEntry* t = tab + (size_t)z * nn;
   double part = 0;
-  for (int k = threadIdx.x; k < nn; k += blockDim.x) {
+  for (int k = lane; k < nn; k += WAVE) {
     double lo = k == 0 ? 0.0 : c[k - 1];
     double pk = (k == nn - 1 ? 1.0 : c[k]) - lo;
     if (!(pk > 0)) pk = 0;   // also a NaN CDF
@@ -925,26 +932,64 @@ __global__ void __launch_bounds__(64) alias_kernel(const double* cdf, int nn, Al
     part += pk;
   }
   const double tot = wave_sum(part);
-  __syncthreads();
-  if (threadIdx.x != 0) return;
   if (!(tot > 0)) {
-    for (int k = 0; k < nn; k++) t[k] = AliasEntry{0.f, nn - 1};
+    for (int k = lane; k < nn; k += WAVE) t[k] = AliasEntry{0.f, nn - 1};
     return;
   }
-  int ns = 0, nl = 0;
   const double scale = (double)nn / tot;
-  for (int k = 0; k < nn; k++) {
-    sc[k] *= scale;
-    if (sc[k] < 1.0) small[ns++] = k; else large[nl++] = k;
+  int ns = 0, nl = 0;
+  for (int k0 = 0; k0 < nn; k0 += WAVE) {
+    const int k = k0 + lane;
+    bool is_small = false, valid = k < nn;
+    if (valid) {
+      const double v = sc[k] * scale;
+      sc[k] = v;
+      is_small = v < 1.0;
+    }
+    const uint64_t bs = __ballot(valid && is_small), bl = __ballot(valid && !is_small);
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    if (valid && is_small) small[ns + __popcll(bs & below)] = (uint16_t)k;
+    if (valid && !is_small) large[nl + __popcll(bl & below)] = (uint16_t)k;
+    ns += __popcll(bs);
+    nl += __popcll(bl);
   }
-  while (ns > 0 && nl > 0) {
-    const int l = small[--ns], g = large[--nl];
-    t[l] = AliasEntry{(float)sc[l], g};
-    sc[g] = (sc[g] + sc[l]) - 1.0;
-    if (sc[g] < 1.0) small[ns++] = g; else large[nl++] = g;
+  __syncthreads();
+  if (lane != 0) return;
+  int is = 0, il = 0;
+  if (nl > 0) {
+    int g = large[0];
+    double r = sc[g];
+    // the next small and large columns and their weights are read one step
+    // ahead, so the LDS loads overlap the dependent update of r
+    int l = ns > 0 ? small[0] : 0;
+    double pl = sc[l];
+    int g2 = nl > 1 ? large[1] : g;
+    double p2 = sc[g2];
+    while (is < ns) {
+      const int l_next = is + 1 < ns ? small[is + 1] : l;
+      const double pl_next = sc[l_next];
+      t[l] = AliasEntry{(float)pl, g};
+      r = (r + pl) - 1.0;
+      ++is;
+      // the large column has turned small: the next large one fills it
+      while (r < 1.0 && il + 1 < nl) {
+        ++il;
+        const int g3 = il + 1 < nl ? large[il + 1] : g2;
+        const double p3 = sc[g3];
+        t[g] = AliasEntry{(float)r, g2};
+        r = (p2 + r) - 1.0;
+        g = g2;
+        g2 = g3;
+        p2 = p3;
+      }
+      if (r < 1.0) break;            // no large column left to fill from
+      l = l_next;
+      pl = pl_next;
+    }
+    t[g] = AliasEntry{1.f, g};       // the last large column (or a leftover)
+    for (int k = il + 1; k < nl; k++) t[large[k]] = AliasEntry{1.f, large[k]};
   }
-  while (nl > 0) { const int g = large[--nl]; t[g] = AliasEntry{1.f, g}; }
-  while (ns > 0) { const int l = small[--ns]; t[l] = AliasEntry{1.f, l}; }
+  for (; is < ns; is++) t[small[is]] = AliasEntry{1.f, small[is]};
 }
 
 __global__ void to_f32_kernel(const double* src, float* dst, int n)
@@ -1342,7 +1387,7 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
     if (st) return st;
     hipLaunchKernelGGL(pack_tables_kernel, dim3(grid_blocks((int64_t)ntab, 256)), dim3(256), 0, ctx->stream,
                        tables->abs, tables->scat, params->omit_scattering, ntab, ke);
-    hipLaunchKernelGGL(alias_kernel, dim3(nz), dim3(64), (sizeof(double) + 2 * sizeof(int32_t)) * nn,
+    hipLaunchKernelGGL(alias_kernel, dim3(nz), dim3(64), (sizeof(double) + 2 * sizeof(uint16_t)) * nn,
                        ctx->stream, tables->emis_cdf, nn, alias);
     SGPU_LAUNCH_CHECK();
     float* nu32 = e32;
@@ -1549,6 +1594,20 @@ extern "C" int sgpu_normalize_tallies(sgpu_ctx* ctx, const sgpu_grid* grid, cons
                "tally arrays missing");
   hipLaunchKernelGGL(normalize_kernel, dim3(grid->n_zones), dim3(256), 0, ctx->stream, *nu_grid, grid->vol,
                      params->dt, params->store_Jnu, *ta);
+  SGPU_LAUNCH_CHECK();
+  return SGPU_OK;
+}
+
+extern "C" int sgpu_alias_tables(sgpu_ctx* ctx, const double* emis_cdf, int32_t n_zones, int32_t n_nu,
+                                 sgpu_alias_entry* table)
+{
+  SGPU_REQUIRE(ctx && emis_cdf && table, "null argument");
+  SGPU_REQUIRE(n_zones >= 0 && n_nu >= 1 && n_nu <= ALIAS_LDS_MAX, "n_nu = %d outside [1, %d]", n_nu,
+               ALIAS_LDS_MAX);
+  static_assert(sizeof(sgpu_alias_entry) == sizeof(AliasEntry), "alias entry layout");
+  if (n_zones == 0) return SGPU_OK;
+  hipLaunchKernelGGL(alias_kernel, dim3(n_zones), dim3(64), (sizeof(double) + 2 * sizeof(uint16_t)) * n_nu,
+                     ctx->stream, emis_cdf, n_nu, (AliasEntry*)table);
   SGPU_LAUNCH_CHECK();
   return SGPU_OK;
 }
