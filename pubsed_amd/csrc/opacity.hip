@@ -1013,7 +1013,9 @@ extern "C" int sgpu_opacity_build_nlte(sgpu_ctx* ctx, const sgpu_locate* nu_grid
   std::vector<int32_t> h_atom, h_zone, h_n, h_sys_of((size_t)nz * at.n_atoms, -1);
   std::vector<int64_t> h_roff(1, 0), h_voff(1, 0);
   std::vector<int32_t> lev_off(at.n_atoms + 1);
-  SGPU_HIP(hipMemcpy(lev_off.data(), at.lev_off, sizeof(int32_t) * (at.n_atoms + 1), hipMemcpyDeviceToHost));
+  SGPU_HIP(hipMemcpyAsync(lev_off.data(), at.lev_off, sizeof(int32_t) * (at.n_atoms + 1), hipMemcpyDeviceToHost,
+                          ctx->stream));
+  SGPU_HIP(hipStreamSynchronize(ctx->stream));
   for (int z = 0; z < nz; ++z)
     for (int a = 0; a < at.n_atoms; ++a) {
       if (!atom_nlte[a]) continue;
