@@ -25,6 +25,7 @@ namespace sgpu {
 constexpr int RATE_CHUNK = 1024;   // lines (levels) staged in LDS per chunk
 constexpr int RATE_MAX_STAGES = 256;   // ion stages of one atom held in LDS
 constexpr int RATE_LDS_DOUBLES = 6400;  // matrices up to 80 x 80 are assembled in LDS
+constexpr int RATE_MAX_VOIGT = 256;     // Voigt quadrature points (the reference's loop makes 201)
 constexpr int RATE_THREADS = 256;
 
 struct RateArgs {
@@ -280,7 +281,20 @@ __global__ void __launch_bounds__(RATE_THREADS) radiative_rates_kernel(RadArgs A
   const double fac1 = 2 / C_LIGHT / C_LIGHT;
   const double lam_t = sqrt(H_PLANCK * H_PLANCK / (2 * PI * M_ELECTRON * K_BOLTZ * T));
   const double saha_fac = lam_t * lam_t * lam_t / 2.0;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+  const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE, n_wave = blockDim.x / WAVE;
+  // the Voigt abscissae x = -5, -5 + dx, ... as the reference's loop forms
+  // them, by repeated addition (its rounding fixes the points and their count)
+  const double x_max = 5., dx = 0.05;
+  __shared__ double xs[RATE_MAX_VOIGT];
+  __shared__ int n_x;
+  if (threadIdx.x == 0) {
+    int m = 0;
+    for (double x = -1 * x_max; x <= x_max && m < RATE_MAX_VOIGT; x += dx) xs[m++] = x;
+    n_x = m;
+  }
+  __syncthreads();
+  // one wave per level: the lanes share the level's frequency sum
+  for (int j = wave; j < n; j += n_wave) {
     const int k = l0 + j;
     const int kc = continuum_level(at, a, k);
     double P = 0, Rr = 0;
@@ -288,7 +302,7 @@ __global__ void __launch_bounds__(RATE_THREADS) radiative_rates_kernel(RadArgs A
       const int ion = at.lev_ion[k];
       const double chi_ion = at.ion_chi[i0 + ion];
       const double chi = chi_ion - at.lev_E[k];
-      for (int i = 1; i < nn; ++i) {
+      for (int i = 1 + lane; i < nn; i += WAVE) {
         const double nuc = loc_center(A.nu, i);
         const double E_ergs = H_PLANCK * nuc;
         const double E_ev = E_ergs * ERGS_TO_EV;
@@ -299,32 +313,43 @@ __global__ void __launch_bounds__(RATE_THREADS) radiative_rates_kernel(RadArgs A
         P += Jterm * dnu;
         Rr += (sigma * fac1 * nuc * nuc + Jterm) * exp(-1.0 * (E_ev - chi) / K_EV / T) * dnu;
       }
+      P = wave_sum(P);
+      Rr = wave_sum(Rr);
       const double gl_o_gc = (1.0 * at.lev_g[k]) / (1.0 * at.lev_g[kc]);
       P *= 4 * PI;
       Rr *= 4 * PI * gl_o_gc * saha_fac;
     }
-    Pic[k] = P;
-    Rci[k] = Rr;
+    if (lane == 0) {
+      Pic[k] = P;
+      Rci[k] = Rr;
+    }
   }
   double vd = sqrt(2 * K_BOLTZ * T / M_PROTON / A.elem_A[a]);
   if (A.line_velocity_width > 0) vd = A.line_velocity_width;
   const double beta = vd / C_LIGHT;
-  const double x_max = 5., dx = 0.05;
   double* lJ = A.line_J + (size_t)s * at.n_blines;
-  for (int64_t l = threadIdx.x; l < at.n_blines; l += blockDim.x) {
+  const int nx = n_x;
+  // one wave per line: lane q takes the trapezoids (x_{q-1}, x_q), with
+  // f_{-1} = 0 as the reference's J0 starts
+  for (int64_t l = wave; l < at.n_blines; l += n_wave) {
     const int ll = at.bline_l[l] - l0;
     if (ll < 0 || ll >= n) continue;
     const double nu0 = at.bline_nu[l];
     const double nu_d = nu0 * beta;
     const double a_voigt = at.bline_A[l] / 4 / PI / nu_d;
-    double sum = 0, J0 = 0;
-    for (double x = -1 * x_max; x <= x_max; x += dx) {
-      const double phi = voigt_profile(x, a_voigt);
-      const double J1 = J[loc_within(A.nu, nu0 + x * nu_d)] * phi;   // locate_array::value_at
+    double sum = 0;
+    for (int q = lane; q < nx; q += WAVE) {
+      const double x1 = xs[q];
+      const double J1 = J[loc_within(A.nu, nu0 + x1 * nu_d)] * voigt_profile(x1, a_voigt);   // value_at
+      double J0 = 0;
+      if (q > 0) {
+        const double x0 = xs[q - 1];
+        J0 = J[loc_within(A.nu, nu0 + x0 * nu_d)] * voigt_profile(x0, a_voigt);
+      }
       sum += 0.5 * (J1 + J0) * dx;
-      J0 = J1;
     }
-    lJ[l] = sum;
+    sum = wave_sum(sum);
+    if (lane == 0) lJ[l] = sum;
   }
 }
 
