@@ -363,3 +363,26 @@ def test_nlte_pass_in_blackbody_field_returns_lte(gpu):
         bs, ls = b[vo[s]:vo[s + 1]], l[vo[s]:vo[s + 1]]
         big = ls >= 1e-6 * ls.sum()
         assert np.max(np.abs(bs[big] - 1)) < 2e-2, (s, np.max(np.abs(bs[big] - 1)))
+
+
+@pytest.mark.parametrize("precision,tol", [(abi.NLTE_FP64, 1e-10), (abi.NLTE_FP32_REFINE, 1e-8)])
+def test_batched_matches_committed_lapack_fixture(gpu, precision, tol):
+    """The batched solve against LAPACK's solutions of the committed fixture
+    (tests/golden/nlte_lu_systems.npz); 1e-8 is the NLTE error bound of the
+    mixed form."""
+    import os
+    import torch
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "nlte_lu_systems.npz"))
+    ns = len(g["n"])
+    d = {k: torch.from_numpy(np.ascontiguousarray(g[k])).cuda() for k in ("n", "rate_off", "vec_off", "rates", "lte")}
+    b = torch.zeros(len(g["lte"]), dtype=torch.float64, device="cuda:0")
+    it = torch.zeros(ns, dtype=torch.int32, device="cuda:0")
+    print("case golden batched", precision, flush=True)
+    gpu.nlte_batched(ns, d["n"], d["rate_off"], d["vec_off"], d["rates"], d["lte"], b, None, precision, 1e-13, it)
+    gpu.sync()
+    bh = b.cpu().numpy()
+    for s in range(ns):
+        sl = slice(g["vec_off"][s], g["vec_off"][s + 1])
+        e = np.max(np.abs(bh[sl] - g["b"][sl])) / np.max(np.abs(g["b"][sl]))
+        print(s, int(g["n"][s]), e)
+        assert e <= tol, (s, e)
