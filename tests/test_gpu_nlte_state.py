@@ -86,3 +86,27 @@ def test_first_step_and_no_nlte_atom_are_the_lte_build(gpu):
     gpu.sync()
     for k in ("abs", "n_elec", "lev_n"):
         assert np.array_equal(P1.get(k), P2.get(k)), k
+
+
+def test_grey_zones_beside_nlte_zones(gpu):
+    """Zones with a grey opacity skip the gas state (transport_opacity.cpp:
+    114-136); their NLTE systems are carried with finite inputs and never
+    read.  Two of four zones grey: the gas zones match the oracle as in the
+    single-NLTE-atom case, the grey ones match exactly."""
+    Pg = runtime.Problem(runtime.Space("cuda:0"), "c2_line", **OVER)
+    Ph = runtime.Problem(runtime.Space(None), "c2_line", **OVER)
+    grey = np.array([0.0, 0.2, 0.0, 0.2])
+    Pg.set("grey", grey)
+    Ph.set("grey", grey)
+    J = planck_field(Ph, 0.1)
+    print("case nlte build with grey zones", flush=True)
+    gpu.opacity_build_nlte(Pg, torch.from_numpy(np.ascontiguousarray(J)).cuda(), (1, 0))
+    gpu.sync()
+    OracleBackend().opacity_build_nlte(Ph, J, np.array((1, 0), np.int32))
+    ne_g, ne_h = Pg.get("n_elec"), Ph.get("n_elec")
+    assert ne_g[1] == ne_h[1] == 0 and ne_g[3] == ne_h[3] == 0
+    assert relerr(ne_g, ne_h, 1e-300) <= 1e-7
+    nn = Ph.n_nu
+    for z in (1, 3):
+        assert np.allclose(Pg.get("abs")[z * nn:(z + 1) * nn], Ph.get("abs")[z * nn:(z + 1) * nn], rtol=1e-13, atol=0)
+    assert relerr(Pg.get("abs"), Ph.get("abs"), 1e-12 * np.abs(Ph.get("abs")).max()) <= 2e-6
