@@ -302,7 +302,15 @@ def time_step(backend, P, dt, n_emit, scratch_pk, step_index, counters=None):
     backend.homologous_step(P, dt)
     P.tparams.dt = dt
     P.op_params.dt = dt
-    backend.opacity_build(P)
+    if step_index == 0:
+        P.op_params.first_step = 1     # transport_init.cpp:251, cleared at transport.cpp:138
+    nlte = P.cfg.get("nlte_atoms")     # opacity_atoms_in_nlte: per-atom flags
+    if nlte is not None and P.atoms is not None:
+        # transport_opacity.cpp:109-137: the gas state from the previous
+        # step's J_nu (LTE on the first step, inside the call)
+        backend.opacity_build_nlte(P, P.a["J_nu"], nlte, P.cfg.get("use_collisions_nlte", 1))
+    else:
+        backend.opacity_build(P)
     backend.wipe(P)
     n0 = P.n_packets
     got = backend.emit(P, n_emit, id_base=(step_index + 1) << 32, offset=n0)

@@ -114,3 +114,38 @@ def test_final_state_is_charge_balanced(flags):
             nd = rho[z] * mf[a] / (Am[a] * MP)
             f += nd * np.sum(ln[z, off[a]:off[a + 1]] * ion[off[a]:off[a + 1]])
         assert abs(f / P.get("n_elec")[z] - 1) < 1e-5, (z, f, P.get("n_elec")[z])
+
+
+def test_time_step_builds_the_nlte_state_from_the_previous_step():
+    """runtime.time_step with nlte_atoms set: the first step is the LTE build
+    (every table and tally bit for bit the same as an LTE-configured run);
+    the second builds the gas state in NLTE from the J_nu the first step
+    tallied, before the wipe (transport_opacity.cpp:109-137).  (CPU: the
+    step plumbing, with the oracle as the backend.)"""
+    over = dict(steady=0, dt=0.2 * W.DAY, radiative_eq=0)
+    flags = (1, 0)
+    Pa, Pb = problem(nlte_atoms=flags, **over), problem(**over)
+    ob = OracleBackend()
+    n_emit, dt = 3000, 0.2 * W.DAY
+    for P in (Pa, Pb):
+        P.alloc_packets(2 * n_emit)
+        runtime.time_step(ob, P, dt, n_emit, None, 0)
+    for k in KEYS + ("T",):
+        assert np.array_equal(Pa.get(k), Pb.get(k)), k
+    # the oracle's threads add tallies in varying order
+    for k in ("J_nu", "e_abs"):
+        assert np.allclose(Pa.get(k), Pb.get(k), rtol=1e-12, atol=0), k
+    J = Pa.get("J_nu").copy()
+    assert J.max() > 0
+    runtime.time_step(ob, Pa, dt, n_emit, None, 1)
+    # the same build by hand on the twin: grid moved, then NLTE from J
+    ob.homologous_step(Pb, dt)
+    Pb.op_params.dt = dt
+    assert ob.opacity_build_nlte(Pb, J, np.array(flags, np.int32)) == 0
+    for k in KEYS:
+        assert np.array_equal(Pa.get(k), Pb.get(k)), k
+    lte = problem(**over)
+    lte.alloc_packets(2 * n_emit)
+    for s in range(2):
+        runtime.time_step(ob, lte, dt, n_emit, None, s)
+    assert not np.array_equal(Pa.get("n_elec"), lte.get("n_elec"))
