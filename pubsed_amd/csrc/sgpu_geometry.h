@@ -83,6 +83,16 @@ struct GridView {
   Axis<R> r_out, x_out, z_out;
 };
 
+// The mixed form draws the alias column and coin of an event's possible
+// frequency redistribution when the event starts and loads the table entry
+// then, so that the load is in flight during the event's geometry; an event
+// that does not redistribute discards them.  The FP64 form draws in the
+// reference's order and carries nothing.
+struct AliasPre { float q; int32_t a; int k; float coin; bool valid; };
+struct NoPre {};
+template <class R> struct PreOf { typedef NoPre type; };
+template <> struct PreOf<float> { typedef AliasPre type; };
+
 template <class R>
 struct Pkt {
   R x[3], D[3], nu;
@@ -90,11 +100,12 @@ struct Pkt {
   int ind, type;
   R dvds;
   typename RngOf<R>::type rng;
+  typename PreOf<R>::type pre;
 };
 
 template <class R> __device__ __forceinline__ R rsqrt_(R v);
 template <> __device__ __forceinline__ double rsqrt_<double>(double v) { return sqrt(v); }
-template <> __device__ __forceinline__ float rsqrt_<float>(float v) { return sqrtf(v); }
+template <> __device__ __forceinline__ float rsqrt_<float>(float v) { return __builtin_amdgcn_sqrtf(v); }   // v_sqrt_f32, 1 ulp
 template <class R> __device__ __forceinline__ R rlog(R v);
 template <> __device__ __forceinline__ double rlog<double>(double v) { return log(v); }
 template <> __device__ __forceinline__ float rlog<float>(float v) { return __logf(v); }

@@ -274,10 +274,21 @@ __device__ __forceinline__ void sample_photon_frequency(const PropArgs<float>& A
 {
   if (p.type == SGPU_PHOTON) {
     const int nn = A.nu.n;
-    const float u_col = p.rng.uniform_f();
-    const float u_coin = p.rng.uniform_f();
-    const int k = min((int)(u_col * (float)nn), nn - 1);
-    const AliasEntry e = A.tab.alias[(size_t)p.ind * nn + k];
+    int k;
+    float u_coin;
+    AliasEntry e;
+    if (p.pre.valid) {          // drawn and loaded when the event started
+      k = p.pre.k;
+      u_coin = p.pre.coin;
+      e.q = p.pre.q;
+      e.a = p.pre.a;
+      p.pre.valid = false;
+    } else {
+      const float u_col = p.rng.uniform_f();
+      u_coin = p.rng.uniform_f();
+      k = min((int)(u_col * (float)nn), nn - 1);
+      e = A.tab.alias[(size_t)p.ind * nn + k];
+    }
     const int inu = u_coin < e.q ? k : e.a;
     p.nu = ax_sample(A.nu, inu, p.rng.uniform_f());
     c.redist++;
@@ -646,17 +657,29 @@ __device__ __forceinline__ int mc_event(const PropArgs<R>& A, Pkt<R>& p, double 
   const int nn = A.nu.n;
   const R cl = (R)C_LIGHT;
   c.events++;
-  p.rng.begin_event();
+  if constexpr (sizeof(R) == 4) {
+    // a scattering event keeps its zone, so the entry of its redistribution
+    // is known now
+    const float u_col = p.rng.uniform_f();
+    p.pre.coin = p.rng.uniform_f();
+    p.pre.k = min((int)(u_col * (float)nn), nn - 1);
+    const AliasEntry e = A.tab.alias[(size_t)p.ind * nn + p.pre.k];
+    p.pre.q = e.q;
+    p.pre.a = e.a;
+    p.pre.valid = p.type == SGPU_PHOTON;
+  }
+  // the opacity lookup before the boundary geometry (both are pure functions
+  // of the packet), so that its table load is in flight during the geometry
+  R dshift = dshift_lab_to_comoving(A.g, p);
+  R kappa, eps;
+  int i_nu = get_opacity(A, p, dshift, &kappa, &eps);
   R d_bn = 0;
   int new_ind = get_next_zone(A.g, p.x, p.D, p.ind, &d_bn);
   d_bn = floor_boundary_step(p, d_bn);
-  R dshift = dshift_lab_to_comoving(A.g, p);
   bool new_cell_ddmc = false;
   R sig_n = 0, dr_n = 0;
   if constexpr (DDMC)
     if (new_ind >= 0) new_cell_ddmc = ddmc_zone(A, p, new_ind, dshift, &sig_n, &dr_n);
-  R kappa, eps;
-  int i_nu = get_opacity(A, p, dshift, &kappa, &eps);
   R d_nu = qdiv(qdiv(ax_delta(A.nu, i_nu), p.nu) * cl, p.dvds);
   if (p.dvds == 0) d_nu = rinf<R>();
   if (d_nu < 0) d_nu = -1 * d_nu;
@@ -741,9 +764,9 @@ __device__ __forceinline__ void init_rng(Rng& r, uint64_t seed, uint64_t id, uin
   r = make_rng(seed, id, draw);
 }
 
-__device__ __forceinline__ void init_rng(RngF& r, uint64_t seed, uint64_t id, uint32_t draw)
+__device__ __forceinline__ void init_rng(RngX& r, uint64_t seed, uint64_t id, uint32_t draw)
 {
-  r = make_rngf(seed, id, draw);
+  r = make_rngx(seed, id, draw);
 }
 
 template <class R>
@@ -756,6 +779,7 @@ __device__ __forceinline__ void load_packet(const PropArgs<R>& A, int64_t i, Pkt
   p.ind = k.ind[i]; p.type = k.type[i];
   init_rng(p.rng, cold_args<R>()->p.seed, k.id[i], k.draw[i]);
   p.dvds = 0;
+  if constexpr (sizeof(R) == 4) p.pre.valid = false;
 }
 
 template <class R>
