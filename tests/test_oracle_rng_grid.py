@@ -65,12 +65,26 @@ def _loc(g):
     return abi.Locate(g["n"], g["type"], g["min"], g["del_"], x.ctypes.data), x
 
 
-@pytest.mark.parametrize("kind", ["log", "lin", "flex"])
+@pytest.mark.parametrize("kind", ["log", "log_edges", "lin", "flex"])
 def test_locate_brute_force(kind):
+    """locate against upper_bound (searchsorted right) on random values, on
+    every edge and at min.  On the 4096-bin line grid the log formula lands
+    one bin low at 4095 of its 4096 edges (the repeated-multiplication edges
+    sit just below min (1+del)^k), so the verify-and-fall-back branch
+    (locate_array.cpp:191-230) is taken at every one of them -- except at
+    the first edge, where the formula's 0 is returned unverified (the
+    reference verifies only 0 < ind < size): locate(x[0]) = 0 there, one
+    below upper_bound.  On the 1024-bin grid the formula is exact at every
+    edge."""
     lib = oracle_py.load()
     rng = np.random.default_rng(1)
     if kind == "log":
         g = W.nu_log_grid(1e14, 1e16, 1024)
+    elif kind == "log_edges":
+        g = W.nu_log_grid(W.C_LIGHT / 1e-4, W.C_LIGHT / 1e-5, 4096)
+        misses = sum(int(np.floor(np.log(v / g["min"]) / np.log(1 + g["del_"])))
+                     != np.searchsorted(g["x"], v, side="right") for v in g["x"][:-1])
+        assert misses == 4095
     elif kind == "lin":
         g = W.lin_init_del(-3.0, 7.0, 0.37)
     else:
@@ -80,6 +94,8 @@ def test_locate_brute_force(kind):
     vals = np.concatenate([rng.uniform(lo - 0.1 * abs(hi), hi * 1.1, 4000), g["x"], [lo]])
     for v in vals:
         want = 0 if v < lo else int(np.searchsorted(g["x"], v, side="right"))
+        if kind == "log_edges" and v == g["x"][0]:
+            want = 0
         assert lib.or_locate(ctypes.byref(L), float(v)) == want
         assert lib.or_locate_within_bounds(ctypes.byref(L), float(v)) == min(want, g["n"] - 1)
 
