@@ -1442,7 +1442,7 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
     //   SGPU_PROP_WG_PER_CU  persistent workgroups per CU (default 12)
     //   SGPU_PROP_LDS    0: zone tallies as global atomics instead of LDS
     //   SGPU_PROP_OCC    waves per SIMD the register allocation must allow:
-    //                    4 (default: 128 VGPRs, 43 spilled; 1% faster than 3
+    //                    4 (default: 128 VGPRs, 32 spilled; 1% faster than 3
     //                    at 1e7 packets, 6% at 1e6), 3 (168 VGPRs), 1 (no bound)
     const char* es = getenv("SGPU_PROP_SHAPE");
     const int shape = es ? atoi(es) : 0;
