@@ -5,7 +5,12 @@
 // directory, the spectra (spectrum_0.dat, gamma_0.dat) and the zone file
 // plt_00000.dat are written there in the reference's text formats.
 //
-// usage: drive_grey [n_packets] [precision 0|1] [output_dir]
+// With a checkpoint directory as the fourth argument it instead runs three
+// time-dependent steps of dt = 2 d twice -- straight through, and with a
+// checkpoint written after the first step and read into a fresh transport
+// that runs the other two -- and prints both runs' results.
+//
+// usage: drive_grey [n_packets] [precision 0|1] [output_dir] [checkpoint_dir]
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -20,6 +25,7 @@ int main(int argc, char** argv)
   const int64_t n_packets = argc > 1 ? atoll(argv[1]) : 100000;
   const int precision = argc > 2 ? atoi(argv[2]) : SGPU_PREC_MIXED;
   const std::string out_dir = argc > 3 ? argv[3] : "";
+  const std::string chk_dir = argc > 4 ? argv[4] : "";
   const double PI_REF = 3.14159, M_SUN = 1.98892e33, DAY = 86400.0;
   const double kappa_T = 0.66523e-24 * 0.5 / 1.67262158e-24;
   const int nz = 100;
@@ -60,6 +66,40 @@ int main(int argc, char** argv)
   p.precision = precision;
   p.spectrum_name = out_dir.empty() ? "" : out_dir + "/spectrum";
   p.gamma_name = out_dir.empty() ? "" : out_dir + "/gamma";
+
+  if (!chk_dir.empty()) {
+    p.transport_steady_iterate = 0;
+    p.particles_max_total = 3 * n_packets;
+    const double dt = 2 * DAY;
+    auto summary = [&](transport& tr) {
+      double dep = 0, eabs = 0, J = 0;
+      auto L = tr.L_radio_dep();
+      auto A = tr.e_abs();
+      for (int i = 0; i < nz; ++i) { dep += L[i] * vol[i]; eabs += A[i] * vol[i]; }
+      for (double j : tr.J_nu()) J += j;
+      char b[512];
+      snprintf(b, sizeof(b), "{\"alive\": %lld, \"escaped\": %lld, \"t_now\": %.17g, \"L_dep\": %.17g, "
+               "\"E_abs_rate\": %.17g, \"J_sum\": %.17g}", (long long)tr.n_particles(),
+               (long long)tr.n_escaped_last(), tr.t_now(), dep, eabs, J);
+      return std::string(b);
+    };
+    transport a, b, c;
+    if (a.init(p, g, nullptr) || b.init(p, g, nullptr) || c.init(p, g, nullptr)) {
+      fprintf(stderr, "init: %s%s%s\n", a.error().c_str(), b.error().c_str(), c.error().c_str());
+      return 1;
+    }
+    for (int s = 0; s < 3; ++s)
+      if (a.step(dt) != SGPU_OK) { fprintf(stderr, "step: %s\n", a.error().c_str()); return 1; }
+    if (b.step(dt) != SGPU_OK || b.write_checkpoint(chk_dir) != SGPU_OK) {
+      fprintf(stderr, "checkpoint: %s\n", b.error().c_str());
+      return 1;
+    }
+    if (c.read_checkpoint(chk_dir) != SGPU_OK) { fprintf(stderr, "restart: %s\n", c.error().c_str()); return 1; }
+    for (int s = 0; s < 2; ++s)
+      if (c.step(dt) != SGPU_OK) { fprintf(stderr, "step: %s\n", c.error().c_str()); return 1; }
+    printf("{\"straight\": %s, \"restarted\": %s}\n", summary(a).c_str(), summary(c).c_str());
+    return 0;
+  }
 
   transport tr;
   if (tr.init(p, g, nullptr) != SGPU_OK) { fprintf(stderr, "init: %s\n", tr.error().c_str()); return 1; }

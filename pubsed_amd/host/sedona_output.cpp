@@ -2,8 +2,12 @@
 // host from tallies copied back from the device.
 //   write_spectrum_dat: spectrum_array::print (src/transport/spectrum_array.cpp:253-311)
 //   write_plotfile_1d:  grid_1D_sphere::write_plotfile (src/grid/grid_1D_sphere.cpp:598-636)
+// and the .npy datasets of a checkpoint (write_npy / read_npy, NumPy format
+// version 1.0: magic, header length, a Python-literal header dict, raw data).
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include "sedona_transport.hpp"
 
 namespace sedona_gpu {
@@ -70,6 +74,63 @@ int write_plotfile_1d(const std::string& path, double t, double r_min, const std
     fprintf(out, "\n");
   }
   return fclose(out) == 0 ? 0 : -1;
+}
+
+int write_npy(const std::string& path, const char* descr, const std::vector<size_t>& shape, const void* data,
+              size_t bytes)
+{
+  std::string sh = "(";
+  for (size_t i = 0; i < shape.size(); i++) sh += std::to_string(shape[i]) + (shape.size() == 1 ? "," : i + 1 < shape.size() ? ", " : "");
+  sh += ")";
+  std::string hdr = std::string("{'descr': '") + descr + "', 'fortran_order': False, 'shape': " + sh + ", }";
+  // magic (6) + version (2) + length (2) + header, padded with spaces and a
+  // newline to a multiple of 64 bytes
+  size_t total = 10 + hdr.size() + 1;
+  hdr.append((64 - total % 64) % 64, ' ');
+  hdr += '\n';
+  FILE* f = fopen(path.c_str(), "wb");
+  if (!f) return -1;
+  const unsigned char head[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
+  const uint16_t len = (uint16_t)hdr.size();
+  const unsigned char l2[2] = {(unsigned char)(len & 0xff), (unsigned char)(len >> 8)};
+  bool ok = fwrite(head, 1, 8, f) == 8 && fwrite(l2, 1, 2, f) == 2 && fwrite(hdr.data(), 1, hdr.size(), f) == hdr.size() &&
+            (bytes == 0 || fwrite(data, 1, bytes, f) == bytes);
+  return (fclose(f) == 0 && ok) ? 0 : -1;
+}
+
+int read_npy(const std::string& path, const char* descr, std::vector<size_t>* shape, std::vector<char>* data)
+{
+  FILE* f = fopen(path.c_str(), "rb");
+  if (!f) return -1;
+  unsigned char head[10];
+  int rc = -1;
+  if (fread(head, 1, 10, f) == 10 && head[0] == 0x93 && memcmp(head + 1, "NUMPY", 5) == 0 && head[6] == 1) {
+    const size_t len = head[8] | ((size_t)head[9] << 8);
+    std::string hdr(len, '\0');
+    if (fread(&hdr[0], 1, len, f) == len) {
+      const std::string want = std::string("'descr': '") + descr + "'";
+      const size_t sp = hdr.find("'shape': (");
+      if (hdr.find(want) != std::string::npos && hdr.find("'fortran_order': False") != std::string::npos &&
+          sp != std::string::npos) {
+        shape->clear();
+        const char* c = hdr.c_str() + sp + 10;
+        size_t count = 1;
+        while (*c && *c != ')') {
+          char* end;
+          const unsigned long long v = strtoull(c, &end, 10);
+          if (end == c) { ++c; continue; }
+          shape->push_back((size_t)v);
+          count *= (size_t)v;
+          c = end;
+        }
+        size_t item = (size_t)atoi(descr + 2);
+        data->resize(count * item);
+        rc = (count == 0 || fread(data->data(), 1, data->size(), f) == data->size()) ? 0 : -1;
+      }
+    }
+  }
+  fclose(f);
+  return rc;
 }
 
 }  // namespace sedona_gpu

@@ -2,8 +2,12 @@
 // sedona_transport.hpp).  Only C-ABI calls; no HIP or torch types.
 #include "sedona_transport.hpp"
 
+#include <sys/stat.h>
+
+#include <cerrno>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace sedona_gpu {
@@ -308,6 +312,145 @@ int transport::output_spectrum(int it)
       write_spectrum_dat(p_.gamma_name + base, tg, ng, mg, pg, download(ta_.gamma.flux, nn_),
                          download(ta_.gamma.click, nn_)))
     return fail(SGPU_ERR_ARG, "output_spectrum: cannot write the gamma spectrum");
+  return SGPU_OK;
+}
+
+namespace {
+// value of a numeric key in the flat meta.json (the first number after the
+// key's colon; an array's first element); dflt for a missing key or null
+double meta_number(const std::string& js, const char* key, double dflt)
+{
+  const std::string k = std::string("\"") + key + "\"";
+  size_t p = js.find(k);
+  if (p == std::string::npos) return dflt;
+  p = js.find(':', p + k.size());
+  if (p == std::string::npos) return dflt;
+  const char* c = js.c_str() + p + 1;
+  while (*c == ' ' || *c == '\n' || *c == '[') ++c;
+  char* end;
+  const double v = strtod(c, &end);
+  return end == c ? dflt : v;
+}
+
+bool make_dir(const std::string& d) { return mkdir(d.c_str(), 0755) == 0 || errno == EEXIST; }
+}  // namespace
+
+int transport::write_checkpoint(const std::string& dir)
+{
+  if (!make_dir(dir) || !make_dir(dir + "/particles") || !make_dir(dir + "/state"))
+    return fail(SGPU_ERR_ARG, "write_checkpoint: cannot create the directory");
+  const size_t n = (size_t)n_particles_;
+  auto d2h = [&](void* h, const void* d, size_t bytes) { return bytes ? sgpu_memcpy_d2h(ctx_, h, d, bytes) : 0; };
+  std::vector<double> c[9];
+  const double* src[9] = {pk_.x, pk_.y, pk_.z, pk_.Dx, pk_.Dy, pk_.Dz, pk_.t, pk_.e, pk_.nu};
+  for (int k = 0; k < 9; k++) {
+    c[k].resize(n);
+    if (d2h(c[k].data(), src[k], n * 8)) return fail(SGPU_ERR_HIP, "write_checkpoint: copy");
+  }
+  std::vector<int32_t> ind(n), type(n), fate(n);
+  std::vector<uint32_t> draw(n);
+  std::vector<uint64_t> id(n);
+  if (d2h(ind.data(), pk_.ind, n * 4) || d2h(type.data(), pk_.type, n * 4) || d2h(fate.data(), pk_.fate, n * 4) ||
+      d2h(draw.data(), pk_.draw, n * 4) || d2h(id.data(), pk_.id, n * 8))
+    return fail(SGPU_ERR_HIP, "write_checkpoint: copy");
+  std::vector<double> x(3 * n), D(3 * n);
+  for (size_t i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) { x[3 * i + k] = c[k][i]; D[3 * i + k] = c[3 + k][i]; }
+  const std::string pd = dir + "/particles/", sd = dir + "/state/";
+  int bad = 0;
+  bad |= write_npy(pd + "x.npy", "<f8", {n, 3}, x.data(), 24 * n);
+  bad |= write_npy(pd + "D.npy", "<f8", {n, 3}, D.data(), 24 * n);
+  bad |= write_npy(pd + "t.npy", "<f8", {n}, c[6].data(), 8 * n);
+  bad |= write_npy(pd + "e.npy", "<f8", {n}, c[7].data(), 8 * n);
+  bad |= write_npy(pd + "nu.npy", "<f8", {n}, c[8].data(), 8 * n);
+  bad |= write_npy(pd + "type.npy", "<i4", {n}, type.data(), 4 * n);
+  bad |= write_npy(pd + "ind.npy", "<i4", {n}, ind.data(), 4 * n);
+  bad |= write_npy(pd + "fate.npy", "<i4", {n}, fate.data(), 4 * n);
+  bad |= write_npy(pd + "draw.npy", "<u4", {n}, draw.data(), 4 * n);
+  bad |= write_npy(pd + "id.npy", "<u8", {n}, id.data(), 8 * n);
+  const size_t nz = nz_, nzn = (size_t)nz_ * nn_;
+  const struct { const char* name; const double* d; size_t len; } st[] = {
+      {"rho", d_rho_, nz}, {"T", zs_.T_gas, nz}, {"vol", d_vol_, nz}, {"r_out", d_r_out_, nz},
+      {"e_abs", ta_.e_abs, nz}, {"J_nu", ta_.J_nu, nzn}, {"fx_rad", ta_.fx_rad, nz}, {"fy_rad", ta_.fy_rad, nz},
+      {"fz_rad", ta_.fz_rad, nz}, {"fr_rad", ta_.fr_rad, nz}, {"L_radio_dep", ta_.L_radio_dep, nz},
+      {"L_radio_emit", ta_.L_radio_emit, nz}, {"e_rad", ta_.e_rad, nz}};
+  for (const auto& f : st) {
+    std::vector<double> h = download(f.d, f.len);
+    bad |= write_npy(sd + f.name + ".npy", "<f8", {f.len}, h.data(), 8 * f.len);
+  }
+  FILE* m = fopen((dir + "/meta.json").c_str(), "w");
+  if (!m) bad = 1;
+  else {
+    fprintf(m, "{\n \"format\": \"sedona-mi355x checkpoint 1\",\n \"config\": \"host\",\n \"seed\": %llu,\n"
+               " \"n_zones\": %d,\n \"n_nu\": %d,\n \"n_packets\": %lld,\n \"t_now\": %.17g,\n"
+               " \"grid_t_now\": %.17g,\n \"grid_min\": [%.17g, %.17g, %.17g],\n \"first_step\": %d,\n"
+               " \"rng_seed\": %llu,\n \"step_index\": %llu\n}\n",
+            (unsigned long long)p_.transport_rng_seed, nz_, nn_, (long long)n_particles_, t_now_, grid_.t_now,
+            grid_.r_out.min, grid_.x_out.min, grid_.z_out.min, first_step_,
+            (unsigned long long)p_.transport_rng_seed, (unsigned long long)step_index_);
+    bad |= fclose(m) != 0;
+  }
+  return bad ? fail(SGPU_ERR_ARG, "write_checkpoint: cannot write the files") : SGPU_OK;
+}
+
+int transport::read_checkpoint(const std::string& dir)
+{
+  std::string js;
+  if (FILE* m = fopen((dir + "/meta.json").c_str(), "r")) {
+    char buf[4096];
+    size_t k;
+    while ((k = fread(buf, 1, sizeof(buf), m)) > 0) js.append(buf, k);
+    fclose(m);
+  }
+  if (js.find("\"sedona-mi355x checkpoint 1\"") == std::string::npos)
+    return fail(SGPU_ERR_ARG, "read_checkpoint: no checkpoint of this format");
+  if ((int)meta_number(js, "n_zones", -1) != nz_ || (int)meta_number(js, "n_nu", -1) != nn_)
+    return fail(SGPU_ERR_ARG, "read_checkpoint: zone or frequency count differs");
+  const int64_t n = (int64_t)meta_number(js, "n_packets", -1);
+  if (n < 0 || n > pk_.capacity) return fail(SGPU_ERR_SIZE, "read_checkpoint: packet count exceeds capacity");
+  std::vector<size_t> shape;
+  std::vector<char> raw;
+  auto load = [&](const std::string& path, const char* descr, size_t want, void* d, size_t item) {
+    if (read_npy(path, descr, &shape, &raw) || raw.size() != want * item) return -1;
+    return want ? sgpu_memcpy_h2d(ctx_, d, raw.data(), raw.size()) : 0;
+  };
+  const std::string pd = dir + "/particles/", sd = dir + "/state/";
+  int bad = 0;
+  if (n > 0) {
+    double* dst[2][3] = {{pk_.x, pk_.y, pk_.z}, {pk_.Dx, pk_.Dy, pk_.Dz}};
+    const char* nm[2] = {"x.npy", "D.npy"};
+    for (int v = 0; v < 2; v++) {
+      if (read_npy(pd + nm[v], "<f8", &shape, &raw) || raw.size() != (size_t)n * 24) { bad = 1; break; }
+      const double* a = (const double*)raw.data();
+      std::vector<double> col(n);
+      for (int k = 0; k < 3; k++) {
+        for (int64_t i = 0; i < n; i++) col[i] = a[3 * i + k];
+        bad |= sgpu_memcpy_h2d(ctx_, dst[v][k], col.data(), 8 * n) != 0;
+      }
+    }
+    bad |= load(pd + "t.npy", "<f8", n, pk_.t, 8) != 0;
+    bad |= load(pd + "e.npy", "<f8", n, pk_.e, 8) != 0;
+    bad |= load(pd + "nu.npy", "<f8", n, pk_.nu, 8) != 0;
+    bad |= load(pd + "type.npy", "<i4", n, pk_.type, 4) != 0;
+    bad |= load(pd + "ind.npy", "<i4", n, pk_.ind, 4) != 0;
+    bad |= load(pd + "fate.npy", "<i4", n, pk_.fate, 4) != 0;
+    bad |= load(pd + "draw.npy", "<u4", n, pk_.draw, 4) != 0;
+    bad |= load(pd + "id.npy", "<u8", n, pk_.id, 8) != 0;
+  }
+  const size_t nz = nz_, nzn = (size_t)nz_ * nn_;
+  const struct { const char* name; double* d; size_t len; } st[] = {
+      {"rho", d_rho_, nz}, {"T", (double*)zs_.T_gas, nz}, {"vol", d_vol_, nz}, {"r_out", d_r_out_, nz},
+      {"e_abs", ta_.e_abs, nz}, {"J_nu", ta_.J_nu, nzn}, {"fx_rad", ta_.fx_rad, nz}, {"fy_rad", ta_.fy_rad, nz},
+      {"fz_rad", ta_.fz_rad, nz}, {"fr_rad", ta_.fr_rad, nz}, {"L_radio_dep", ta_.L_radio_dep, nz},
+      {"L_radio_emit", ta_.L_radio_emit, nz}, {"e_rad", ta_.e_rad, nz}};
+  for (const auto& f : st) bad |= load(sd + f.name + ".npy", "<f8", f.len, f.d, 8) != 0;
+  if (bad) return fail(SGPU_ERR_ARG, "read_checkpoint: missing or malformed dataset");
+  n_particles_ = n;
+  t_now_ = meta_number(js, "t_now", t_now_);
+  grid_.t_now = meta_number(js, "grid_t_now", grid_.t_now);
+  grid_.r_out.min = meta_number(js, "grid_min", grid_.r_out.min);
+  first_step_ = (int)meta_number(js, "first_step", first_step_);
+  step_index_ = (uint64_t)meta_number(js, "step_index", (double)step_index_);
   return SGPU_OK;
 }
 

@@ -76,6 +76,13 @@ int write_plotfile_1d(const std::string& path, double t, double r_min, const std
                       const std::vector<double>& L_emit, const std::vector<double>* X,
                       const std::vector<int32_t>& elems_Z, const std::vector<int32_t>& elems_A);
 
+// NumPy .npy (format 1.0) datasets of a checkpoint; descr such as "<f8",
+// "<i4", "<u4", "<u8".  read_npy checks descr and C order and returns the
+// shape and the raw data.
+int write_npy(const std::string& path, const char* descr, const std::vector<size_t>& shape, const void* data,
+              size_t bytes);
+int read_npy(const std::string& path, const char* descr, std::vector<size_t>* shape, std::vector<char>* data);
+
 // 1D spherical homologous grid, host side (grid_1D_sphere's fields)
 struct Grid1D {
   std::vector<double> r_out;      // shell outer radii
@@ -128,6 +135,16 @@ class transport {
   int output_spectrum(int it);
   // grid_1D_sphere::write_plotfile's plt_<iw>.dat, under dir
   int write_plotfile(int iw, double tt, int write_mass_fracs = 0, const std::string& dir = ".");
+  // transport::writeCheckpointParticles / readCheckpointParticles
+  // (transport_output.cpp:300-365, 501-590) with .npy datasets under a
+  // directory in place of HDF5 groups: particles/{x,D (n x 3),type,ind,t,e,
+  // nu,fate,draw,id}.npy (draw and id are the packet's RNG stream state),
+  // state/{rho,T,vol,r_out and the radiation tallies}.npy and meta.json with
+  // the clocks -- the layout pubsed_amd.checkpoint reads and writes.  A
+  // transport initialised with the same parameters and grid and restored
+  // from a checkpoint continues the run packet for packet.
+  int write_checkpoint(const std::string& dir);
+  int read_checkpoint(const std::string& dir);
   int64_t n_particles() const { return n_particles_; }
   int64_t n_escaped_last() const { return n_escaped_; }
   const std::string& error() const { return err_; }

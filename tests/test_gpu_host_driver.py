@@ -75,3 +75,35 @@ def test_host_driver_writes_reference_text_files(gpu, tmp_path):
     assert plt.shape == (100, 8 + 4)
     assert np.allclose(plt[:, 3], P.get("T"), rtol=1e-8)
     assert np.allclose(plt[:, 1], P.get("rho"), rtol=1e-8)
+
+
+@pytest.mark.parametrize("precision", [abi.PREC_FP64, abi.PREC_MIXED])
+def test_host_checkpoint_restart(gpu, tmp_path, precision):
+    """transport::write_checkpoint / read_checkpoint: three time-dependent
+    steps straight through and 1 + checkpoint + 2 in a fresh transport end
+    with the same census, escapes, clock and tallies (up to the order of the
+    atomic additions); the checkpoint the C++ class wrote loads into the
+    Python problem of the same grid through pubsed_amd.checkpoint."""
+    from pubsed_amd import checkpoint
+    n = 20000
+    chk = str(tmp_path / "chk")
+    print("case host checkpoint", precision, flush=True)
+    r = subprocess.run([DRIVER, str(n), str(precision), "", chk], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = json.loads(r.stdout.strip().splitlines()[-1])
+    a, c = got["straight"], got["restarted"]
+    print(a, c)
+    assert a["alive"] == c["alive"] > 0 and a["escaped"] == c["escaped"] and a["t_now"] == c["t_now"]
+    for k in ("L_dep", "E_abs_rate", "J_sum"):
+        assert abs(c[k] / a[k] - 1) < 1e-10, k
+    P = runtime.Problem(runtime.Space(None), "c1_grey")
+    P.alloc_packets(3 * n)
+    meta = checkpoint.read_checkpoint(P, chk)
+    assert P.n_packets == meta["n_packets"] > 0
+    pk = P.packets_numpy(P.n_packets)
+    assert np.unique(pk["id"]).size == P.n_packets
+    # census packets stop at the step's end (the mixed form's FP32 flight
+    # distance puts them there to FP32 accuracy)
+    tol = 1e-12 if precision == abi.PREC_FP64 else 1e-6
+    assert np.all(pk["fate"] == abi.STOPPED) and np.allclose(pk["t"], meta["t_now"], rtol=tol, atol=0)
+    assert np.all((pk["ind"] >= 0) & (pk["ind"] < P.nz))
