@@ -191,6 +191,9 @@ typedef struct {
   int32_t use_bound_free;          /* opacity_bound_free (GasState::bound_free_opacity,
                                       hydrogenic cross-sections)                  */
   int32_t no_ground_recomb;        /* opacity_no_ground_recomb                     */
+  int32_t use_bound_bound;         /* opacity_bound_bound: Voigt-profile line opacity
+                                      (GasState_opacities.cpp:83-91,379-408,
+                                      AtomicSpecies_opacities.cpp:224-282)      */
 } sgpu_opacity_params;
 
 /* Opacity-build outputs.  Tables [nz][n_nu]; scalars [nz]; optional

@@ -477,6 +477,50 @@ static int opacity_build(const sgpu_locate* nu, const sgpu_zone_state* zs, const
           emis[j] += aopac[j] * n_elec;
         }
       }
+      /* GasState::bound_bound_opacity (GasState_opacities.cpp:83-91,379-408) over
+       * AtomicSpecies::bound_bound_opacity (AtomicSpecies_opacities.cpp:224-282):
+       * each level-resolved line of the atom in stored order, Doppler width
+       * from line_beta_dop (GasState.cpp:218-220), Voigt profile over the bins
+       * [locate(nu0 - 5 dnu), locate(nu0 + 5 dnu)) at their centres */
+      if (p->use_bound_bound && at->n_blines > 0) {
+        for (int j = 0; j < nn; j++) { opac[j] = 0; aopac[j] = 0; }
+        for (int a = 0; a < at->n_atoms; a++) {
+          double n_dens = dens * mf[a] / (zs->elem_A[a] * OR_M_P);
+          double vd = sqrt(2 * OR_K * temp / OR_M_P / zs->elem_A[a]);
+          if (p->line_velocity_width > 0) vd = p->line_velocity_width;
+          double beta_dop = vd / OR_C;
+          for (int j = 0; j < nn; j++) { atom_opac[j] = 0; emis2[j] = 0; }
+          const int64_t* off = at->bline_bin_off + (size_t)a * (nn + 1);
+          for (int64_t l = off[0]; l < off[nn]; l++) {
+            int ll = at->bline_l[l], lu = at->bline_u[l];
+            double nl = ln[ll], nup = ln[lu];
+            double gl = 1.0 * at->lev_g[ll], gu = 1.0 * at->lev_g[lu];
+            double nu_0 = at->bline_nu[l];
+            double dnu = beta_dop * nu_0;
+            double A_ul = at->bline_A[l];
+            double a_voigt = A_ul / 4 / OR_PI / dnu;
+            if (nl == 0) continue;
+            double alpha_0 = nl * n_dens * gu / gl * A_ul / (8 * OR_PI) * OR_C * OR_C;
+            alpha_0 = alpha_0 * (1 - nup * gl / (nl * gu));
+            if (alpha_0 <= 0) continue;
+            int inu1 = or_locate_within_bounds(nu, nu_0 - dnu * 5);
+            int inu2 = or_locate_within_bounds(nu, nu_0 + dnu * 5);
+            double line_j = A_ul * nup * n_dens * OR_H / (4.0 * OR_PI);
+            for (int j = inu1; j < inu2; ++j) {
+              double nuc = or_center(nu, j);
+              double x = (nu_0 - nuc) / dnu;
+              double phi = or_voigt_profile(x, a_voigt) / dnu;
+              atom_opac[j] += alpha_0 / nuc / nuc * phi;
+              emis2[j] += line_j * nuc * phi;
+            }
+          }
+          for (int j = 0; j < nn; j++) { opac[j] += atom_opac[j]; aopac[j] += emis2[j]; }
+        }
+        for (int j = 0; j < nn; j++) {
+          abs[j] += opac[j];
+          emis[j] += aopac[j];
+        }
+      }
       /* GasState::line_expansion_opacity (GasState_opacities.cpp:424-454) with
        * AtomicSpecies::line_expansion_opacity (AtomicSpecies_opacities.cpp:329-375) */
       if (p->use_line_expansion && at->n_blines > 0) {

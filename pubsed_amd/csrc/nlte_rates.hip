@@ -230,24 +230,6 @@ __global__ void __launch_bounds__(RATE_THREADS) populations_kernel(PopArgs A)
 
 __global__ void zero_status_kernel(int32_t* st) { st[0] = 0; }
 
-// VoigtProfile::getProfile (src/transport/VoigtProfile.cpp:37-55), with its
-// own constants
-__device__ inline double voigt_profile(double x, double a)
-{
-  const double sqrt_pi = 1.77245385091;
-  const double pi = 3.14159265359;
-  const double xsq = x * x;
-  const double c = (xsq - 0.855) / (xsq + 3.42);
-  double q;
-  if (c < 0) q = 0;
-  else {
-    const double pic = 5.674 * c * c * c * c - 9.207 * c * c * c + 4.421 * c * c + 0.1117 * c;
-    q = (1 + 21 / xsq) * a / pi / (xsq + 1) * pic;
-  }
-  const double H = q * sqrt_pi + exp(-xsq);
-  return H / sqrt_pi;
-}
-
 struct RadArgs {
   sgpu_atoms at;
   sgpu_locate nu;

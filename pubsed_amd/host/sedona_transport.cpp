@@ -193,6 +193,7 @@ int transport::set_opacity(double dt)
   op_.use_free_free = p_.opacity_free_free;
   op_.use_bound_free = p_.opacity_bound_free;
   op_.no_ground_recomb = p_.opacity_no_ground_recomb;
+  op_.use_bound_bound = p_.opacity_bound_bound;
   op_.radiative_eq = p_.transport_radiative_equilibrium;
   op_.first_step = first_step_;
   op_.line_velocity_width = p_.line_velocity_width;

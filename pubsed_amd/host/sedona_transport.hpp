@@ -36,6 +36,7 @@ struct Params {
   int opacity_free_free = 0;
   int opacity_bound_free = 0;
   int opacity_no_ground_recomb = 0;
+  int opacity_bound_bound = 0;                // Voigt-profile line opacity (level-resolved lines)
   int opacity_use_nlte = 0;                   // NLTE gas state from the last step's J_nu
   std::vector<int32_t> opacity_atoms_in_nlte; // per-atom flags (element order)
   int opacity_use_collisions_nlte = 1;

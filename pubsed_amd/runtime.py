@@ -183,7 +183,9 @@ class Problem:
         self.op_params = abi.OpacityParams(
             self.t_now, 1.0 if cfg["steady"] else cfg.get("dt", 0.1 * W.DAY), cfg["epsilon"],
             1000.0, 1e8, 1e40, 0.0, 0.0, cfg["escat"], cfg["fuzz"], 0, 0, cfg["radiative_eq"], 0,
-            cfg.get("lexp", 0), cfg.get("ff", 0), cfg.get("bf", 0), cfg.get("no_ground_recomb", 0))
+            cfg.get("lexp", 0), cfg.get("ff", 0), cfg.get("bf", 0), cfg.get("no_ground_recomb", 0),
+            cfg.get("bb", 0))
+        self.op_params.line_velocity_width = cfg.get("line_velocity_width", 0.0)
         for k in ("abs", "scat", "emis_cdf"):
             a[k] = S.zeros(nz * nn, "f64")
         for k in ("L_thermal", "planck_mean", "rosseland_mean", "n_elec", "compton_opac",

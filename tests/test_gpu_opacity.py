@@ -105,3 +105,22 @@ def test_free_free_and_bound_free_match_oracle(gpu, ff, bf, lexp, ngr):
     P0 = runtime.Problem(runtime.Space(None), "c2_line", nz=32, lexp=lexp)
     OracleBackend().opacity_build(P0)
     assert not np.allclose(P0.get("abs"), Ph.get("abs"), rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("width,extra", [(3e7, {}), (3e8, dict(ff=1, bf=1, lexp=1))])
+def test_bound_bound_matches_oracle(gpu, width, extra):
+    """GasState::bound_bound_opacity (Voigt-profile level-resolved lines over
+    +-5 Doppler widths, one thread per (zone, line), per-bin FP64 atomics) in
+    the reference's order after the continua, alone and beside them and the
+    line expansion, 32 zones of the line ejecta with line_velocity_width set
+    so that lines span several of the 4096 bins (at 3e7 cm/s a line's +-5
+    widths cover about one bin: many lines add nothing)."""
+    over = dict(nz=32, bb=1, line_velocity_width=width, **extra)
+    Pg, Ph = run_both("c2_line", gpu, **over)
+    for k in KEYS_TABLE + KEYS_ZONE:
+        e = relerr(Pg.get(k), Ph.get(k), floor=1e-300)
+        print(k, e)
+        assert e <= RTOL, (k, e)
+    P0 = runtime.Problem(runtime.Space(None), "c2_line", **dict(over, bb=0))
+    OracleBackend().opacity_build(P0)
+    assert not np.allclose(P0.get("abs"), Ph.get("abs"), rtol=1e-6, atol=0)
