@@ -162,3 +162,33 @@ def test_time_step_nlte_from_tallied_field(gpu):
     d = np.abs(Pg.get("emis_cdf") - Ph.get("emis_cdf")).max() / np.abs(Ph.get("emis_cdf")).max()
     print("emis_cdf", d)
     assert d <= 0.1
+
+
+def test_nlte_build_with_bound_bound_lines(gpu):
+    """The Voigt-profile bound-bound opacity inside the NLTE-coupled build
+    (lines 3e8 cm/s wide, atom 0 in NLTE, W = 0.1): n_e and populations
+    within the single-NLTE-atom bars, the tables within the bar of the
+    cases whose least determined populations reach them (median 1e-5), and
+    with no atom flagged the build equals the LTE build bit for bit."""
+    over = dict(OVER, bb=1, line_velocity_width=3e8)
+    Pg = runtime.Problem(runtime.Space("cuda:0"), "c2_line", **over)
+    Ph = runtime.Problem(runtime.Space(None), "c2_line", **over)
+    J = planck_field(Ph, 0.1)
+    print("case nlte build with bound-bound lines", flush=True)
+    st, ev = gpu.opacity_build_nlte(Pg, torch.from_numpy(np.ascontiguousarray(J)).cuda(), (1, 0))
+    gpu.sync()
+    assert st == 0 == OracleBackend().opacity_build_nlte(Ph, J, np.array((1, 0), np.int32))
+    e = relerr(Pg.get("n_elec"), Ph.get("n_elec"), 1e-300)
+    nl = Ph.atoms.n_levels
+    lg, lh = Pg.get("lev_n").reshape(NZ, nl), Ph.get("lev_n").reshape(NZ, nl)
+    p = max(np.abs(lg[z] - lh[z]).sum() / lh[z].sum() for z in range(NZ))
+    ra = np.abs(Pg.get("abs") / np.maximum(Ph.get("abs"), 1e-300) - 1)
+    print("n_elec", e, "lev_n L1", p, "abs median", np.median(ra))
+    assert e <= 1e-7 and p <= 1e-3 and np.median(ra) <= 1e-5
+    P1 = runtime.Problem(runtime.Space("cuda:0"), "c2_line", **over)
+    P2 = runtime.Problem(runtime.Space("cuda:0"), "c2_line", **over)
+    gpu.opacity_build(P1)
+    gpu.opacity_build_nlte(P2, torch.from_numpy(np.ascontiguousarray(J)).cuda(), (0, 0))
+    gpu.sync()
+    for k in ("abs", "emis_cdf", "n_elec"):
+        assert np.array_equal(P1.get(k), P2.get(k)), k
