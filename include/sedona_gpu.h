@@ -194,6 +194,10 @@ typedef struct {
   int32_t use_bound_bound;         /* opacity_bound_bound: Voigt-profile line opacity
                                       (GasState_opacities.cpp:83-91,379-408,
                                       AtomicSpecies_opacities.cpp:224-282)      */
+  double minimum_extinction;       /* opacity_minimum_extinction: bound-bound lines
+                                      with alpha_0 / (nu_0^2 dnu) below it are
+                                      skipped (AtomicSpecies_opacities.cpp:259,
+                                      set at transport_init.cpp:225-226)        */
 } sgpu_opacity_params;
 
 /* Opacity-build outputs.  Tables [nz][n_nu]; scalars [nz]; optional

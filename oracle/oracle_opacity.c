@@ -503,6 +503,8 @@ static int opacity_build(const sgpu_locate* nu, const sgpu_zone_state* zs, const
             double alpha_0 = nl * n_dens * gu / gl * A_ul / (8 * OR_PI) * OR_C * OR_C;
             alpha_0 = alpha_0 * (1 - nup * gl / (nl * gu));
             if (alpha_0 <= 0) continue;
+            /* AtomicSpecies_opacities.cpp:259 */
+            if (alpha_0 / (nu_0 * nu_0 * dnu) < p->minimum_extinction) continue;
             int inu1 = or_locate_within_bounds(nu, nu_0 - dnu * 5);
             int inu2 = or_locate_within_bounds(nu, nu_0 + dnu * 5);
             double line_j = A_ul * nup * n_dens * OR_H / (4.0 * OR_PI);

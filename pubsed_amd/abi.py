@@ -58,7 +58,8 @@ class OpacityParams(C.Structure):
                 ("use_fuzz_expansion", i32), ("omit_scattering", i32),
                 ("omit_composition_decay", i32), ("radiative_eq", i32), ("first_step", i32),
                 ("use_line_expansion", i32), ("use_free_free", i32), ("use_bound_free", i32),
-                ("no_ground_recomb", i32), ("use_bound_bound", i32)]
+                ("no_ground_recomb", i32), ("use_bound_bound", i32),
+                ("minimum_extinction", f64)]
 
 
 class OpacityOut(C.Structure):

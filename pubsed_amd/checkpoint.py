@@ -16,7 +16,9 @@ the spectrum has counted them), so there is no ``particles_escaped`` group.
 Besides the particles: the zone state that the next step reads (density,
 temperature, grid edges and volumes, which the homologous step has moved),
 the radiation tallies (J_nu feeds the next step's NLTE gas state), the
-clocks and the grid's inner edges, in ``state/*.npy`` and ``meta.json``.
+optical and gamma escape spectra (flux and click counts, which a
+time-dependent run keeps adding to), the clocks and the grid's inner edges,
+in ``state/*.npy`` and ``meta.json``.
 A run restored into a Problem of the same configuration continues as the
 uninterrupted run does: the same packets take the same paths, and the
 tallies differ only by the order of their floating-point atomic additions
@@ -29,7 +31,11 @@ import os
 import numpy as np
 
 PARTICLE_SCALARS = ("type", "ind", "t", "e", "nu", "fate", "draw", "id")
-TALLIES = ("e_abs", "J_nu", "fx_rad", "fy_rad", "fz_rad", "fr_rad", "L_radio_dep", "L_radio_emit", "e_rad")
+TALLIES = ("e_abs", "J_nu", "fx_rad", "fy_rad", "fz_rad", "fr_rad", "L_radio_dep", "L_radio_emit", "e_rad",
+           # the escape spectra, which accumulate over the steps of a
+           # time-dependent run (transport::writeCheckpointSpectra,
+           # transport_output.cpp:492, read back at transport_init.cpp:164)
+           "opt_flux", "opt_click", "gam_flux", "gam_click")
 FORMAT = "sedona-mi355x checkpoint 1"
 
 

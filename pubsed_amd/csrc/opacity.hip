@@ -306,70 +306,72 @@ struct BinArgs {
   int n_lev;
   const double* fffac;   // [nz] (free-free only)
   const double* phifac;  // [nz][n_lev] (bound-free only)
-  const double* bbo;     // [nz][n_nu] bound-bound opacity and emissivity (bound-bound only)
-  const double* bbe;
 };
 
-// AtomicSpecies::bound_bound_opacity (AtomicSpecies_opacities.cpp:224-282):
-// one thread per (zone, level-resolved line).  Doppler width
-// line_beta_dop nu_0 with line_beta_dop = v_D / c, v_D = sqrt(2 k T / (m_p A))
-// or line_velocity_width when set (GasState.cpp:218-220); Voigt damping A_ul
-// / (4 pi dnu); extinction nl n g_u/g_l A_ul c^2 / (8 pi) corrected for
-// stimulated emission; the profile added over the bins [locate(nu_0 - 5 dnu),
-// locate(nu_0 + 5 dnu)) at their centres, so a line narrower than its bin
-// adds nothing, as in the reference.  The per-bin sums are FP64 atomics.
-struct BbArgs {
-  sgpu_locate nu;
-  sgpu_zone_state zs;
-  sgpu_atoms at;
-  double line_velocity_width;
-  const double* temp_used;
-  const double* levn;
-  const double* ndens;
-  int n_lev;
-  double* bbo;
-  double* bbe;
-};
-
-__global__ void __launch_bounds__(256) bb_kernel(BbArgs A)
+// GasState::bound_bound_opacity (GasState_opacities.cpp:379-392) over
+// AtomicSpecies::bound_bound_opacity (AtomicSpecies_opacities.cpp:224-282),
+// gathered for one (zone, bin): the reference adds each line's Voigt profile
+// into the bins [locate(nu_0 - 5 dnu), locate(nu_0 + 5 dnu)) at their
+// centres, atom by atom; here the bin collects, atom by atom in element
+// order, the lines of that atom whose window holds it, in the atom's line
+// order.  Doppler width line_beta_dop nu_0 with line_beta_dop = v_D / c,
+// v_D = sqrt(2 k T / (m_p A)) or line_velocity_width when set
+// (GasState.cpp:218-220); Voigt damping A_ul / (4 pi dnu); extinction
+// nl n g_u/g_l A_ul c^2 / (8 pi) corrected for stimulated emission; lines
+// with alpha_0 / (nu_0^2 dnu) below opacity_minimum_extinction skipped.  A
+// line can cover bin j only if nu_0 (1 + 5 beta) >= x[j-1] and
+// nu_0 (1 - 5 beta) < x[j+1] (locate is upper_bound up to the log formula's
+// unverified first bin), so the candidates are the lines of the atom's CSR
+// bins holding nu_0 in [x[j-1] / (1 + 5 beta), x[j+1] / (1 - 5 beta)],
+// widened by 1e-9 relative; each candidate is then tested with the same
+// locate calls as the reference.  Every sum runs in a fixed order, so two
+// builds are bit-identical.
+__device__ inline void bound_bound_bin(const BinArgs& A, int z, int j, double T, double& opac, double& emis)
 {
-  const int z = blockIdx.y;
-  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= A.at.n_blines) return;
-  if (A.zs.grey_opacity && A.zs.grey_opacity[z] != 0) return;
-  const int ll = A.at.bline_l[l], lu = A.at.bline_u[l];
-  int a = 0;
-  while (a + 1 < A.at.n_atoms && ll >= A.at.lev_off[a + 1]) ++a;
-  const double* ln = A.levn + (size_t)z * A.n_lev;
-  const double nl = ln[ll], nup = ln[lu];
-  if (nl == 0) return;
-  const double n_dens = A.ndens[(size_t)z * A.at.n_atoms + a];
-  const double gl = 1.0 * A.at.lev_g[ll], gu = 1.0 * A.at.lev_g[lu];
-  const double nu_0 = A.at.bline_nu[l];
-  double vd = sqrt(2 * K_BOLTZ * A.temp_used[z] / M_PROTON / A.zs.elem_A[a]);
-  if (A.line_velocity_width > 0) vd = A.line_velocity_width;
-  const double dnu = vd / C_LIGHT * nu_0;
-  const double A_ul = A.at.bline_A[l];
-  const double a_voigt = A_ul / 4 / PI / dnu;
-  double alpha_0 = nl * n_dens * gu / gl * A_ul / (8 * PI) * C_LIGHT * C_LIGHT;
-  alpha_0 = alpha_0 * (1 - nup * gl / (nl * gu));
-  if (alpha_0 <= 0) return;
-  const int inu1 = loc_within(A.nu, nu_0 - dnu * 5);
-  const int inu2 = loc_within(A.nu, nu_0 + dnu * 5);
-  const double line_j = A_ul * nup * n_dens * H_PLANCK / (4.0 * PI);
   const int nn = A.nu.n;
-  for (int j = inu1; j < inu2; ++j) {
-    const double nu = loc_center(A.nu, j);
-    const double x = (nu_0 - nu) / dnu;
-    const double phi = voigt_profile(x, a_voigt) / dnu;
-    atomic_add_f64(&A.bbo[(size_t)z * nn + j], alpha_0 / nu / nu * phi);
-    atomic_add_f64(&A.bbe[(size_t)z * nn + j], line_j * nu * phi);
+  const double nuc = loc_center(A.nu, j);
+  const double* ln = A.levn + (size_t)z * A.n_lev;
+  opac = 0;
+  emis = 0;
+  for (int a = 0; a < A.at.n_atoms; ++a) {
+    const double n_dens = A.ndens[(size_t)z * A.at.n_atoms + a];
+    double vd = sqrt(2 * K_BOLTZ * T / M_PROTON / A.zs.elem_A[a]);
+    if (A.p.line_velocity_width > 0) vd = A.p.line_velocity_width;
+    const double beta = vd / C_LIGHT;
+    int b_lo = 0, b_hi = nn - 1;
+    if (5 * beta < 0.5) {
+      if (j >= 1) b_lo = upper_bound_dev(A.nu.x, nn, A.nu.x[j - 1] / (1 + 5 * beta) * (1 - 1e-9));
+      if (j + 1 <= nn - 1) b_hi = upper_bound_dev(A.nu.x, nn, A.nu.x[j + 1] / (1 - 5 * beta) * (1 + 1e-9));
+      b_lo = b_lo > 0 ? b_lo - 1 : 0;
+      b_hi = b_hi + 1 < nn ? b_hi + 1 : nn - 1;
+    }
+    const int64_t* off = A.at.bline_bin_off + (size_t)a * (nn + 1);
+    double atom_opac = 0, atom_emis = 0;
+    for (int64_t l = off[b_lo]; l < off[b_hi + 1]; ++l) {
+      const int ll = A.at.bline_l[l], lu = A.at.bline_u[l];
+      const double nl = ln[ll], nup = ln[lu];
+      if (nl == 0) continue;
+      const double gl = 1.0 * A.at.lev_g[ll], gu = 1.0 * A.at.lev_g[lu];
+      const double nu_0 = A.at.bline_nu[l];
+      const double dnu = beta * nu_0;
+      const double A_ul = A.at.bline_A[l];
+      double alpha_0 = nl * n_dens * gu / gl * A_ul / (8 * PI) * C_LIGHT * C_LIGHT;
+      alpha_0 = alpha_0 * (1 - nup * gl / (nl * gu));
+      if (alpha_0 <= 0) continue;
+      if (alpha_0 / (nu_0 * nu_0 * dnu) < A.p.minimum_extinction) continue;
+      const int inu1 = loc_within(A.nu, nu_0 - dnu * 5);
+      const int inu2 = loc_within(A.nu, nu_0 + dnu * 5);
+      if (j < inu1 || j >= inu2) continue;
+      const double a_voigt = A_ul / 4 / PI / dnu;
+      const double line_j = A_ul * nup * n_dens * H_PLANCK / (4.0 * PI);
+      const double x = (nu_0 - nuc) / dnu;
+      const double phi = voigt_profile(x, a_voigt) / dnu;
+      atom_opac += alpha_0 / nuc / nuc * phi;
+      atom_emis += line_j * nuc * phi;
+    }
+    opac += atom_opac;
+    emis += atom_emis;
   }
-}
-
-__global__ void fill_f64_kernel(double* p, size_t n, double v)
-{
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
 __global__ void __launch_bounds__(256) bins_kernel(BinArgs A)
@@ -444,11 +446,12 @@ __global__ void __launch_bounds__(256) bins_kernel(BinArgs A)
       ab += opac;
       em += emis * ne;
     }
-    // GasState::bound_bound_opacity (GasState_opacities.cpp:83-91), summed
-    // over the lines by bb_kernel
-    if (A.bbo) {
-      ab += A.bbo[(size_t)z * nn + j];
-      em += A.bbe[(size_t)z * nn + j];
+    // GasState::bound_bound_opacity (GasState_opacities.cpp:83-91)
+    if (A.p.use_bound_bound && A.at.n_atoms > 0 && A.at.n_blines > 0) {
+      double bo, be;
+      bound_bound_bin(A, z, j, T, bo, be);
+      ab += bo;
+      em += be;
     }
     // GasState::line_expansion_opacity (GasState_opacities.cpp:424-454),
     // AtomicSpecies::line_expansion_opacity (AtomicSpecies_opacities.cpp:329-375):
@@ -949,33 +952,6 @@ __global__ void zero_i32_kernel(int32_t* p) { p[0] = 0; }
 
 using namespace sgpu;
 
-namespace {
-// bound-bound opacity and emissivity of every gas zone into two [nz][n_nu]
-// scratch tables (zeroed here), from the zone's level populations and atom
-// densities; null tables when the option is off
-int bound_bound_tables(sgpu_ctx* ctx, const sgpu_locate* nu_grid, const sgpu_zone_state* zones,
-                       const sgpu_atoms& at, const sgpu_opacity_params* params, const double* temp_used,
-                       const double* levn, const double* ndens, int n_lev, double** bbo, double** bbe)
-{
-  *bbo = *bbe = nullptr;
-  if (!params->use_bound_bound || at.n_atoms == 0 || at.n_blines == 0) return SGPU_OK;
-  SGPU_REQUIRE(at.bline_l && at.bline_u && at.bline_nu && at.bline_A,
-               "bound-bound opacity needs the level-resolved line arrays");
-  const int nz = zones->n_zones;
-  const size_t n = (size_t)nz * nu_grid->n;
-  int st = 0;
-  double* t = (double*)scratch(ctx, "op_bb", sizeof(double) * 2 * n, &st);
-  if (st) return st;
-  hipLaunchKernelGGL(fill_f64_kernel, dim3(1024), dim3(256), 0, ctx->stream, t, 2 * n, 0.0);
-  BbArgs A{*nu_grid, *zones, at, params->line_velocity_width, temp_used, levn, ndens, n_lev, t, t + n};
-  hipLaunchKernelGGL(bb_kernel, dim3((unsigned)((at.n_blines + 255) / 256), nz), dim3(256), 0, ctx->stream, A);
-  SGPU_LAUNCH_CHECK();
-  *bbo = t;
-  *bbe = t + n;
-  return SGPU_OK;
-}
-}  // namespace
-
 extern "C" int sgpu_opacity_build(sgpu_ctx* ctx, const sgpu_locate* nu_grid,
                                   const sgpu_zone_state* zones, const sgpu_atoms* atoms,
                                   const sgpu_opacity_params* params, sgpu_opacity_out* out)
@@ -1028,6 +1004,9 @@ extern "C" int sgpu_opacity_build(sgpu_ctx* ctx, const sgpu_locate* nu_grid,
   if (lexp)
     SGPU_REQUIRE(at.bline_l && at.bline_u && at.bline_nu && at.bline_f && at.bline_bin && at.bline_bin_off,
                  "line expansion needs the level-resolved line arrays");
+  if (bb)
+    SGPU_REQUIRE(at.bline_l && at.bline_u && at.bline_nu && at.bline_A && at.bline_bin_off,
+                 "bound-bound opacity needs the level-resolved line arrays");
   if (lexp || bf || bb) {
     levn = (double*)scratch(ctx, "op_levn", sizeof(double) * (size_t)nz * n_lev, &st);
     if (st) return st;
@@ -1047,11 +1026,8 @@ extern "C" int sgpu_opacity_build(sgpu_ctx* ctx, const sgpu_locate* nu_grid,
   StateArgs sa{*zones, at, *params, n_lev, n_ion, temp_used, nion, upart, *out, levn, ndens, fffac, phifac};
   hipLaunchKernelGGL(state_kernel, dim3(nz), dim3(WAVE), 0, ctx->stream, sa);
   SGPU_LAUNCH_CHECK();
-  double *bbo, *bbe;
-  st = bound_bound_tables(ctx, nu_grid, zones, at, params, temp_used, levn, ndens, n_lev, &bbo, &bbe);
-  if (st) return st;
   BinArgs ba{*nu_grid, *zones, at, *params, n_ion, temp_used, nion, upart,
-             out->abs, out->scat, out->emis_cdf, out->n_elec, levn, ndens, n_lev, fffac, phifac, bbo, bbe};
+             out->abs, out->scat, out->emis_cdf, out->n_elec, levn, ndens, n_lev, fffac, phifac};
   hipLaunchKernelGGL(bins_kernel, dim3((nu_grid->n + 255) / 256, nz), dim3(256), 0, ctx->stream, ba);
   SGPU_LAUNCH_CHECK();
   FinArgs fa{*nu_grid, *zones, *params, temp_used, *out};
@@ -1109,6 +1085,9 @@ extern "C" int sgpu_opacity_build_nlte(sgpu_ctx* ctx, const sgpu_locate* nu_grid
   if (lexp)
     SGPU_REQUIRE(at.bline_l && at.bline_u && at.bline_nu && at.bline_f && at.bline_bin && at.bline_bin_off,
                  "line expansion needs the level-resolved line arrays");
+  if (params->use_bound_bound && at.n_blines > 0)
+    SGPU_REQUIRE(at.bline_l && at.bline_u && at.bline_nu && at.bline_A && at.bline_bin_off,
+                 "bound-bound opacity needs the level-resolved line arrays");
 
   // (zone, NLTE atom) systems, zone-major
   std::vector<int32_t> h_atom, h_zone, h_n, h_sys_of((size_t)nz * at.n_atoms, -1);
@@ -1207,11 +1186,8 @@ extern "C" int sgpu_opacity_build_nlte(sgpu_ctx* ctx, const sgpu_locate* nu_grid
   if (n_evals) *n_evals = evals;
   hipLaunchKernelGGL(nlte_finish_kernel, dim3(nz), dim3(WAVE), 0, ctx->stream, A);
   SGPU_LAUNCH_CHECK();
-  double *bbo, *bbe;
-  st = bound_bound_tables(ctx, nu_grid, zones, at, params, temp_used, levn, ndens, n_lev, &bbo, &bbe);
-  if (st) return st;
   BinArgs ba{*nu_grid, *zones, at, *params, n_ion, temp_used, nion, upart,
-             out->abs, out->scat, out->emis_cdf, out->n_elec, levn, ndens, n_lev, fffac, phifac, bbo, bbe};
+             out->abs, out->scat, out->emis_cdf, out->n_elec, levn, ndens, n_lev, fffac, phifac};
   hipLaunchKernelGGL(bins_kernel, dim3((nu_grid->n + 255) / 256, nz), dim3(256), 0, ctx->stream, ba);
   SGPU_LAUNCH_CHECK();
   FinArgs fa{*nu_grid, *zones, *params, temp_used, *out};

@@ -103,9 +103,9 @@ struct Pkt {
   typename PreOf<R>::type pre;
 };
 
-template <class R> __device__ __forceinline__ R rsqrt_(R v);
-template <> __device__ __forceinline__ double rsqrt_<double>(double v) { return sqrt(v); }
-template <> __device__ __forceinline__ float rsqrt_<float>(float v) { return __builtin_amdgcn_sqrtf(v); }   // v_sqrt_f32, 1 ulp
+template <class R> __device__ __forceinline__ R sqrt_(R v);
+template <> __device__ __forceinline__ double sqrt_<double>(double v) { return sqrt(v); }
+template <> __device__ __forceinline__ float sqrt_<float>(float v) { return __builtin_amdgcn_sqrtf(v); }   // v_sqrt_f32, 1 ulp
 template <class R> __device__ __forceinline__ R rlog(R v);
 template <> __device__ __forceinline__ double rlog<double>(double v) { return log(v); }
 template <> __device__ __forceinline__ float rlog<float>(float v) { return __logf(v); }
@@ -150,12 +150,12 @@ template <class R>
 __device__ __forceinline__ int get_zone(const GridView<R>& g, const R* x)
 {
   if (g.kind == SGPU_GRID_1D_SPHERE) {
-    R r = rsqrt_(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    R r = sqrt_(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
     if (r < g.r_out.min) return -1;
     if (r >= g.r_out.x[g.r_out.n - 1]) return -2;
     return ax_within(g.r_out, r);
   }
-  R p = rsqrt_(x[0] * x[0] + x[1] * x[1]);
+  R p = sqrt_(x[0] * x[0] + x[1] * x[1]);
   int nx = g.x_out.n, nz = g.z_out.n;
   if (p < g.x_out.min) return -2;
   if (p > g.x_out.x[nx - 1]) return -2;
@@ -173,7 +173,7 @@ __device__ __forceinline__ int next_zone_1d(const GridView<R>& g, const R* x, co
   R rsq = (x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
   R xdotD = (D[0] * x[0] + D[1] * x[1] + D[2] * x[2]);
   R r_o = g.r_out.x[i];
-  R l_out = -1 * xdotD + rsqrt_(xdotD * xdotD + r_o * r_o - rsq);
+  R l_out = -1 * xdotD + sqrt_(xdotD * xdotD + r_o * r_o - rsq);
   R r_in;
   int ind_in;
   if (i != 0) { r_in = g.r_out.x[i - 1]; ind_in = i - 1; }
@@ -184,7 +184,7 @@ __device__ __forceinline__ int next_zone_1d(const GridView<R>& g, const R* x, co
   else {
     R rad = xdotD * xdotD + r_in * r_in - rsq;
     if (rad < 0) l_in = -1;
-    else l_in = -1 * xdotD - rsqrt_(rad);
+    else l_in = -1 * xdotD - sqrt_(rad);
   }
   int ind;
   const R tiny_offset = (R)(1 + 1e-6);
@@ -205,7 +205,7 @@ template <class R>
 __device__ __forceinline__ int next_zone_2d(const GridView<R>& g, const R* x, const R* D, int i, R* l)
 {
   const int nz = g.z_out.n, nx = g.x_out.n;
-  R p = rsqrt_(x[0] * x[0] + x[1] * x[1]);
+  R p = sqrt_(x[0] * x[0] + x[1] * x[1]);
   R z = x[2];
   R Dz = D[2];
   int ix = i / nz, iz = i - ix * nz;
@@ -232,13 +232,13 @@ __device__ __forceinline__ int next_zone_2d(const GridView<R>& g, const R* x, co
     R det = b * b - 4 * a * c;
     R lp_out, lp_in;
     if (det < 0) lp_out = rinf<R>();
-    else lp_out = (-1 * b + rsqrt_(det)) / (2 * a);
+    else lp_out = (-1 * b + sqrt_(det)) / (2 * a);
     if (lp_out < 0) lp_out = rinf<R>();
     pt = ax_left(g.x_out, ix) - dx * tiny;
     c = p * p - pt * pt;
     det = b * b - 4 * a * c;
     if (det < 0) lp_in = rinf<R>();
-    else lp_in = (-1 * b - rsqrt_(det)) / (2 * a);
+    else lp_in = (-1 * b - sqrt_(det)) / (2 * a);
     if (lp_in < 0) lp_in = rinf<R>();
     if (ix == 0) lp_in = rinf<R>();
     if (lp_in < lp_out) { lp = lp_in; d_ip = -1; }
@@ -313,7 +313,7 @@ __device__ __forceinline__ void isotropic_dir(Pkt<R>& p, R* Dn)
 {
   R mu = 1 - 2 * p.rng.template u<R>();
   R phi = 2 * (R)PI * p.rng.template u<R>();
-  R smu = rsqrt_(1 - mu * mu);
+  R smu = sqrt_(1 - mu * mu);
   R s, c;
   rsincos(phi, &s, &c);
   Dn[0] = smu * c; Dn[1] = smu * s; Dn[2] = mu;

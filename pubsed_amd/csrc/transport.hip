@@ -347,7 +347,7 @@ __device__ __forceinline__ void isotropic_scatter(const PropArgs<R>& A, Pkt<R>& 
   for (int k = 0; k < 3; k++)
     p.D[k] = qdiv((R)1, vd_out) *
              (Dn[k] - qdiv(gamma * V[k], cl) * (1 - qdiv(qdiv(gamma * vdp, cl), gamma + 1)));
-  R norm = rsqrt_(p.D[0] * p.D[0] + p.D[1] * p.D[1] + p.D[2] * p.D[2]);
+  R norm = sqrt_(p.D[0] * p.D[0] + p.D[1] * p.D[1] + p.D[2] * p.D[2]);
   p.D[0] = qdiv(p.D[0], norm); p.D[1] = qdiv(p.D[1], norm); p.D[2] = qdiv(p.D[2], norm);
   p.e = p.e * (double)vd_out;
   p.nu = p.nu * vd_out;
@@ -701,7 +701,7 @@ __device__ __forceinline__ int mc_event(const PropArgs<R>& A, Pkt<R>& p, double 
   const double ds = (double)dshift, kap = (double)kappa;
   const bool photon = p.type == SGPU_PHOTON;
   if (photon) zs.jnu((int64_t)z * (A.p.store_Jnu ? nn : 1) + (A.p.store_Jnu ? i_nu : 0), this_E);
-  R rr = rsqrt_(p.x[0] * p.x[0] + p.x[1] * p.x[1] + p.x[2] * p.x[2]);
+  R rr = sqrt_(p.x[0] * p.x[0] + p.x[1] * p.x[1] + p.x[2] * p.x[2]);
   R xdotD = p.x[0] * p.D[0] + p.x[1] * p.D[1] + p.x[2] * p.D[2];
   zs.event(z, photon, photon ? this_E * ds * kap * (double)eps * ds : 0.0,
            this_E * ds * kap * (double)p.D[0] * ds, this_E * ds * kap * (double)p.D[1] * ds,

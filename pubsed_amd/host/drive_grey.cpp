@@ -8,7 +8,8 @@
 // With a checkpoint directory as the fourth argument it instead runs three
 // time-dependent steps of dt = 2 d twice -- straight through, and with a
 // checkpoint written after the first step and read into a fresh transport
-// that runs the other two -- and prints both runs' results.
+// that runs the other two -- and prints both runs' results, including the
+// escape spectrum (half-day time bins, 4 mu bins) accumulated over the steps.
 //
 // usage: drive_grey [n_packets] [precision 0|1] [output_dir] [checkpoint_dir]
 #include <cmath>
@@ -70,17 +71,27 @@ int main(int argc, char** argv)
   if (!chk_dir.empty()) {
     p.transport_steady_iterate = 0;
     p.particles_max_total = 3 * n_packets;
+    // the escape spectrum in half-day time bins and 4 mu bins, accumulating
+    // over the steps
+    p.spectrum_time_grid[0] = 18 * DAY;
+    p.spectrum_time_grid[1] = 28 * DAY;
+    p.spectrum_time_grid[2] = 0.5 * DAY;
+    p.spectrum_n_mu = 4;
     const double dt = 2 * DAY;
     auto summary = [&](transport& tr) {
-      double dep = 0, eabs = 0, J = 0;
+      double dep = 0, eabs = 0, J = 0, spec = 0, spec_w = 0, clicks = 0;
       auto L = tr.L_radio_dep();
       auto A = tr.e_abs();
       for (int i = 0; i < nz; ++i) { dep += L[i] * vol[i]; eabs += A[i] * vol[i]; }
       for (double j : tr.J_nu()) J += j;
-      char b[512];
+      auto S = tr.optical_spectrum();
+      auto C = tr.optical_clicks();
+      for (size_t i = 0; i < S.size(); ++i) { spec += S[i]; spec_w += S[i] * (double)(i + 1); clicks += C[i]; }
+      char b[768];
       snprintf(b, sizeof(b), "{\"alive\": %lld, \"escaped\": %lld, \"t_now\": %.17g, \"L_dep\": %.17g, "
-               "\"E_abs_rate\": %.17g, \"J_sum\": %.17g}", (long long)tr.n_particles(),
-               (long long)tr.n_escaped_last(), tr.t_now(), dep, eabs, J);
+               "\"E_abs_rate\": %.17g, \"J_sum\": %.17g, \"spec_flux\": %.17g, \"spec_flux_weighted\": %.17g, "
+               "\"spec_clicks\": %.17g, \"spec_bins\": %zu}", (long long)tr.n_particles(),
+               (long long)tr.n_escaped_last(), tr.t_now(), dep, eabs, J, spec, spec_w, clicks, S.size());
       return std::string(b);
     };
     transport a, b, c;

@@ -4,6 +4,7 @@
 
 #include <sys/stat.h>
 
+#include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -29,6 +30,26 @@ std::vector<double> log_edges(double lo, double hi, int n, double* del_out)
   }
   *del_out = 0;
   return {};
+}
+
+// locate_array::init(start, stop, double del) (locate_array.cpp:25-46) and
+// init(start, stop, int n) (:88-108); both give one catch-all bin (type none)
+// for an empty range
+sgpu_locate lin_grid(double start, double stop, double del, int n, HostGrid* h)
+{
+  const bool none = n < 0 ? start >= stop : (start == stop || n == 0);
+  if (none) {
+    h->min = -INFINITY;
+    h->x.assign(1, INFINITY);
+    return {1, SGPU_LOC_NONE, -INFINITY, 0.0, nullptr};
+  }
+  if (n < 0) n = std::max((int)std::ceil((stop - start) / del), 1);
+  else del = (stop - start) / (double)n;
+  h->min = start;
+  h->x.resize(n);
+  for (int i = 0; i < n - 1; i++) h->x[i] = start + (i + 1) * del;
+  h->x[n - 1] = stop;
+  return {n, SGPU_LOC_LIN, start, del, nullptr};
 }
 }  // namespace
 
@@ -145,20 +166,21 @@ int transport::init(const Params& p, const Grid1D& g, const AtomicData* atoms, i
   out_.zone_status = (int32_t*)dalloc(sizeof(int32_t) * nz_);
   tab_ = {out_.abs, out_.scat, out_.emis_cdf, out_.eps_imc, out_.compton_opac, out_.photoion_opac};
 
-  // spectrum grids: spectrum_time_grid {0,1,1}, the transport nu grid, one mu / phi bin
-  static const double one[1] = {1.0};
-  static const double two_pi[1] = {2 * PI_REF};
-  double* t_x = upload(std::vector<double>(one, one + 1));
-  double* mu_x = upload(std::vector<double>(one, one + 1));
-  double* ph_x = upload(std::vector<double>(two_pi, two_pi + 1));
-  sgpu_locate tg = {1, SGPU_LOC_LIN, 0.0, 1.0, t_x};
-  sgpu_locate mg = {1, SGPU_LOC_LIN, -1.0, 2.0, mu_x};
-  sgpu_locate pg = {1, SGPU_LOC_LIN, 0.0, 2 * PI_REF, ph_x};
+  // spectrum_array::init (spectrum_array.cpp:36-83) as transport_init.cpp:
+  // 166-173 calls it: spectrum_time_grid, the transport nu grid, n_mu bins on
+  // [-1, 1], n_phi on [0, 2 pi]; the gamma spectrum on the same grids
+  sgpu_locate tg = lin_grid(p.spectrum_time_grid[0], p.spectrum_time_grid[1], p.spectrum_time_grid[2], -1, &spec_t_);
+  sgpu_locate mg = lin_grid(-1.0, 1.0, 0.0, p.spectrum_n_mu, &spec_mu_);
+  sgpu_locate pg = lin_grid(0.0, 2 * PI_REF, 0.0, p.spectrum_n_phi, &spec_phi_);
+  tg.x = upload(spec_t_.x);
+  mg.x = upload(spec_mu_.x);
+  pg.x = upload(spec_phi_.x);
+  n_spec_ = (size_t)tg.n * nn_ * mg.n * pg.n;
   ta_ = {};
   ta_.e_abs = z(nz_); ta_.J_nu = z(T); ta_.fx_rad = z(nz_); ta_.fy_rad = z(nz_); ta_.fz_rad = z(nz_);
   ta_.fr_rad = z(nz_); ta_.L_radio_dep = z(nz_); ta_.L_radio_emit = z(nz_); ta_.e_rad = z(nz_);
-  ta_.optical = {tg, nu_, mg, pg, z(nn_), z(nn_)};
-  ta_.gamma = {tg, nu_, mg, pg, z(nn_), z(nn_)};
+  ta_.optical = {tg, nu_, mg, pg, z(n_spec_), z(n_spec_)};
+  ta_.gamma = {tg, nu_, mg, pg, z(n_spec_), z(n_spec_)};
 
   const int64_t cap = p.particles_max_total;
   for (sgpu_packets* q : {&pk_, &scr_}) {
@@ -197,6 +219,7 @@ int transport::set_opacity(double dt)
   op_.radiative_eq = p_.transport_radiative_equilibrium;
   op_.first_step = first_step_;
   op_.line_velocity_width = p_.line_velocity_width;
+  op_.minimum_extinction = p_.opacity_minimum_extinction;
   int rc;
   // transport_opacity.cpp:30-36, 133-136: the NLTE gas state from the J_nu of
   // the previous step (LTE on the first step, inside the call)
@@ -292,26 +315,25 @@ std::vector<double> transport::J_nu() const { return download(ta_.J_nu, (size_t)
 std::vector<double> transport::e_rad() const { return download(ta_.e_rad, nz_); }
 std::vector<double> transport::e_abs() const { return download(ta_.e_abs, nz_); }
 std::vector<double> transport::L_radio_dep() const { return download(ta_.L_radio_dep, nz_); }
-std::vector<double> transport::optical_spectrum() const { return download(ta_.optical.flux, nn_); }
+std::vector<double> transport::optical_spectrum() const { return download(ta_.optical.flux, n_spec_); }
+std::vector<double> transport::optical_clicks() const { return download(ta_.optical.click, n_spec_); }
 std::vector<double> transport::abs_opacity() const { return download(out_.abs, (size_t)nz_ * nn_); }
 std::vector<double> transport::T_gas() const { return download(zs_.T_gas, nz_); }
 std::vector<double> transport::n_elec() const { return download(out_.n_elec, nz_); }
 
 int transport::output_spectrum(int it)
 {
-  // the spectrum grids set up in init: one time bin {0, 1}, the transport nu
-  // grid, one mu bin on [-1, 1], one phi bin on [0, 2 pi]
-  HostGrid tg{0.0, {1.0}}, mg{-1.0, {1.0}}, pg{0.0, {2 * PI_REF}};
+  // the spectrum grids set up in init
   HostGrid ng{p_.transport_nu_grid[0], nu_edges_};
   const std::string base = it < 0 ? std::string("_final") : "_" + std::to_string(it);
   if (p_.spectrum_suppress_txt) return SGPU_OK;
   if (!p_.spectrum_name.empty() &&
-      write_spectrum_dat(p_.spectrum_name + base, tg, ng, mg, pg, download(ta_.optical.flux, nn_),
-                         download(ta_.optical.click, nn_)))
+      write_spectrum_dat(p_.spectrum_name + base, spec_t_, ng, spec_mu_, spec_phi_,
+                         download(ta_.optical.flux, n_spec_), download(ta_.optical.click, n_spec_)))
     return fail(SGPU_ERR_ARG, "output_spectrum: cannot write the optical spectrum");
   if (!p_.gamma_name.empty() &&
-      write_spectrum_dat(p_.gamma_name + base, tg, ng, mg, pg, download(ta_.gamma.flux, nn_),
-                         download(ta_.gamma.click, nn_)))
+      write_spectrum_dat(p_.gamma_name + base, spec_t_, ng, spec_mu_, spec_phi_,
+                         download(ta_.gamma.flux, n_spec_), download(ta_.gamma.click, n_spec_)))
     return fail(SGPU_ERR_ARG, "output_spectrum: cannot write the gamma spectrum");
   return SGPU_OK;
 }
@@ -379,6 +401,16 @@ int transport::write_checkpoint(const std::string& dir)
     std::vector<double> h = download(f.d, f.len);
     bad |= write_npy(sd + f.name + ".npy", "<f8", {f.len}, h.data(), 8 * f.len);
   }
+  // the escape spectra accumulate over the steps of a time-dependent run
+  // (transport::writeCheckpointSpectra, transport_output.cpp:492; read back at
+  // transport_init.cpp:164)
+  const struct { const char* name; const double* d; } sp[] = {
+      {"opt_flux", ta_.optical.flux}, {"opt_click", ta_.optical.click},
+      {"gam_flux", ta_.gamma.flux}, {"gam_click", ta_.gamma.click}};
+  for (const auto& f : sp) {
+    std::vector<double> h = download(f.d, n_spec_);
+    bad |= write_npy(sd + f.name + ".npy", "<f8", {n_spec_}, h.data(), 8 * n_spec_);
+  }
   FILE* m = fopen((dir + "/meta.json").c_str(), "w");
   if (!m) bad = 1;
   else {
@@ -445,6 +477,10 @@ int transport::read_checkpoint(const std::string& dir)
       {"fz_rad", ta_.fz_rad, nz}, {"fr_rad", ta_.fr_rad, nz}, {"L_radio_dep", ta_.L_radio_dep, nz},
       {"L_radio_emit", ta_.L_radio_emit, nz}, {"e_rad", ta_.e_rad, nz}};
   for (const auto& f : st) bad |= load(sd + f.name + ".npy", "<f8", f.len, f.d, 8) != 0;
+  const struct { const char* name; double* d; } sp[] = {
+      {"opt_flux", ta_.optical.flux}, {"opt_click", ta_.optical.click},
+      {"gam_flux", ta_.gamma.flux}, {"gam_click", ta_.gamma.click}};
+  for (const auto& f : sp) bad |= load(sd + f.name + ".npy", "<f8", n_spec_, f.d, 8) != 0;
   if (bad) return fail(SGPU_ERR_ARG, "read_checkpoint: missing or malformed dataset");
   n_particles_ = n;
   t_now_ = meta_number(js, "t_now", t_now_);

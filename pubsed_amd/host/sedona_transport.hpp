@@ -41,6 +41,7 @@ struct Params {
   std::vector<int32_t> opacity_atoms_in_nlte; // per-atom flags (element order)
   int opacity_use_collisions_nlte = 1;
   double line_velocity_width = 0;
+  double opacity_minimum_extinction = 0;     // bound-bound lines below this alpha_0/(nu_0^2 dnu) are skipped
   double opacity_maximum_opacity = 1e40;
   double limits_temp_min = 1000;
   double limits_temp_max = 1e8;
@@ -48,6 +49,9 @@ struct Params {
   int64_t particles_max_total = 10000000;
   uint64_t transport_rng_seed = 20261203;
   int precision = SGPU_PREC_MIXED;
+  double spectrum_time_grid[3] = {0, 1, 1};  // {start, stop, del}; start >= stop: one catch-all bin
+  int spectrum_n_mu = 1;                      // mu = D_z bins on [-1, 1]
+  int spectrum_n_phi = 1;                     // phi bins on [0, 2 pi]
   std::string spectrum_name = "spectrum";   // output file stem; "" writes nothing
   std::string gamma_name = "";
   int spectrum_suppress_txt = 0;
@@ -126,7 +130,8 @@ class transport {
   std::vector<double> e_rad() const;
   std::vector<double> e_abs() const;
   std::vector<double> L_radio_dep() const;
-  std::vector<double> optical_spectrum() const; // flux per bin (raw counts of energy)
+  std::vector<double> optical_spectrum() const; // flux per [t][nu][mu][phi] bin (raw sums of energy)
+  std::vector<double> optical_clicks() const;   // packets per bin
   std::vector<double> abs_opacity() const;
   std::vector<double> T_gas() const;
   std::vector<double> n_elec() const;
@@ -173,6 +178,8 @@ class transport {
   std::string err_;
   std::vector<void*> owned_;
   std::vector<double> nu_edges_;
+  HostGrid spec_t_, spec_mu_, spec_phi_;   // spectrum_array's time / mu / phi grids
+  size_t n_spec_ = 0;                      // n_t n_nu n_mu n_phi
   Grid1D g_;
   sgpu_locate nu_{};
   sgpu_grid grid_{};

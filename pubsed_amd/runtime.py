@@ -104,10 +104,15 @@ CONFIGS = {
     "c2_line": dict(grid="1d", nz=128, nu=(W.C_LIGHT / 1e-4, W.C_LIGHT / 1e-5, 4096),
                     elements=((28, 56), (14, 28)), grey=0.0, epsilon=1.0, escat=1, fuzz=1,
                     radiative_eq=1, steady=1, n_lines=100_000, n_packets=10_000_000),
-    # 2D (varpi, z) 128 x 256 axis ratio 2, time-dependent with census
+    # 2D (varpi, z) 128 x 256 axis ratio 2, time-dependent with census; the
+    # escape spectrum in 10 mu bins and 0.1-day time bins as the reference's
+    # 2D benchmark (tests/ellipsoidal_outflow_2D/2D_benchmark/param.lua:45-48),
+    # the time grid starting 2 d before the ejecta's 20 d (an escaping
+    # packet's t_obs = t - x.D / c reaches back v_max t / c = 1.3 d)
     "c4_2d": dict(grid="2d", nx=128, nzz=256, nu=(W.C_LIGHT / 1e-4, W.C_LIGHT / 1e-5, 4096),
                   elements=((28, 56), (14, 28)), grey=0.0, epsilon=1.0, escat=1, fuzz=1,
-                  radiative_eq=1, steady=0, n_lines=100_000, n_packets=100_000_000),
+                  radiative_eq=1, steady=0, n_lines=100_000, n_packets=100_000_000,
+                  spec_t0=18 * W.DAY, spec_t1=28 * W.DAY, spec_dt=0.1 * W.DAY, spec_n_mu=10),
 }
 
 
@@ -186,6 +191,7 @@ class Problem:
             cfg.get("lexp", 0), cfg.get("ff", 0), cfg.get("bf", 0), cfg.get("no_ground_recomb", 0),
             cfg.get("bb", 0))
         self.op_params.line_velocity_width = cfg.get("line_velocity_width", 0.0)
+        self.op_params.minimum_extinction = cfg.get("minimum_extinction", 0.0)
         for k in ("abs", "scat", "emis_cdf"):
             a[k] = S.zeros(nz * nn, "f64")
         for k in ("L_thermal", "planck_mean", "rosseland_mean", "n_elec", "compton_opac",
@@ -210,17 +216,20 @@ class Problem:
                                            seed, steady, cfg["radiative_eq"], 0, 0, 1, 0,
                                            abi.PREC_FP64, 0)
         # tallies and spectra (spectrum_time_grid default {0,1,1}: one bin;
-        # n_mu = n_phi = 1; the gamma spectrum uses the optical nu grid, as
-        # transport_init.cpp:171-173 does)
+        # spectrum_n_mu / spectrum_n_phi default 1; the gamma spectrum uses
+        # the optical grids, as transport_init.cpp:166-173 does; mu on
+        # [-1, 1] and phi on [0, 2 pi] in equal bins, spectrum_array.cpp:66-70)
         for k in ("e_abs", "fx_rad", "fy_rad", "fz_rad", "fr_rad", "L_radio_dep",
                   "L_radio_emit", "e_rad"):
             a[k] = S.zeros(nz, "f64")
         a["J_nu"] = S.zeros(nz * nn, "f64")
         tg = W.lin_init_del(0.0, 1.0, 1.0) if steady else W.lin_init_del(
             cfg.get("spec_t0", 0.0), cfg.get("spec_t1", 1e8), cfg.get("spec_dt", 1e8))
-        self.spec_grids = dict(time=tg, nu=self.nu_np, mu=W.lin_init_n(-1, 1, 1),
-                               phi=W.lin_init_n(0, 2 * W.PI_REF, 1))
-        self.n_spec = tg["n"] * nn
+        n_mu, n_phi = int(cfg.get("spec_n_mu", 1)), int(cfg.get("spec_n_phi", 1))
+        self.spec_grids = dict(time=tg, nu=self.nu_np, mu=W.lin_init_n(-1, 1, n_mu),
+                               phi=W.lin_init_n(0, 2 * W.PI_REF, n_phi))
+        self.spec_shape = (tg["n"], nn, self.spec_grids["mu"]["n"], self.spec_grids["phi"]["n"])
+        self.n_spec = int(np.prod(self.spec_shape))
         specs = []
         for nm in ("opt", "gam"):
             a[nm + "_flux"] = S.zeros(self.n_spec, "f64")
@@ -313,7 +322,10 @@ def time_step(backend, P, dt, n_emit, scratch_pk, step_index, counters=None):
         backend.opacity_build_nlte(P, P.a["J_nu"], nlte, P.cfg.get("use_collisions_nlte", 1))
     else:
         backend.opacity_build(P)
-    backend.wipe(P)
+    # transport::wipe_radiation clears the zone tallies every step; the
+    # escape spectra only in steady iteration (SedonaClass.cpp:349), so a
+    # time-dependent run's spectrum holds every step's escapes
+    backend.wipe(P, wipe_spectra=int(P.cfg["steady"]))
     n0 = P.n_packets
     got = backend.emit(P, n_emit, id_base=(step_index + 1) << 32, offset=n0)
     n = n0 + got

@@ -60,8 +60,11 @@ def test_restart_continues_the_run(tmp_path, grid):
         assert np.array_equal(a[k], c[k]), k
     for k in ("rho", "T", "vol"):
         assert np.array_equal(A.get(k), C.get(k)), k
-    for k in ("e_abs", "J_nu", "L_radio_dep"):
+    for k in ("e_abs", "J_nu", "L_radio_dep", "opt_flux", "gam_flux"):
         assert np.allclose(A.get(k), C.get(k), rtol=1e-12, atol=0), k
+    # the spectra hold every step's escapes, the first step's from the checkpoint
+    for k in ("opt_click", "gam_click"):
+        assert np.array_equal(A.get(k), C.get(k)), k
 
 
 def test_checkpoint_rejects_another_configuration(tmp_path):

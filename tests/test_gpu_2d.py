@@ -115,7 +115,10 @@ def test_propagate_fp64_2d_matches_oracle(gpu, steady):
         assert np.sum(b["fate"] == abi.STOPPED) > 0
         st = b["fate"] == abi.STOPPED
         assert np.allclose(a["t"][st & same], b["t"][st & same], rtol=1e-12, atol=0)
-    for k in ("e_abs", "J_nu", "L_radio_dep", "opt_flux"):
+    # the escape spectrum in the 2D benchmark's 10 mu bins (and 0.1-day time
+    # bins when time-dependent), bin for bin
+    assert Pg.spec_shape[2] == 10 and (steady or Pg.spec_shape[0] == 100)
+    for k in ("e_abs", "J_nu", "L_radio_dep", "opt_flux", "opt_click"):
         d = frac_diff(Pg.get(k), Ph.get(k))
         assert d <= 10 * (1 - same.mean()) + 1e-9, (k, d)
 

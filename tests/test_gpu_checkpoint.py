@@ -50,5 +50,8 @@ def test_gpu_restart_continues_the_run(gpu, tmp_path, grid, precision):
     assert np.array_equal(a["id"], c["id"])
     for k in FIELDS:
         assert np.array_equal(a[k], c[k]), k
-    for k in ("e_abs", "J_nu", "L_radio_dep"):
+    for k in ("e_abs", "J_nu", "L_radio_dep", "opt_flux", "gam_flux"):
         assert np.allclose(A.get(k), C.get(k), rtol=1e-10, atol=0), k
+    # the spectra hold every step's escapes, the first step's from the checkpoint
+    for k in ("opt_click", "gam_click"):
+        assert np.array_equal(A.get(k), C.get(k)), k

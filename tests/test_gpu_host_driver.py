@@ -94,8 +94,13 @@ def test_host_checkpoint_restart(gpu, tmp_path, precision):
     a, c = got["straight"], got["restarted"]
     print(a, c)
     assert a["alive"] == c["alive"] > 0 and a["escaped"] == c["escaped"] and a["t_now"] == c["t_now"]
-    for k in ("L_dep", "E_abs_rate", "J_sum"):
+    for k in ("L_dep", "E_abs_rate", "J_sum", "spec_flux", "spec_flux_weighted"):
         assert abs(c[k] / a[k] - 1) < 1e-10, k
+    # the spectrum (half-day time bins from 18 d, 4 mu bins) holds the
+    # escapes of all three steps in both runs: the restarted run's first-step
+    # escapes come from the checkpoint
+    assert a["spec_clicks"] == c["spec_clicks"] > 0
+    assert a["spec_bins"] == c["spec_bins"] == 20 * 1024 * 4
     P = runtime.Problem(runtime.Space(None), "c1_grey")
     P.alloc_packets(3 * n)
     meta = checkpoint.read_checkpoint(P, chk)
@@ -107,3 +112,5 @@ def test_host_checkpoint_restart(gpu, tmp_path, precision):
     tol = 1e-12 if precision == abi.PREC_FP64 else 1e-6
     assert np.all(pk["fate"] == abi.STOPPED) and np.allclose(pk["t"], meta["t_now"], rtol=tol, atol=0)
     assert np.all((pk["ind"] >= 0) & (pk["ind"] < P.nz))
+    spec = np.load(os.path.join(chk, "state", "opt_click.npy"))
+    assert spec.shape == (20 * 1024 * 4,) and spec.sum() > 0

@@ -107,20 +107,33 @@ def test_free_free_and_bound_free_match_oracle(gpu, ff, bf, lexp, ngr):
     assert not np.allclose(P0.get("abs"), Ph.get("abs"), rtol=1e-6, atol=0)
 
 
-@pytest.mark.parametrize("width,extra", [(3e7, {}), (3e8, dict(ff=1, bf=1, lexp=1))])
+@pytest.mark.parametrize("width,extra", [(0.0, {}), (3e7, {}), (3e8, dict(ff=1, bf=1, lexp=1)),
+                                         (3e8, dict(minimum_extinction=1e-24))])
 def test_bound_bound_matches_oracle(gpu, width, extra):
     """GasState::bound_bound_opacity (Voigt-profile level-resolved lines over
-    +-5 Doppler widths, one thread per (zone, line), per-bin FP64 atomics) in
+    +-5 Doppler widths, gathered per (zone, bin) in a fixed line order) in
     the reference's order after the continua, alone and beside them and the
-    line expansion, 32 zones of the line ejecta with line_velocity_width set
-    so that lines span several of the 4096 bins (at 3e7 cm/s a line's +-5
-    widths cover about one bin: many lines add nothing)."""
+    line expansion, and with opacity_minimum_extinction cutting about 12 %
+    of the bins' terms, 32 zones of the line ejecta; thermal widths (0:
+    almost every line narrower than its bin) and line_velocity_width set so
+    that lines span several of the 4096 bins (at 3e7 cm/s a line's +-5
+    widths cover about one bin).  Two GPU builds are bit-identical."""
     over = dict(nz=32, bb=1, line_velocity_width=width, **extra)
     Pg, Ph = run_both("c2_line", gpu, **over)
     for k in KEYS_TABLE + KEYS_ZONE:
         e = relerr(Pg.get(k), Ph.get(k), floor=1e-300)
         print(k, e)
         assert e <= RTOL, (k, e)
-    P0 = runtime.Problem(runtime.Space(None), "c2_line", **dict(over, bb=0))
-    OracleBackend().opacity_build(P0)
-    assert not np.allclose(P0.get("abs"), Ph.get("abs"), rtol=1e-6, atol=0)
+    P2 = runtime.Problem(runtime.Space("cuda:0"), "c2_line", **over)
+    gpu.opacity_build(P2)
+    gpu.sync()
+    for k in KEYS_TABLE + KEYS_ZONE:
+        assert np.array_equal(P2.get(k), Pg.get(k)), k
+    if width > 0:
+        P0 = runtime.Problem(runtime.Space(None), "c2_line", **dict(over, bb=0))
+        OracleBackend().opacity_build(P0)
+        assert not np.allclose(P0.get("abs"), Ph.get("abs"), rtol=1e-6, atol=0)
+    if extra.get("minimum_extinction"):
+        P1 = runtime.Problem(runtime.Space(None), "c2_line", **dict(over, minimum_extinction=0.0))
+        OracleBackend().opacity_build(P1)
+        assert not np.array_equal(P1.get("abs"), Ph.get("abs"))

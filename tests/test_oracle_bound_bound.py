@@ -99,3 +99,33 @@ def test_line_narrower_than_its_bin_adds_nothing():
     z2 = OneZone([], T=1.0e4, dens=1e-13)
     add_bb_line(z2, 0, 1, 4.7e8, 3e8)
     assert np.count_nonzero(z2.run()["abs"]) >= 5
+
+
+@pytest.mark.parametrize("T", [8.0e3, 2.0e4])
+def test_minimum_extinction_cut(T):
+    """opacity_minimum_extinction (AtomicSpecies_opacities.cpp:259): a line
+    is skipped when alpha_0 / (nu_0^2 dnu) falls below the cut, with
+    alpha_0 = n_l n g_u/g_l A c^2 / (8 pi) (1 - r) and dnu = (v / c) nu_0.
+    The threshold is computed here from the state the build returns; a cut
+    0.1 % above it removes the line, 0.1 % below leaves the tables as with
+    no cut (a threshold formula off by a factor nu_0, dnu or the stimulated
+    term fails one side)."""
+    width, A_ul, dens = 3e8, 4.7e8, 1e-13
+    z0 = OneZone([], T=T, dens=dens)
+    nu0 = add_bb_line(z0, 0, 1, A_ul, width)
+    o0 = {k: v.copy() for k, v in z0.run().items()}
+    assert np.count_nonzero(o0["abs"]) >= 5
+    gl, gu = 2.0, 8.0
+    nl, nup = o0["lev_n"][0], o0["lev_n"][1]
+    n_dens = dens / MP
+    alpha_0 = nl * n_dens * gu / gl * A_ul / (8 * PI_REF) * C * C * (1 - nup * gl / (nl * gu))
+    thr = alpha_0 / (nu0 * nu0 * (width / C * nu0))
+    for fac, kept in ((0.999, True), (1.001, False)):
+        z = OneZone([], T=T, dens=dens)
+        add_bb_line(z, 0, 1, A_ul, width)
+        z.p.minimum_extinction = fac * thr
+        o = z.run()
+        if kept:
+            assert np.array_equal(o["abs"], o0["abs"]) and np.array_equal(o["emis_cdf"], o0["emis_cdf"])
+        else:
+            assert np.all(o["abs"] == 0)
