@@ -85,8 +85,14 @@ def read_checkpoint(P, path):
         raise ValueError("checkpoint holds %d packets; the problem has room for %d"
                          % (n, 0 if P.packets is None else P.packets.capacity))
     ld = lambda sub, k: np.load(os.path.join(path, sub, k + ".npy"), allow_pickle=False)
-    for k in _zone_keys(P) + TALLIES:
-        P.set(k, ld("state", k))
+    state = {k: ld("state", k) for k in _zone_keys(P) + TALLIES}
+    for k, v in state.items():
+        have = P.a[k].numel() if P.space.on_gpu else P.a[k].size
+        if v.size != have:
+            raise ValueError("checkpoint dataset state/%s holds %d values, the problem %d (another "
+                             "spectrum grid?)" % (k, v.size, have))
+    for k, v in state.items():
+        P.set(k, v)
     if n:
         x, D = ld("particles", "x"), ld("particles", "D")
         arrays = {"x": x[:, 0], "y": x[:, 1], "z": x[:, 2], "Dx": D[:, 0], "Dy": D[:, 1], "Dz": D[:, 2]}

@@ -83,3 +83,15 @@ def test_checkpoint_rejects_another_configuration(tmp_path):
         checkpoint.read_checkpoint(R, str(tmp_path / "chk"))
     with open(str(tmp_path / "chk" / "meta.json")) as f:
         assert json.load(f)["config"] == "c1_grey"
+
+
+def test_checkpoint_rejects_another_spectrum_grid(tmp_path):
+    ob = OracleBackend()
+    P = runtime.Problem(runtime.Space(None), "c1_grey", **OVER)
+    P.alloc_packets(1000)
+    runtime.time_step(ob, P, 0.3 * W.DAY, 500, None, 0)
+    checkpoint.write_checkpoint(P, str(tmp_path / "chk"))
+    Q = runtime.Problem(runtime.Space(None), "c1_grey", **dict(OVER, spec_n_mu=4))
+    Q.alloc_packets(1000)
+    with pytest.raises(ValueError, match="spectrum"):
+        checkpoint.read_checkpoint(Q, str(tmp_path / "chk"))

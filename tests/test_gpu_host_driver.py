@@ -101,7 +101,8 @@ def test_host_checkpoint_restart(gpu, tmp_path, precision):
     # escapes come from the checkpoint
     assert a["spec_clicks"] == c["spec_clicks"] > 0
     assert a["spec_bins"] == c["spec_bins"] == 20 * 1024 * 4
-    P = runtime.Problem(runtime.Space(None), "c1_grey")
+    P = runtime.Problem(runtime.Space(None), "c1_grey", steady=0, spec_t0=18 * 86400.0, spec_t1=28 * 86400.0,
+                        spec_dt=0.5 * 86400.0, spec_n_mu=4)
     P.alloc_packets(3 * n)
     meta = checkpoint.read_checkpoint(P, chk)
     assert P.n_packets == meta["n_packets"] > 0
