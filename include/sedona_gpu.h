@@ -22,6 +22,7 @@
  *                                                             src/opacity/GasState.cpp:208-337
  *                                                             src/opacity/GasState_opacities.cpp:11-136,464-494,564-666
  *   sgpu_emit_radioactive   transport::emit_radioactive       src/transport/emission.cpp:165-243
+ *   sgpu_initialize_particles transport::initialize_particles src/transport/emission.cpp:112-159
  *   sgpu_propagate          the packet loop of transport::step src/transport/transport.cpp:57-88
  *                           + propagate / propagate_monte_carlo :202-484
  *   sgpu_compact            transport::clean_up_particle_vector src/transport/transport.cpp:164-194
@@ -367,6 +368,29 @@ int sgpu_emit_radioactive(sgpu_ctx* ctx,
                           int64_t n_emit, int32_t force_rproc, uint64_t id_base,
                           sgpu_packets* packets, int64_t offset,
                           sgpu_tallies* tallies, int64_t* n_out);
+
+/* Initial radiation field (transport::initialize_particles,
+ * src/transport/emission.cpp:112-159, with create_isotropic_particle
+ * :54-105): n_init photon packets into packets[offset .. offset+n_init),
+ * stream ids id_base + q, the zone drawn from a CDF over e_rad[z] vol[z],
+ * the position uniform in the zone's volume, the direction isotropic and the
+ * frequency from B_nu(T_gas) dnu in the comoving frame, energy
+ * sum(e_rad vol) / n_init, time params->t_now, then Lorentz-transformed to
+ * the lab frame.  As the reference overwrites its emissivity, emis_cdf
+ * [nz][n_nu] receives every zone's normalised blackbody CDF (the next
+ * opacity build replaces it).  e_rad [nz] erg/cm^3.  As the reference,
+ * nothing is done (*n_out = 0, emis_cdf kept) when n_init is 0 or the
+ * packets do not fit the buffer. */
+int sgpu_initialize_particles(sgpu_ctx* ctx,
+                              const sgpu_grid* grid,
+                              const sgpu_locate* nu_grid,
+                              const sgpu_zone_state* zones,
+                              const double* e_rad,
+                              double* emis_cdf,
+                              const sgpu_transport_params* params,
+                              int64_t n_init, uint64_t id_base,
+                              sgpu_packets* packets, int64_t offset,
+                              int64_t* n_out);
 
 /* ------------------------------------------------------------------ */
 /* K4: propagation                                                     */

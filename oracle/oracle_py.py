@@ -57,6 +57,10 @@ def load():
                                             P(abi.Tables), P(abi.TransportParams), I64, I,
                                             ctypes.c_uint64, P(abi.Packets), I64, P(abi.Tallies),
                                             P(I64)]
+        lib.or_initialize_particles.restype = I
+        lib.or_initialize_particles.argtypes = [P(abi.Grid), P(abi.Locate), P(abi.ZoneState), V, V,
+                                                P(abi.TransportParams), I64, ctypes.c_uint64,
+                                                P(abi.Packets), I64, P(I64)]
         lib.or_propagate.restype = I
         lib.or_propagate.argtypes = [P(abi.Grid), P(abi.Locate), P(abi.Tables),
                                      P(abi.TransportParams), P(abi.Packets), I64, P(abi.Tallies),
@@ -138,6 +142,16 @@ class OracleBackend:
                                      ctypes.byref(P.tables), ctypes.byref(P.tparams), n_emit,
                                      force_rproc, id_base, ctypes.byref(P.packets), offset,
                                      ctypes.byref(P.tallies), ctypes.byref(nout))
+        return nout.value
+
+    def initialize_particles(self, P, n_init, id_base=0, offset=0):
+        """transport::initialize_particles from P's e_rad tally; rewrites
+        P's emissivity CDF with the zones' blackbody distributions."""
+        nout = ctypes.c_int64(0)
+        self.lib.or_initialize_particles(ctypes.byref(P.grid), ctypes.byref(P.nu), ctypes.byref(P.zs),
+                                         ptr(P.a["e_rad"]), ptr(P.a["emis_cdf"]), ctypes.byref(P.tparams),
+                                         n_init, id_base, ctypes.byref(P.packets), offset,
+                                         ctypes.byref(nout))
         return nout.value
 
     def propagate(self, P, n, counters=None):

@@ -816,6 +816,68 @@ int or_emit_radioactive(const sgpu_grid* grid, const sgpu_locate* nu_grid,
   return 0;
 }
 
+/* transport::initialize_particles (emission.cpp:112-159) with
+ * create_isotropic_particle (:54-105): the zone CDF over e_rad V, each
+ * zone's emissivity overwritten with its normalised B_nu(T_gas) dnu at the
+ * bin centres (written into emis_cdf, as the reference overwrites
+ * emissivity_), E_p = sum(e_rad V) / n, photons at t_now.  Packet q draws,
+ * in order: zone, 3 position, 2 direction, 2 frequency.  Nothing is done
+ * (and emis_cdf is kept) for n_init = 0 or when the packets do not fit, as
+ * the reference returns early in both cases. */
+int or_initialize_particles(const sgpu_grid* grid, const sgpu_locate* nu_grid,
+                            const sgpu_zone_state* zs, const double* e_rad, double* emis_cdf,
+                            const sgpu_transport_params* prm, int64_t n_init, uint64_t id_base,
+                            sgpu_packets* pk, int64_t offset, int64_t* n_out)
+{
+  *n_out = 0;
+  if (n_init <= 0 || offset + n_init > pk->capacity) return 0;
+  int nz = grid->n_zones, nn = nu_grid->n;
+  double* cdf = (double*)malloc(sizeof(double) * nz);
+  double E_sum = 0;
+  for (int i = 0; i < nz; i++) {
+    double E_zone = e_rad[i] * grid->vol[i];
+    cdf[i] = i == 0 ? E_zone : cdf[i - 1] + E_zone;
+    E_sum += E_zone;
+    double T = zs->T_gas[i];
+    double* em = emis_cdf + (size_t)i * nn;
+    for (int j = 0; j < nn; j++) {
+      double nu_m = or_center(nu_grid, j);
+      double e = or_blackbody_nu(T, nu_m) * or_delta(nu_grid, j);
+      em[j] = j == 0 ? e : em[j - 1] + e;
+    }
+    or_cdf_normalize(em, nn);
+  }
+  or_cdf_normalize(cdf, nz);
+  double Ep = E_sum / (1.0 * n_init);
+  sgpu_tables tab = {0};
+  tab.emis_cdf = emis_cdf;
+  tctx_t c = {grid, nu_grid, &tab, prm, 0, 0, 0, 0, 0, 0};
+  for (int64_t q = 0; q < n_init; q++) {
+    pkt_t p;
+    memset(&p, 0, sizeof(p));
+    p.id = id_base + (uint64_t)q;
+    p.draw = 0;
+    int i = or_cdf_sample(cdf, nz, urand(&c, &p));
+    p.ind = i;
+    p.type = SGPU_PHOTON;
+    double ran[3];
+    ran[0] = urand(&c, &p); ran[1] = urand(&c, &p); ran[2] = urand(&c, &p);
+    sample_in_zone(grid, i, ran, p.x);
+    double mu = 1 - 2.0 * urand(&c, &p);
+    double phi = 2.0 * OR_PI * urand(&c, &p);
+    double smu = sqrt(1 - mu * mu);
+    p.D[0] = smu * cos(phi); p.D[1] = smu * sin(phi); p.D[2] = mu;
+    sample_photon_frequency(&c, &p);
+    p.e = Ep;
+    lorentz_transform(&c, &p, 1);
+    p.t = prm->t_now;
+    store_pkt(pk, offset + q, &p, SGPU_MOVING);
+  }
+  *n_out = n_init;
+  free(cdf);
+  return 0;
+}
+
 /* transport::reduce_radiation normalisation, transport_mpi.cpp:415-441 */
 int or_normalize_tallies(const sgpu_grid* grid, const sgpu_locate* nu,
                          const sgpu_transport_params* prm, sgpu_tallies* ta)

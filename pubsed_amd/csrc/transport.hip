@@ -1147,6 +1147,119 @@ __global__ void __launch_bounds__(256) emit_kernel(EmitArgs A)
 }
 
 // ------------------------------------------------------------------
+// initial radiation field (transport::initialize_particles,
+// emission.cpp:112-159, with create_isotropic_particle :54-105)
+// ------------------------------------------------------------------
+// y[0..n) := cdf_array::set_value's running sum then normalize
+// (cdf_array.h:53-75) by one workgroup of NT threads: each thread scans a
+// contiguous chunk, the chunk totals are scanned across the block; a NaN or
+// zero total makes the CDF all ones.  Returns the total to every thread.
+template <int NT>
+__device__ double block_cdf(double* y, int n, double* wsum)
+{
+  const int tid = threadIdx.x;
+  const int per = (n + NT - 1) / NT;
+  const int j0 = tid * per, j1 = min(n, j0 + per);
+  double local = 0;
+  for (int j = j0; j < j1; ++j) local += y[j];
+  const int lane = tid % WAVE, w = tid / WAVE;
+  double incl = local;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    double t = __shfl_up(incl, o, WAVE);
+    if (lane >= o) incl += t;
+  }
+  if (lane == WAVE - 1) wsum[w] = incl;
+  __syncthreads();
+  double base = 0, total = 0;
+  for (int i = 0; i < NT / WAVE; i++) {
+    if (i < w) base += wsum[i];
+    total += wsum[i];
+  }
+  double run = base + incl - local;
+  for (int j = j0; j < j1; ++j) { run += y[j]; y[j] = run; }
+  __syncthreads();
+  const double last = y[n - 1];
+  const bool flat = isnan(last) || last == 0;
+  __syncthreads();   // every thread has read the total before any rewrites it
+  for (int j = j0; j < j1; ++j) y[j] = flat ? 1.0 : y[j] / last;
+  return total;
+}
+
+constexpr int INIT_THREADS = 256;
+
+// zone CDF over e_rad V and E_sum (one workgroup)
+__global__ void __launch_bounds__(1024) init_zone_cdf_kernel(const double* e_rad, const double* vol, int nz,
+                                                             double* cdf, double* E_sum)
+{
+  __shared__ double wsum[1024 / WAVE];
+  for (int i = threadIdx.x; i < nz; i += blockDim.x) cdf[i] = e_rad[i] * vol[i];
+  __syncthreads();
+  const double tot = block_cdf<1024>(cdf, nz, wsum);
+  if (threadIdx.x == 0) *E_sum = tot;
+}
+
+// every zone's emissivity CDF from B_nu(T_gas) dnu at the bin centres
+// (transport::blackbody_nu, transport_opacity.cpp:322-326); one workgroup per zone
+__global__ void __launch_bounds__(INIT_THREADS) init_blackbody_cdf_kernel(sgpu_locate nu, const double* T_gas,
+                                                                          double* emis_cdf)
+{
+  __shared__ double wsum[INIT_THREADS / WAVE];
+  const int z = blockIdx.x, nn = nu.n;
+  const double T = T_gas[z];
+  double* y = emis_cdf + (size_t)z * nn;
+  for (int j = threadIdx.x; j < nn; j += INIT_THREADS) {
+    const double nu_m = loc_center(nu, j);
+    const double zeta = H_PLANCK * nu_m / K_BOLTZ / T;
+    y[j] = 2.0 * nu_m * nu_m * nu_m * H_PLANCK / C_LIGHT / C_LIGHT / (exp(zeta) - 1) * loc_delta(nu, j);
+  }
+  __syncthreads();
+  block_cdf<INIT_THREADS>(y, nn, wsum);
+}
+
+struct InitArgs {
+  GridView<double> g;
+  Axis<double> nu;
+  const double* emis_cdf;
+  const double* zcdf;
+  const double* E_sum;
+  double t_now;
+  uint64_t seed, id_base;
+  int64_t n_init, offset;
+  sgpu_packets pk;
+};
+
+// one photon per thread: zone, 3 position, 2 direction and 2 frequency draws
+__global__ void __launch_bounds__(256) init_particles_kernel(InitArgs A)
+{
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= A.n_init) return;
+  const int nz = A.g.n_zones;
+  Pkt<double> p;
+  p.rng = make_rng(A.seed, A.id_base + (uint64_t)q, 0);
+  const int i = min(cdf_sample(A.zcdf, nz, p.rng.uniform()), nz - 1);
+  p.ind = i;
+  p.type = SGPU_PHOTON;
+  double ran[3];
+  ran[0] = p.rng.uniform(); ran[1] = p.rng.uniform(); ran[2] = p.rng.uniform();
+  sample_in_zone(A.g, i, ran, p.x);
+  isotropic_dir(p, p.D);
+  const int nn = A.nu.n;
+  const int inu = min(cdf_sample(A.emis_cdf + (size_t)i * nn, nn, p.rng.uniform()), nn - 1);
+  p.nu = ax_sample(A.nu, inu, p.rng.uniform());
+  p.e = *A.E_sum / (1.0 * A.n_init);
+  lorentz_transform(A.g, p, 1);
+  p.t = A.t_now;
+  const int64_t o = A.offset + q;
+  const sgpu_packets& k = A.pk;
+  k.x[o] = p.x[0]; k.y[o] = p.x[1]; k.z[o] = p.x[2];
+  k.Dx[o] = p.D[0]; k.Dy[o] = p.D[1]; k.Dz[o] = p.D[2];
+  k.t[o] = p.t; k.e[o] = p.e; k.nu[o] = p.nu;
+  k.ind[o] = p.ind; k.type[o] = p.type; k.fate[o] = SGPU_MOVING;
+  k.draw[o] = p.rng.draw; k.id[o] = A.id_base + (uint64_t)q;
+}
+
+// ------------------------------------------------------------------
 // K6: stream compaction (clean_up_particle_vector), order-keeping
 // ------------------------------------------------------------------
 constexpr int CMP_THREADS = 256;
@@ -1525,6 +1638,46 @@ extern "C" int sgpu_emit_radioactive(sgpu_ctx* ctx, const sgpu_grid* grid, const
   SGPU_HIP(hipMemcpyAsync(&hl, ltot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   SGPU_HIP(hipStreamSynchronize(ctx->stream));
   *n_out = (hl == 0) ? 0 : n_emit;
+  return SGPU_OK;
+}
+
+extern "C" int sgpu_initialize_particles(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_locate* nu_grid,
+                                         const sgpu_zone_state* zones, const double* e_rad, double* emis_cdf,
+                                         const sgpu_transport_params* params, int64_t n_init, uint64_t id_base,
+                                         sgpu_packets* packets, int64_t offset, int64_t* n_out)
+{
+  SGPU_REQUIRE(ctx && nu_grid && zones && params && n_out, "null argument");
+  int st = check_grid(grid);
+  if (st) return st;
+  st = check_packets(packets);
+  if (st) return st;
+  SGPU_REQUIRE(zones->n_zones == grid->n_zones, "zone count mismatch");
+  SGPU_REQUIRE(e_rad && emis_cdf && zones->T_gas, "missing arrays");
+  SGPU_REQUIRE(nu_grid->n >= 1 && nu_grid->x, "bad frequency grid");
+  SGPU_REQUIRE(n_init >= 0 && offset >= 0, "negative count or offset");
+  *n_out = 0;
+  // emission.cpp:116-129: nothing to do, or no room -- return before the
+  // emissivity is rewritten
+  if (n_init == 0 || offset + n_init > packets->capacity) return SGPU_OK;
+  const int nz = grid->n_zones;
+  double* zcdf = (double*)scratch(ctx, "init_zcdf", sizeof(double) * ((size_t)nz + 1), &st);
+  if (st) return st;
+  double* esum = zcdf + nz;
+  timer_begin(ctx, "emit");
+  hipLaunchKernelGGL(init_zone_cdf_kernel, dim3(1), dim3(1024), 0, ctx->stream, e_rad, grid->vol, nz, zcdf, esum);
+  hipLaunchKernelGGL(init_blackbody_cdf_kernel, dim3(nz), dim3(INIT_THREADS), 0, ctx->stream, *nu_grid,
+                     zones->T_gas, emis_cdf);
+  SGPU_LAUNCH_CHECK();
+  InitArgs A;
+  A.g = make_grid<double>(*grid, grid->r_out.x, grid->x_out.x, grid->z_out.x);
+  A.nu = make_axis<double>(*nu_grid, nu_grid->x);
+  A.emis_cdf = emis_cdf; A.zcdf = zcdf; A.E_sum = esum;
+  A.t_now = params->t_now; A.seed = params->seed; A.id_base = id_base;
+  A.n_init = n_init; A.offset = offset; A.pk = *packets;
+  hipLaunchKernelGGL(init_particles_kernel, dim3(grid_blocks(n_init, 256)), dim3(256), 0, ctx->stream, A);
+  SGPU_LAUNCH_CHECK();
+  timer_end(ctx, "emit");
+  *n_out = n_init;
   return SGPU_OK;
 }
 

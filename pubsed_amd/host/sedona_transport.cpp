@@ -15,6 +15,7 @@ namespace sedona_gpu {
 
 namespace {
 constexpr double PI_REF = 3.14159;   // src/utils/physical_constants.h:9
+constexpr double A_RAD_REF = 7.5657e-15;   // radiation constant, physical_constants.h:17
 
 // locate_array::log_init with a spacing nudged until it yields exactly n bins
 std::vector<double> log_edges(double lo, double hi, int n, double* del_out)
@@ -196,6 +197,25 @@ int transport::init(const Params& p, const Grid1D& g, const AtomicData* atoms, i
     if (!q->id) return fail(SGPU_ERR_HIP, "packet buffers");
   }
   if (sgpu_wipe_radiation(ctx_, nz_, nn_, &ta_, 1) != SGPU_OK) return fail(SGPU_ERR_HIP, "wipe");
+  // transport_init.cpp:376-379: the initial radiation field from the zones'
+  // e_rad (a T_gas^4 unless the model gives it, grid_1D_sphere.cpp:130-134, 354)
+  if (p.particles_n_initialize > 0) {
+    std::vector<double> er = g.e_rad;
+    if (er.empty())
+      for (int i = 0; i < nz_; ++i) er.push_back(A_RAD_REF * std::pow(g.T_gas[i], 4.));
+    if ((int)er.size() != nz_) return fail(SGPU_ERR_ARG, "e_rad needs one value per zone");
+    if (sgpu_memcpy_h2d(ctx_, ta_.e_rad, er.data(), sizeof(double) * nz_) != SGPU_OK)
+      return fail(SGPU_ERR_HIP, "e_rad upload");
+    sgpu_transport_params tp{};
+    tp.t_now = t_now_;
+    tp.seed = p.transport_rng_seed;
+    int64_t got = 0;
+    // stream ids above every emission's (step_index << 32 | q)
+    int rc = sgpu_initialize_particles(ctx_, &grid_, &nu_, &zs_, ta_.e_rad, out_.emis_cdf, &tp,
+                                       p.particles_n_initialize, (uint64_t)0xFFFFFFFFull << 32, &pk_, 0, &got);
+    if (rc < 0) return fail(rc, "initialize_particles");
+    n_particles_ = got;
+  }
   return SGPU_OK;
 }
 

@@ -45,6 +45,7 @@ struct Params {
   double opacity_maximum_opacity = 1e40;
   double limits_temp_min = 1000;
   double limits_temp_max = 1e8;
+  int64_t particles_n_initialize = 0;        // initial radiation field packets (initialize_particles)
   int64_t particles_n_emit_radioactive = 0;
   int64_t particles_max_total = 10000000;
   uint64_t transport_rng_seed = 20261203;
@@ -94,6 +95,7 @@ struct Grid1D {
   double r_min = 0;
   double t_now = 0;               // grid clock (velocity = r / t_now)
   std::vector<double> rho, T_gas;
+  std::vector<double> e_rad;      // radiation energy density; empty: a T_gas^4 (grid_1D_sphere.cpp:354)
   std::vector<double> X;          // [nz][n_elems]
   std::vector<int32_t> elems_Z, elems_A;
   int n_zones() const { return (int)r_out.size(); }

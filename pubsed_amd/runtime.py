@@ -391,6 +391,18 @@ class GpuBackend:
                                              ctypes.byref(nout)), "sgpu_emit_radioactive")
         return nout.value
 
+    def initialize_particles(self, P, n_init, id_base=0, offset=0):
+        """sgpu_initialize_particles from P's e_rad tally; rewrites P's
+        emissivity CDF with the zones' blackbody distributions."""
+        from . import check
+        nout = ctypes.c_int64(0)
+        check(self.lib.sgpu_initialize_particles(self.ctx, ctypes.byref(P.grid), ctypes.byref(P.nu),
+                                                 ctypes.byref(P.zs), abi.ptr(P.a["e_rad"]),
+                                                 abi.ptr(P.a["emis_cdf"]), ctypes.byref(P.tparams), n_init,
+                                                 id_base, ctypes.byref(P.packets), offset, ctypes.byref(nout)),
+              "sgpu_initialize_particles")
+        return nout.value
+
     def opacity_build_nlte(self, P, J_nu, atom_nlte, use_collisions=1, lu_precision=None):
         """sgpu_opacity_build_nlte: J_nu a device tensor [nz][n_nu], atom_nlte a
         host sequence of per-atom flags.  Returns (status, n_e evaluation passes)."""
