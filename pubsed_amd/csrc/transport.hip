@@ -1681,22 +1681,14 @@ extern "C" int sgpu_initialize_particles(sgpu_ctx* ctx, const sgpu_grid* grid, c
   return SGPU_OK;
 }
 
-extern "C" int sgpu_compact(sgpu_ctx* ctx, sgpu_packets* packets, int64_t n, sgpu_packets* scr,
-                            int64_t* n_alive, int64_t* n_escaped)
+namespace {
+// the three compaction kernels and the copy back of the survivors, with the
+// per-block counts blk [2 * nb] and the two totals tot in caller-chosen device
+// memory; *n_alive / *n_escaped come back through a synchronous copy
+int compact_with(sgpu_ctx* ctx, sgpu_packets* packets, int64_t n, sgpu_packets* scr, int32_t* blk,
+                 long long* tot, int64_t* n_alive, int64_t* n_escaped)
 {
-  SGPU_REQUIRE(ctx && n_alive && n_escaped, "null argument");
-  int st = check_packets(packets);
-  if (st) return st;
-  st = check_packets(scr);
-  if (st) return st;
-  SGPU_REQUIRE(n >= 0 && n <= packets->capacity && n <= scr->capacity, "n exceeds capacity");
-  *n_alive = 0; *n_escaped = 0;
-  if (n == 0) return SGPU_OK;
   const unsigned nb = grid_blocks(n, CMP_THREADS);
-  int32_t* blk = (int32_t*)scratch(ctx, "cmp_blk", sizeof(int32_t) * 2 * (size_t)nb + 64, &st);
-  if (st) return st;
-  long long* tot = (long long*)scratch(ctx, "cmp_tot", sizeof(long long) * 2, &st);
-  if (st) return st;
   hipLaunchKernelGGL(keep_count_kernel, dim3(nb), dim3(CMP_THREADS), 0, ctx->stream, packets->fate, n, blk,
                      blk + nb);
   hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(1024), 0, ctx->stream, blk, blk + nb, (int)nb, tot);
@@ -1729,6 +1721,33 @@ extern "C" int sgpu_compact(sgpu_ctx* ctx, sgpu_packets* packets, int64_t n, sgp
   *n_alive = k;
   *n_escaped = ht[1];
   return SGPU_OK;
+}
+
+int compact_args(sgpu_ctx* ctx, sgpu_packets* packets, int64_t n, sgpu_packets* scr, int64_t* n_alive,
+                 int64_t* n_escaped)
+{
+  SGPU_REQUIRE(ctx && n_alive && n_escaped, "null argument");
+  int st = check_packets(packets);
+  if (st) return st;
+  st = check_packets(scr);
+  if (st) return st;
+  SGPU_REQUIRE(n >= 0 && n <= packets->capacity && n <= scr->capacity, "n exceeds capacity");
+  *n_alive = 0; *n_escaped = 0;
+  return SGPU_OK;
+}
+}  // namespace
+
+extern "C" int sgpu_compact(sgpu_ctx* ctx, sgpu_packets* packets, int64_t n, sgpu_packets* scr,
+                            int64_t* n_alive, int64_t* n_escaped)
+{
+  int st = compact_args(ctx, packets, n, scr, n_alive, n_escaped);
+  if (st || n == 0) return st;
+  const unsigned nb = grid_blocks(n, CMP_THREADS);
+  int32_t* blk = (int32_t*)scratch(ctx, "cmp_blk", sizeof(int32_t) * 2 * (size_t)nb + 64, &st);
+  if (st) return st;
+  long long* tot = (long long*)scratch(ctx, "cmp_tot", sizeof(long long) * 2, &st);
+  if (st) return st;
+  return compact_with(ctx, packets, n, scr, blk, tot, n_alive, n_escaped);
 }
 
 static int fill(sgpu_ctx* ctx, double* p, int64_t n)
