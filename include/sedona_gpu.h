@@ -26,6 +26,7 @@
  *   sgpu_propagate          the packet loop of transport::step src/transport/transport.cpp:57-88
  *                           + propagate / propagate_monte_carlo :202-484
  *   sgpu_compact            transport::clean_up_particle_vector src/transport/transport.cpp:164-194
+ *   sgpu_compact_noalloc    (no reference counterpart: sgpu_compact for callers that cannot allocate)
  *   sgpu_wipe_radiation     transport::wipe_radiation          src/transport/transport_mpi.cpp:18-34
  *   sgpu_normalize_tallies  transport::reduce_radiation (normalisation part)
  *                                                             src/transport/transport_mpi.cpp:415-441
@@ -416,6 +417,17 @@ int sgpu_propagate(sgpu_ctx* ctx,
  * *n_escaped are host outputs. */
 int sgpu_compact(sgpu_ctx* ctx, sgpu_packets* packets, int64_t n,
                  sgpu_packets* scratch, int64_t* n_alive, int64_t* n_escaped);
+
+/* sgpu_compact without any device allocation: no counterpart in the
+ * reference.  It is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * per-block counts and totals live in a fixed device array of the library
+ * (8 MiB + 256 B per GPU, held until the process ends), so n is limited to
+ * 2^28 = 268,435,456 packets (SGPU_ERR_SIZE beyond).  packets and scr are
+ * caller-owned as for sgpu_compact, and the results are identical to it.
+ * Calls that overlap on one GPU share that array and race. */
+int sgpu_compact_noalloc(sgpu_ctx* ctx, sgpu_packets* packets, int64_t n, sgpu_packets* scr,
+                         int64_t* n_alive, int64_t* n_escaped);
 
 /* Zero the zone and spectrum tallies (transport::wipe_radiation; the spectra
  * are wiped only when wipe_spectra != 0). */

@@ -50,6 +50,7 @@ def _declare(lib):
         "sgpu_propagate": (I, [V, P(abi.Grid), P(abi.Locate), P(abi.Tables),
                                P(abi.TransportParams), P(abi.Packets), I64, P(abi.Tallies), V]),
         "sgpu_compact": (I, [V, P(abi.Packets), I64, P(abi.Packets), P(I64), P(I64)]),
+        "sgpu_compact_noalloc": (I, [V, P(abi.Packets), I64, P(abi.Packets), P(I64), P(I64)]),
         "sgpu_wipe_radiation": (I, [V, I32, I32, P(abi.Tallies), I32]),
         "sgpu_normalize_tallies": (I, [V, P(abi.Grid), P(abi.Locate), P(abi.TransportParams),
                                        P(abi.Tallies)]),

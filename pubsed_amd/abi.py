@@ -122,6 +122,7 @@ EXPORTED_SYMBOLS = [
     "sgpu_last_error", "sgpu_alloc", "sgpu_free", "sgpu_memcpy_h2d", "sgpu_memcpy_d2h",
     "sgpu_last_kernel_ms", "sgpu_enable_timing", "sgpu_opacity_build",
     "sgpu_emit_radioactive", "sgpu_initialize_particles", "sgpu_propagate", "sgpu_compact",
+    "sgpu_compact_noalloc",
     "sgpu_wipe_radiation",
     "sgpu_normalize_tallies", "sgpu_nlte_solve", "sgpu_nlte_batched", "sgpu_nlte_autotune",
     "sgpu_homologous_step", "sgpu_adjoint_forward", "sgpu_adjoint_backward",

@@ -17,6 +17,9 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstdlib>
+#include <map>
+#include <mutex>
+
 #include "sgpu_internal.h"
 #include "sgpu_geometry.h"
 
@@ -1748,6 +1751,63 @@ extern "C" int sgpu_compact(sgpu_ctx* ctx, sgpu_packets* packets, int64_t n, sgp
   long long* tot = (long long*)scratch(ctx, "cmp_tot", sizeof(long long) * 2, &st);
   if (st) return st;
   return compact_with(ctx, packets, n, scr, blk, tot, n_alive, n_escaped);
+}
+
+// ------------------------------------------------------------------
+// sgpu_compact_noalloc: the same compaction with its per-block counts and
+// totals in a fixed file-scope device array, so that the call allocates
+// nothing.  Every GPU holds its own copy of the array, at its own address,
+// until the process ends; the address is looked up once per device with
+// hipGetSymbolAddress and only that address is handed to kernels and copies.
+// Calls that overlap on one GPU share the array and race.
+// ------------------------------------------------------------------
+constexpr int64_t NOALLOC_MAX_PACKETS = (int64_t)1 << 28;                   // 268,435,456
+constexpr size_t NOALLOC_MAX_BLOCKS = (size_t)(NOALLOC_MAX_PACKETS / CMP_THREADS);   // 2^20
+// int32 words: keep and escape counts of every block, then 256 B holding the
+// two 64-bit totals (8 MiB + 256 B in all)
+constexpr size_t NOALLOC_WORDS = 2 * NOALLOC_MAX_BLOCKS + 64;
+static __device__ __align__(256) int32_t compact_scratch[NOALLOC_WORDS];
+
+namespace {
+std::mutex noalloc_mu;
+std::map<int, int32_t*> noalloc_addr;   // device -> device address of compact_scratch
+
+int noalloc_scratch(sgpu_ctx* ctx, int32_t** out)
+{
+  std::lock_guard<std::mutex> lock(noalloc_mu);
+  auto it = noalloc_addr.find(ctx->device);
+  if (it != noalloc_addr.end()) { *out = it->second; return SGPU_OK; }
+  int cur = -1;
+  SGPU_HIP(hipGetDevice(&cur));
+  if (cur != ctx->device) SGPU_HIP(hipSetDevice(ctx->device));
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(compact_scratch));
+  if (cur != ctx->device) SGPU_HIP(hipSetDevice(cur));
+  if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(compact_scratch)", __FILE__, __LINE__);
+  SGPU_REQUIRE(p != nullptr, "no device address for the compaction scratch array");
+  noalloc_addr[ctx->device] = (int32_t*)p;
+  *out = (int32_t*)p;
+  return SGPU_OK;
+}
+}  // namespace
+
+extern "C" int sgpu_compact_noalloc(sgpu_ctx* ctx, sgpu_packets* packets, int64_t n, sgpu_packets* scr,
+                                    int64_t* n_alive, int64_t* n_escaped)
+{
+  int st = compact_args(ctx, packets, n, scr, n_alive, n_escaped);
+  if (st) return st;
+  if (n > NOALLOC_MAX_PACKETS) {
+    set_error("sgpu_compact_noalloc: %lld packets exceed its fixed scratch limit of %lld",
+              (long long)n, (long long)NOALLOC_MAX_PACKETS);
+    return SGPU_ERR_SIZE;
+  }
+  if (n == 0) return SGPU_OK;
+  int32_t* base = nullptr;
+  if ((st = noalloc_scratch(ctx, &base))) return st;
+  // every word the kernels read is written first in the same call (block
+  // counts by keep_count_kernel, totals by scan_blocks_kernel): nothing to clear
+  long long* tot = (long long*)(base + 2 * NOALLOC_MAX_BLOCKS);
+  return compact_with(ctx, packets, n, scr, base, tot, n_alive, n_escaped);
 }
 
 static int fill(sgpu_ctx* ctx, double* p, int64_t n)
