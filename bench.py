@@ -9,11 +9,17 @@ path with inputs resident in HBM: opacity / emissivity build, emission,
 propagation to escape, tally normalisation, radiative-equilibrium gas
 temperature.
 
-Reported: packets/s of the batched mixed-precision form (FP32 propagation,
-FP64 tallies), beside the plain FP64 drop-in at equal packet count, the NLTE
-zone-solves/s of the batched FP32+refinement solve versus the per-system
-FP64 drop-in, the roofline of the propagation kernel and the CPU oracle timed
-on this host's cores.
+Reported (one JSON line): packets/s of the batched mixed-precision form
+(FP32 propagation, FP64 tallies) as `value`; beside it the plain FP64
+drop-in and the batched FP64 form (the FP64 arithmetic with the mixed
+form's launch and tally scheme) at the same packet count when the time
+budget allows; the propagation kernel's roofline on its own algorithmic
+bytes and on the survey formula; the batched NLTE solves (FP32 LU + FP64
+refinement and FP64) against the per-system drop-in and against the CPU
+oracle; the NLTE-coupled opacity build; the 2D time-dependent outflow
+(census, 1e8 packets per step); the differentiable transport's forward /
+adjoint times and its gradient against central finite differences; and the
+CPU oracle timed on this host's cores (all cores and one thread, per phase).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--packets N]
 """
@@ -27,6 +33,7 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+T_START = time.perf_counter()
 
 HBM_PEAK_GBS = 8000.0
 # the mixed-form propagation kernel the default launch runs (4-wave register
@@ -37,44 +44,59 @@ PROP_KERNEL = "propagate_kernel<float, true, true, 4, %d, false>"
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--packets", type=int, default=None,
                     help="packets per step (default: the configuration's own count)")
     ap.add_argument("--config", default="c2_line")
-    ap.add_argument("--plain-packets", type=int, default=1_000_000,
-                    help="packets of the plain FP64 comparison step (its rate is linear in the "
-                         "packet count from about 4e5 packets up; 0: same as --packets)")
+    ap.add_argument("--budget-s", type=float, default=540.0,
+                    help="wall-clock budget of the whole run: the FP64 comparison legs run at the "
+                         "full packet count when their estimated time fits, else at the largest "
+                         "count that does (reported as plain_packets / fp64_batched_packets)")
     ap.add_argument("--traffic-from", default=os.path.join(ROOT, "profiles", "propagate_mixed_c2_line.md"),
                     help="rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the propagation kernel")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (the HBM counter calibration) and exit")
     ap.add_argument("--nlte-only", action="store_true",
                     help="only the batched NLTE measurement (for profiling its kernel)")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="only the headline mixed-precision leg (for profiling its kernel)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-nlte", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-extra", action="store_true", help="skip the 2D and differentiable legs")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison legs")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
     return ap.parse_args()
+
+
+def elapsed():
+    return time.perf_counter() - T_START
+
+
+def log(msg):
+    print("# [%6.1f s] %s" % (elapsed(), msg), file=sys.stderr, flush=True)
 
 
 S_STATE = 96   # packet record in HBM: 9 FP64 + 3 int32 + uint32 draw + uint64 id
 
 
-def packet_bytes(c, nn, precision):
-    """Algorithmic HBM bytes per packet of the propagation kernel,
-    B_pkt = 2 S_state + n_step (2 w_kappa + 8) + n_redist 8 ceil(log2 n_nu) + f_esc 16:
-    the packet record read and written once, per event the opacity gather
-    (w_kappa = 8 B per FP64 entry, 4 B per FP32 entry) and an 8-byte J_nu
-    accumulate, per redistribution the binary search of the FP64 emissivity
-    CDF, 16 B of spectrum per escape.  (The FP32 form samples the CDF through
-    a guide table, one 4-byte entry and one or two 4-byte CDF probes instead
-    of 12 probes: fewer bytes than this figure counts.)"""
+def packet_bytes(c, nn, precision, survey=False):
+    """Algorithmic HBM bytes per packet of the propagation kernel: the packet
+    record read and written once (2 S_state), per event the opacity gather
+    (2 w_kappa: abs and scat at 8 B in FP64, one packed FP32 (kappa, eps)
+    pair in the mixed form) and an 8-byte J_nu accumulate, per
+    redistribution the frequency sample, 16 B of spectrum per escape.  The
+    FP64 forms sample by binary search over the zone's FP64 emissivity CDF
+    (8 ceil(log2 n_nu) B); the mixed form draws one 8-byte Walker alias
+    entry.  survey=True charges every form the binary search (the
+    B_pkt formula as the survey writes it)."""
     n = max(c["n_packets"], 1)
     n_step = c["n_events"] / n
     n_redist = c["n_redist"] / n
     f_esc = c["n_escaped"] / n
-    wk = 8 if precision == 0 else 4
-    return 2 * S_STATE + n_step * (2 * wk + 8) + n_redist * 8 * math.ceil(math.log2(nn)) + f_esc * 16
+    wk = 4 if precision == 1 else 8
+    per_redist = 8 if (precision == 1 and not survey) else 8 * math.ceil(math.log2(nn))
+    return 2 * S_STATE + n_step * (2 * wk + 8) + n_redist * per_redist + f_esc * 16
 
 
 def read_traffic(path, n_packets):
@@ -117,36 +139,29 @@ def main():
     if a.copy_calibration:
         copy_calibration()
         return
-    if a.nlte_only:
-        import numpy as np
-        import torch
-        from pubsed_amd import abi, workloads as W
-        from pubsed_amd.runtime import GpuBackend
-        gpu = GpuBackend(0)
-        gpu.use_torch_stream()
-        gpu.enable_timing(True)
-        print(json.dumps(bench_nlte(gpu, torch, W, abi, np)))
-        return
     import numpy as np
     import torch
     from pubsed_amd import runtime, abi, workloads as W
     from pubsed_amd.runtime import GpuBackend
-
-    dev = "cuda:0"
     gpu = GpuBackend(0)
     gpu.use_torch_stream()
     gpu.enable_timing(True)
-    N = a.packets or runtime.CONFIGS[a.config]["n_packets"]
-    if not runtime.CONFIGS[a.config]["steady"]:
-        print(json.dumps(bench_time_dependent(a, gpu)))
+    if a.nlte_only:
+        print(json.dumps(bench_nlte(gpu, torch, W, abi, np)))
         return
+    if not runtime.CONFIGS[a.config]["steady"]:
+        print(json.dumps(bench_time_dependent(a, gpu, a.steps, a.warmup, a.packets)))
+        return
+
+    dev = "cuda:0"
+    N = a.packets or runtime.CONFIGS[a.config]["n_packets"]
     P = runtime.Problem(runtime.Space(dev), a.config)
     P.alloc_packets(N)
     cnt = torch.zeros(10, dtype=torch.int64, device=dev)
 
     # every step starts from the same gas temperatures: a step's radiative-
     # equilibrium update is computed (and timed) but not carried into the
-    # next step, so that all timed steps (and both forms) run one workload
+    # next step, so that all timed steps (and every form) run one workload
     T0 = P.a["T"].clone()
     radeq = P.cfg["radiative_eq"]
 
@@ -177,18 +192,20 @@ def main():
         c = dict(zip(abi.COUNTER_FIELDS, cnt.cpu().tolist()))
         return dt, prop_ms, c, op_ms
 
-    print("# bench: config %s, %d packets/step, mixed precision" % (a.config, N), file=sys.stderr, flush=True)
+    log("config %s, %d packets/step, mixed precision: %d warm-up + %d timed steps" %
+        (a.config, N, a.warmup, a.steps))
     dt, prop_ms, c, op_ms = timed(abi.PREC_MIXED, N, a.steps, a.warmup)
     value = a.steps * N / dt
     ms_step = 1e3 * dt / a.steps
     prop_avg = float(np.mean(prop_ms))
     bpk = packet_bytes(c, P.n_nu, 1)
+    bpk_survey = packet_bytes(c, P.n_nu, 1, survey=True)
     achieved = bpk * N / (prop_avg * 1e-3) / 1e9
     per_pkt = {k: c[k] / max(c["n_packets"], 1) for k in ("n_events", "n_binshift", "n_redist",
                                                              "n_escaped", "n_scatter")}
-    print("# mixed: %.3f s/step, propagate %.1f ms, events/packet %.1f" %
-          (dt / a.steps, prop_avg, per_pkt["n_events"]), file=sys.stderr, flush=True)
-
+    events_per_s = c["n_events"] / (np.sum(prop_ms) * 1e-3)
+    log("mixed: %.3f s/step, propagate %.1f ms, events/packet %.1f, %.3g events/s" %
+        (dt / a.steps, prop_avg, per_pkt["n_events"], events_per_s))
     traffic, tinfo = read_traffic(a.traffic_from, N)
     traffic_src = None
     if tinfo:
@@ -196,33 +213,26 @@ def main():
             os.path.relpath(a.traffic_from, ROOT), tinfo["packets"], N)
     op_avg = float(np.mean(op_ms))
     n_lines = P.cfg.get("n_lines", 0) if P.cfg.get("fuzz") else 0
-    # K1-K3 algorithmic bytes per zone: abs, scat and emissivity CDF written
-    # (3 nu 8 B), the B_nu / means pass over them (nu 8 B), and the zone's
-    # full line list streamed once (32 B per line: nu, gf, E_l, ion + bin)
-    b_zone = 3 * P.n_nu * 8 + P.n_nu * 8 + 32 * n_lines
+    # K1-K3 per zone: abs, scat and emissivity CDF written (3 nu 8 B) and the
+    # B_nu / means pass over them (nu 8 B); the line list is read through L2
+    # (3.2 MB, shared by all zones), so its bytes are not charged per zone.
+    # The bound is FP64 exp: 3 per (zone, line)
+    b_zone = 4 * P.n_nu * 8
     opacity = {"kernel_ms": op_avg, "zones": P.nz, "lines": n_lines,
                "zone_builds_per_s": P.nz / (op_avg * 1e-3) if op_avg > 0 else None,
                "line_exp_per_s": 3.0 * n_lines * P.nz / (op_avg * 1e-3) if op_avg > 0 else None,
-               "bytes_per_zone": b_zone,
-               "achieved_GBps": b_zone * P.nz / (op_avg * 1e-3) / 1e9 if op_avg > 0 else None}
-
-    # plain FP64 drop-in (its rate is linear in the packet count at these sizes)
-    Np = min(a.plain_packets or N, N)
-    print("# plain FP64: %d packets" % Np, file=sys.stderr, flush=True)
-    dtp, prop_p, cp, _ = timed(abi.PREC_FP64, Np, 1, 0)
-    plain = Np / dtp
-    esc_m = c["n_escaped"] / max(c["n_packets"], 1)
-    esc_p = cp["n_escaped"] / max(cp["n_packets"], 1)
-
-    nlte = None
-    if not a.no_nlte:
-        nlte = bench_nlte(gpu, torch, W, abi, np)
-        nlte["opacity_build_nlte"] = bench_nlte_state(gpu, P, torch, abi, np)
-
-    cpu = None
-    if not a.no_cpu:
-        cpu = bench_cpu(a.config, a.cpu_seconds)
-
+               "table_bytes_per_zone": b_zone}
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "traffic_over_algorithmic": (traffic / (bpk * N)) if traffic else None,
+                "kernel": PROP_KERNEL % 1, "kernel_ms": prop_avg,
+                "bytes_per_packet": bpk,
+                "bytes_per_packet_note": "2x96 B record + per event 8 B (kappa, eps) + 8 B J_nu "
+                                         "+ 8 B alias entry per redistribution + 16 B per escape",
+                "survey_formula_bytes_per_packet": bpk_survey,
+                "survey_formula_frac": bpk_survey * N / (prop_avg * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "events_per_s": events_per_s,
+                "traffic_source": traffic_src}
     out = {
         "metric": "packets/s & NLTE solves/s: batched/mixed vs plain GPU path at matched MC noise",
         "value": value,
@@ -239,18 +249,57 @@ def main():
         "config": {"workload": "line-expansion 1D ejecta: 128 zones, 1e5 fuzz lines, 4 ion stages, "
                                "4096 log bins, steady state",
                    "packets_per_step": N, "zones": P.nz, "nu_bins": P.n_nu},
-        "plain_fp64_packets_per_s": plain,
-        "mixed_over_plain": value / plain,
-        "escape_fraction": {"mixed": esc_m, "plain": esc_p},
         "per_packet": per_pkt,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": PROP_KERNEL % 1, "kernel_ms": prop_avg,
-                     "bytes_per_packet": bpk, "traffic_source": traffic_src},
+        "roofline": roofline,
         "opacity_build": opacity,
-        "nlte": nlte,
-        "cpu_baseline": cpu,
     }
+    if a.mixed_only:
+        print(json.dumps(out))
+        return
+
+    if not a.no_nlte:
+        log("NLTE batched solves (C3)")
+        nlte = bench_nlte(gpu, torch, W, abi, np)
+        log("NLTE-coupled opacity build")
+        nlte["opacity_build_nlte"] = bench_nlte_state(gpu, P, torch, abi, np)
+        out["nlte"] = nlte
+    if not a.no_extra:
+        log("2D time-dependent outflow (C4)")
+        out["c4_2d"] = bench_time_dependent(argparse.Namespace(config="c4_2d"), gpu, 2, 1, None, brief=True)
+        log("differentiable transport (C5)")
+        out["c5_adjoint"] = bench_adjoint(gpu, torch, np)
+    if not a.no_cpu:
+        log("CPU oracle")
+        out["cpu_baseline"] = bench_cpu(a.config, a.cpu_seconds)
+
+    # FP64 comparison legs, the plain drop-in first: each is calibrated on
+    # 2e5 packets, then run at the full packet count when its
+    # estimated time fits the remaining budget (the plain leg keeps room for
+    # the batched FP64 leg at 1e6 packets), else at the largest count that
+    # does; both rates are linear in the count from about 4e5 packets up
+    if not a.no_fp64:
+        esc_m = c["n_escaped"] / max(c["n_packets"], 1)
+        n_cal = min(N, 200_000)
+        rates = {}
+        for key, prec in (("plain_fp64", abi.PREC_FP64), ("fp64_batched", abi.PREC_FP64_BATCHED)):
+            dtc, _, _, _ = timed(prec, n_cal, 1, 0)
+            rates[key] = n_cal / dtc
+        for key, prec in (("plain_fp64", abi.PREC_FP64), ("fp64_batched", abi.PREC_FP64_BATCHED)):
+            left = a.budget_s - elapsed() - 15.0
+            if key == "plain_fp64":
+                left -= min(N, 1_000_000) / rates["fp64_batched"]
+            n_leg = N if N / rates[key] <= left else max(int(rates[key] * left), n_cal)
+            log("%s: %d packets (estimated %.0f s, %.0f s left)" % (key, n_leg, n_leg / rates[key], left))
+            dtp, prop_p, cp, _ = timed(prec, n_leg, 1, 0)
+            out[key] = {"packets": n_leg, "packets_per_s": n_leg / dtp, "propagate_ms": prop_p[0],
+                        "escape_fraction": cp["n_escaped"] / max(cp["n_packets"], 1),
+                        "events_per_packet": cp["n_events"] / max(cp["n_packets"], 1),
+                        "mixed_over_this": value / (n_leg / dtp)}
+        out["escape_fraction_mixed"] = esc_m
+        out["plain_packets"] = out["plain_fp64"]["packets"]
+        out["plain_fp64_packets_per_s"] = out["plain_fp64"]["packets_per_s"]
+        out["mixed_over_plain"] = value / out["plain_fp64_packets_per_s"]
+    log("done")
     print(json.dumps(out))
 
 
@@ -276,7 +325,7 @@ def bench_nlte_state(gpu, P, torch, abi, np):
     return out
 
 
-def bench_time_dependent(a, gpu):
+def bench_time_dependent(a, gpu, steps, warmup, packets, brief=False):
     """Time-dependent configuration (the 2D ellipsoidal outflow): every step
     is the homologous hydro step, opacity build, wipe, emission of N new
     packets after the census, propagation to t + dt, compaction and
@@ -298,7 +347,7 @@ def bench_time_dependent(a, gpu):
         P.tparams.precision = precision
         for k in range(warm):
             runtime.time_step(gpu, P, dt, n_emit, scr, k)
-            print("# warm-up step %d: %d packets on the grid" % (k, P.n_packets), file=sys.stderr, flush=True)
+            log("warm-up step %d: %d packets on the grid" % (k, P.n_packets))
         torch.cuda.synchronize()
         cnt.zero_()
         n_prop, prop_ms, op_ms = 0, [], []
@@ -313,48 +362,58 @@ def bench_time_dependent(a, gpu):
         c = dict(zip(abi.COUNTER_FIELDS, cnt.cpu().tolist()))
         return P, el, n_prop, prop_ms, op_ms, c
 
-    N = a.packets or cfg["n_packets"]
-    print("# bench: config %s, %d new packets per step, dt %.3g s, mixed precision" % (a.config, N, dt),
-          file=sys.stderr, flush=True)
-    P, el, n_prop, prop_ms, op_ms, c = run(abi.PREC_MIXED, N, a.steps, a.warmup)
+    N = packets or cfg["n_packets"]
+    log("config %s, %d new packets per step, dt %.3g s, mixed precision" % (a.config, N, dt))
+    P, el, n_prop, prop_ms, op_ms, c = run(abi.PREC_MIXED, N, steps, warmup)
     value = n_prop / el
     prop_avg = float(np.mean(prop_ms))
     bpk = packet_bytes(c, P.n_nu, 1)
-    achieved = bpk * (n_prop / a.steps) / (prop_avg * 1e-3) / 1e9
-    print("# mixed: %.3f s/step, propagate %.1f ms, %d packets propagated" % (el / a.steps, prop_avg, n_prop),
-          file=sys.stderr, flush=True)
-    Np = min(a.plain_packets or N, N)
-    print("# plain FP64: %d new packets per step" % Np, file=sys.stderr, flush=True)
-    _, elp, n_prop_p, _, _, cp = run(abi.PREC_FP64, Np, 1, a.warmup)
-    plain = n_prop_p / elp
+    achieved = bpk * (n_prop / steps) / (prop_avg * 1e-3) / 1e9
+    log("mixed: %.3f s/step, propagate %.1f ms, %d packets propagated" % (el / steps, prop_avg, n_prop))
     per = {k: c[k] / max(c["n_packets"], 1) for k in ("n_events", "n_binshift", "n_redist", "n_escaped",
                                                        "n_stopped", "n_scatter")}
-    return {
+    spec = P.get("opt_click")
+    res = {
         "metric": "packets/s & NLTE solves/s: batched/mixed vs plain GPU path at matched MC noise",
-        "value": value, "unit": "packets/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": 1e3 * el / a.steps, "higher_is_better": True, "scaling": "weak",
+        "value": value, "unit": "packets/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": 1e3 * el / steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 geometry / f64 tallies",
         "data": "synthetic (seeded ellipsoidal ejecta and line list)",
         "config": {"workload": "2D axisymmetric ellipsoidal outflow: 128 x 256 (varpi, z) zones, axis ratio "
-                               "2, 1e5 fuzz lines, 4096 log bins, time-dependent with census",
+                               "2, 1e5 fuzz lines, 4096 log bins, time-dependent with census, escape "
+                               "spectrum in %d time x %d nu x %d mu bins" % P.spec_shape[:3],
                    "new_packets_per_step": N, "dt_s": dt, "zones": P.nz, "nu_bins": P.n_nu,
                    "packets_on_grid_after": P.n_packets},
-        "plain_fp64_packets_per_s": plain, "plain_new_packets_per_step": Np,
-        "mixed_over_plain": value / plain, "per_packet": per,
+        "per_packet": per, "spectrum_packets": float(spec.sum()),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": PROP_KERNEL % P.grid.kind, "kernel_ms": prop_avg,
                      "bytes_per_packet": bpk},
         "opacity_build": {"kernel_ms": float(np.mean(op_ms)), "zones": P.nz,
                           "lines": P.cfg.get("n_lines", 0)},
-        "nlte": None, "cpu_baseline": None,
     }
+    if brief:
+        for k in ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline", "data"):
+            res.pop(k)
+        return res
+    Np = min(N, 10_000_000)
+    log("plain FP64: %d new packets per step" % Np)
+    _, elp, n_prop_p, _, _, cp = run(abi.PREC_FP64, Np, 1, warmup)
+    res["plain_fp64_packets_per_s"] = n_prop_p / elp
+    res["plain_new_packets_per_step"] = Np
+    res["mixed_over_plain"] = value / res["plain_fp64_packets_per_s"]
+    res["nlte"] = None
+    res["cpu_baseline"] = None
+    return res
 
 
 def bench_nlte(gpu, torch, W, abi, np):
     """Batched NLTE (256 zones x 4 ions = 1024 systems, n ~ U{20..200}):
-    FP32 LU + FP64 refinement in one persistent launch vs the per-system
-    FP64 drop-in (one sgpu_nlte_solve per system)."""
+    FP32 LU + FP64 refinement in one persistent launch and the batched FP64
+    LU, against the per-system FP64 drop-in (one sgpu_nlte_solve per system)
+    and against the CPU oracle's FP64 LU (normwise max |b - b_oracle| /
+    max |b_oracle| over the systems)."""
+    from oracle.oracle_py import OracleBackend
     sy = W.nlte_systems(n_sys=1024, seed=20261203)
     d = {k: torch.from_numpy(np.ascontiguousarray(sy[k])).cuda() for k in
          ("n", "rate_off", "vec_off", "rates", "lte")}
@@ -373,12 +432,17 @@ def bench_nlte(gpu, torch, W, abi, np):
     torch.cuda.synchronize()
     t_mixed = (time.perf_counter() - t0) / reps
     k_ms = gpu.last_ms("nlte")
-    # batched FP64 (same launch, FP64 LU in device memory)
-    t0 = time.perf_counter()
+    # batched FP64 (same launch, FP64 LU)
     gpu.nlte_batched(sy["n_sys"], d["n"], d["rate_off"], d["vec_off"], d["rates"], d["lte"], b64, None,
                      abi.NLTE_FP64, 0.0, None)
     torch.cuda.synchronize()
-    t_b64 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        gpu.nlte_batched(sy["n_sys"], d["n"], d["rate_off"], d["vec_off"], d["rates"], d["lte"], b64, None,
+                         abi.NLTE_FP64, 0.0, None)
+    torch.cuda.synchronize()
+    t_b64 = (time.perf_counter() - t0) / reps
+    k64_ms = gpu.last_ms("nlte")
     # plain drop-in: one call per system
     work = torch.zeros(200 * 200 + 200, dtype=torch.float64, device="cuda:0")
     bp = torch.zeros(nv, dtype=torch.float64, device="cuda:0")
@@ -388,10 +452,22 @@ def bench_nlte(gpu, torch, W, abi, np):
         gpu.nlte_solve(int(ns[s]), d["rates"][ro[s]:], d["lte"][vo[s]:], bp[vo[s]:], None, work)
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
-    x, r = b.cpu().numpy(), bp.cpu().numpy()
-    err = max(np.max(np.abs(x[vo[s]:vo[s + 1]] - r[vo[s]:vo[s + 1]])) / np.max(np.abs(r[vo[s]:vo[s + 1]]))
-              for s in range(sy["n_sys"]))
-    n2 = (ns.astype(np.float64))
+    # the CPU oracle's FP64 LU of every system
+    ob = OracleBackend()
+    ref = np.zeros(nv)
+    for s in range(sy["n_sys"]):
+        m = int(ns[s])
+        out = np.zeros(m)
+        ob.nlte_solve(m, np.ascontiguousarray(sy["rates"][ro[s]:ro[s + 1]]),
+                      np.ascontiguousarray(sy["lte"][vo[s]:vo[s + 1]]), out)
+        ref[vo[s]:vo[s + 1]] = out
+
+    def err(x):
+        return float(max(np.max(np.abs(x[vo[s]:vo[s + 1]] - ref[vo[s]:vo[s + 1]])) /
+                         np.max(np.abs(ref[vo[s]:vo[s + 1]])) for s in range(sy["n_sys"])))
+
+    x, x64, xp = b.cpu().numpy(), b64.cpu().numpy(), bp.cpu().numpy()
+    n2 = ns.astype(np.float64)
     flops = float(np.sum(2.0 / 3.0 * n2 ** 3))
     itv = it.cpu().numpy()
     return {"systems": int(sy["n_sys"]), "n_range": [int(ns.min()), int(ns.max())],
@@ -399,18 +475,59 @@ def bench_nlte(gpu, torch, W, abi, np):
             "batched_fp64_solves_per_s": sy["n_sys"] / t_b64,
             "plain_fp64_solves_per_s": sy["n_sys"] / t_plain,
             "mixed_over_plain": t_plain / t_mixed,
-            "max_rel_err_vs_fp64": float(err),
+            "max_rel_err_vs_oracle": {"batched_mixed": err(x), "batched_fp64": err(x64), "plain_fp64": err(xp)},
             "refine_iters": {"min": int(itv.min()), "max": int(itv.max()),
                              "fp64_fallbacks": int((itv < 0).sum())},
-            "kernel_ms": k_ms,
+            "kernel_ms": k_ms, "kernel_fp64_ms": k64_ms,
             "lu_tflops": flops / (k_ms * 1e-3) / 1e12 if k_ms > 0 else None,
             "autotune": rep.strip().splitlines()[-1] if rep else None}
 
 
+def bench_adjoint(gpu, torch, np):
+    """Differentiable transport at 1e7 packets on 128 zones x 1024 bins of
+    absorption opacity: forward spectrum (recording each packet's escape bin
+    and weight) and the adjoint replay, timed; the adjoint gradient of
+    L = sum (F - F_target)^2 along two random directions against central
+    finite differences with common random numbers (relative step 1e-3)."""
+    from pubsed_amd.diff import DiffTransport
+    n = 10_000_000
+    m = DiffTransport(gpu, n_packets=n)
+    kappa = m.kappa_init()
+    rng = np.random.default_rng(11)
+    noise = torch.from_numpy((1 + 0.1 * rng.uniform(-1, 1, kappa.shape)).astype(np.float32)).cuda()
+    F_target = m.forward(kappa * noise, record=False)
+    m.forward(kappa)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    F = m.forward(kappa)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    dLdF = 2 * (F - F_target)
+    t0 = time.perf_counter()
+    g = m.backward(kappa, dLdF)
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    L = lambda k: float(((m.forward(k, record=False) - F_target) ** 2).sum())
+    errs = []
+    for s in range(2):
+        v = torch.from_numpy(np.random.default_rng(100 + s).standard_normal(kappa.shape).astype(np.float32)).cuda()
+        h = 1e-3
+        kp, km = kappa * (1 + h * v), kappa * (1 - h * v)
+        fd = (L(kp) - L(km)) / 2
+        ad = float((g * (kappa * v).double()).sum()) * h
+        errs.append(abs(ad - fd) / abs(fd))
+    return {"packets": n, "params": list(kappa.shape), "forward_ms": 1e3 * t_fwd, "adjoint_ms": 1e3 * t_bwd,
+            "forward_packets_per_s": n / t_fwd, "adjoint_packets_per_s": n / t_bwd,
+            "grad_rel_err_vs_central_fd": errs}
+
+
 def bench_cpu(config, seconds):
     """The CPU oracle (FP64 restatement with the reference loop structure:
-    OpenMP over packets, atomic tallies) on this host's cores, on a bounded
-    packet sample scaled to about `seconds` of work."""
+    OpenMP over packets, atomic tallies) on this host's cores and on one
+    thread, phase by phase (opacity build, emission, propagation,
+    normalisation, equilibrium temperature) as the reference's step prints
+    them, on a bounded packet sample scaled to about `seconds` of
+    propagation per thread count."""
     from pubsed_amd import runtime
     from oracle.oracle_py import OracleBackend
     cores = os.cpu_count() or 1
@@ -420,32 +537,48 @@ def bench_cpu(config, seconds):
         pass
     if os.environ.get("OMP_NUM_THREADS", "").isdigit():
         cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
-    ob = OracleBackend(n_threads=cores)
-    P = runtime.Problem(runtime.Space(None), config)
-    ob.opacity_build(P)
-    T0 = P.get("T").copy()
-    n = 256
-    rate = None
-    while True:
-        P.alloc_packets(n)
-        P.a["T"][:] = T0
-        ob.wipe(P)
-        t0 = time.perf_counter()
-        ob.opacity_build(P)
-        ob.emit(P, n)
-        ob.propagate(P, n)
-        ob.normalize(P)
-        if P.cfg["radiative_eq"]:
-            ob.solve_eq_temperature(P)
-        dt = time.perf_counter() - t0
-        rate = n / dt
-        if dt > seconds / 4 or n >= 1 << 22:
-            break
-        n = int(min(n * max(2.0, seconds / max(dt, 1e-3) / 2), 1 << 22))
-    return {"value": rate, "unit": "packets/s", "cores": cores, "kind": "port",
-            "sample": "%d packets of the same workload, one full step (opacity, emission, "
-                      "propagation, normalisation, equilibrium temperature), FP64 oracle, OpenMP "
-                      "over packets" % n}
+    res = {}
+    for label, threads in (("all_cores", cores), ("one_thread", 1)):
+        ob = OracleBackend(n_threads=threads)
+        P = runtime.Problem(runtime.Space(None), config)
+        T0 = P.get("T").copy()
+        n = 64
+        while True:
+            P.alloc_packets(n)
+            P.a["T"][:] = T0
+            ob.wipe(P)
+            ph = {}
+            t = time.perf_counter()
+            ob.opacity_build(P)
+            ph["opacity"] = time.perf_counter() - t
+            t = time.perf_counter()
+            ob.emit(P, n)
+            ph["emission"] = time.perf_counter() - t
+            t = time.perf_counter()
+            ob.propagate(P, n)
+            ph["propagation"] = time.perf_counter() - t
+            t = time.perf_counter()
+            ob.normalize(P)
+            ph["normalisation"] = time.perf_counter() - t
+            t = time.perf_counter()
+            if P.cfg["radiative_eq"]:
+                ob.solve_eq_temperature(P)
+            ph["equilibrium_temperature"] = time.perf_counter() - t
+            tot = sum(ph.values())
+            if ph["propagation"] > seconds / 3 or n >= 1 << 22:
+                break
+            n = int(min(n * max(2.0, seconds / 2 / max(ph["propagation"], 1e-3)), 1 << 22))
+        res[label] = {"threads": threads, "packets": n, "packets_per_s": n / tot,
+                      "phase_ms": {k: 1e3 * v for k, v in ph.items()},
+                      "propagation_packets_per_s": n / ph["propagation"]}
+    a = res["all_cores"]
+    return {"value": a["packets_per_s"], "unit": "packets/s", "cores": a["threads"], "kind": "port",
+            "sample": "%d packets (all cores) / %d packets (1 thread) of the same workload, one full "
+                      "step each (opacity, emission, propagation, normalisation, equilibrium "
+                      "temperature), FP64 oracle, OpenMP over packets"
+                      % (a["packets"], res["one_thread"]["packets"]),
+            "one_thread_packets_per_s": res["one_thread"]["packets_per_s"],
+            "phases": res}
 
 
 if __name__ == "__main__":

@@ -80,8 +80,14 @@ extern "C" {
 #define SGPU_GRID_2D_CYLN   2
 
 /* propagation precision */
-#define SGPU_PREC_FP64  0   /* plain drop-in: FP64 everywhere            */
+#define SGPU_PREC_FP64  0   /* plain drop-in: FP64 everywhere, one thread per
+                               packet, per-event tally atomics             */
 #define SGPU_PREC_MIXED 1   /* FP32 geometry/opacity, FP64 tallies        */
+#define SGPU_PREC_FP64_BATCHED 2  /* the FP64 arithmetic of SGPU_PREC_FP64 (the
+                               same packet paths), launched as the mixed form
+                               is: persistent lanes refilling from the packet
+                               list, zone tallies summed in registers over a
+                               zone visit into LDS, J_nu per (zone, bin) run */
 
 /* NLTE precision */
 #define SGPU_NLTE_FP64        0   /* FP64 partial-pivot LU                 */

@@ -22,7 +22,7 @@ PHOTON, GAMMARAY = 0, 1
 MOVING, STOPPED, ESCAPED, ABSORBED = 0, 1, 2, 3
 LOC_FLEX, LOC_LIN, LOC_LOG, LOC_NONE = 0, 1, 2, 3
 GRID_1D_SPHERE, GRID_2D_CYLN = 1, 2
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_FP64_BATCHED = 0, 1, 2
 NLTE_FP64, NLTE_FP32_REFINE = 0, 1
 
 

@@ -1,19 +1,23 @@
 // transport.hip -- K4/K5/K6/K7: packet emission, propagation, compaction,
 // tally wipe / normalisation and the homologous expansion step.
 //
-// Propagation (K4) runs one packet per lane.  Two forms share the per-event
-// code (sgpu_geometry.h):
+// Propagation (K4) runs one packet per lane.  Three forms share the
+// per-event code (sgpu_geometry.h):
 //   SGPU_PREC_FP64   the plain drop-in: one thread per packet, FP64 geometry,
 //                    every tally a global FP64 atomic, as the reference's
 //                    omp-atomic loop (src/transport/transport.cpp:57-88,
 //                    260-484).
+//   SGPU_PREC_FP64_BATCHED  the same FP64 arithmetic (the same packet
+//                    paths), launched as the mixed form: persistent lanes,
+//                    zone tallies summed in registers over a zone visit and
+//                    flushed to LDS, J_nu summed per (zone, bin) run.
 //   SGPU_PREC_MIXED  persistent lanes that refill themselves with the next
 //                    packet when theirs finishes (no lane idles on a
 //                    long-lived neighbour), FP32 geometry and Doppler shifts,
 //                    packed FP32 (kappa, eps) opacity table (one 8-byte
-//                    gather per event), FP32 emissivity CDF, zone tallies
-//                    accumulated in LDS per workgroup and flushed once,
-//                    J_nu and spectra as FP64 global atomics.
+//                    gather per event), Walker alias tables for the
+//                    emissivity, zone tallies summed in registers per zone
+//                    visit into LDS, J_nu and spectra as FP64 global atomics.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstdlib>
@@ -1489,17 +1493,36 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
                tables->photoion_opac && (tables->scat || params->omit_scattering), "opacity tables missing");
   SGPU_REQUIRE(tallies->e_abs && tallies->J_nu && tallies->fx_rad && tallies->fy_rad && tallies->fz_rad &&
                tallies->fr_rad && tallies->L_radio_dep, "tally arrays missing");
-  SGPU_REQUIRE(params->precision == SGPU_PREC_FP64 || params->precision == SGPU_PREC_MIXED,
-               "unknown precision %d", params->precision);
+  SGPU_REQUIRE(params->precision == SGPU_PREC_FP64 || params->precision == SGPU_PREC_MIXED ||
+               params->precision == SGPU_PREC_FP64_BATCHED, "unknown precision %d", params->precision);
   SGPU_REQUIRE(params->use_ddmc == 0 || params->use_ddmc == 2,
                "transport_use_ddmc = %d: only 0 (off) and 2 (DDMC) are supported", params->use_ddmc);
   if (params->use_ddmc == 2)
-    SGPU_REQUIRE(grid->kind == SGPU_GRID_1D_SPHERE && params->precision == SGPU_PREC_FP64 &&
+    SGPU_REQUIRE(grid->kind == SGPU_GRID_1D_SPHERE && params->precision != SGPU_PREC_MIXED &&
                  params->ddmc_tau_threshold >= 0,
                  "DDMC runs on 1D spherical grids in the FP64 form with a threshold >= 0");
   if (n == 0) return SGPU_OK;
   const int nz = grid->n_zones, nn = nu_grid->n;
-  if (params->precision == SGPU_PREC_FP64) {
+  if (params->precision == SGPU_PREC_FP64_BATCHED && params->use_ddmc != 2) {
+    PropArgs<double> A;
+    A.g = make_grid<double>(*grid, grid->r_out.x, grid->x_out.x, grid->z_out.x);
+    A.nu = make_axis<double>(*nu_grid, nu_grid->x);
+    A.tab = {tables->abs, tables->scat, tables->emis_cdf};
+    A.eps_imc = tables->eps_imc; A.compton = tables->compton_opac; A.photoion = tables->photoion_opac;
+    A.p = *params; A.pk = *packets; A.n = n; A.ta = *tallies; A.cnt = counters;
+    A.lds_tallies = nz <= LDS_ZONE_MAX;
+    const size_t lds = A.lds_tallies ? sizeof(double) * T_NFIELDS * nz : 0;
+    const char* ew = getenv("SGPU_PROP_WG_PER_CU");
+    const int wg_per_cu = ew ? std::max(1, atoi(ew)) : 12;
+    const int64_t want = (int64_t)ctx->n_cu * wg_per_cu, need = grid_blocks(n, PROP_THREADS);
+    const dim3 gb((unsigned)(want < need ? want : need)), tb(PROP_THREADS);
+    timer_begin(ctx, "propagate");
+    if (grid->kind == SGPU_GRID_1D_SPHERE)
+      hipLaunchKernelGGL((propagate_kernel<double, true, true, 1, SGPU_GRID_1D_SPHERE>), gb, tb, lds, ctx->stream, A);
+    else
+      hipLaunchKernelGGL((propagate_kernel<double, true, true, 1, SGPU_GRID_2D_CYLN>), gb, tb, lds, ctx->stream, A);
+    SGPU_LAUNCH_CHECK();
+  } else if (params->precision != SGPU_PREC_MIXED) {
     PropArgs<double> A;
     A.g = make_grid<double>(*grid, grid->r_out.x, grid->x_out.x, grid->z_out.x);
     A.nu = make_axis<double>(*nu_grid, nu_grid->x);

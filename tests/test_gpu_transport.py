@@ -63,15 +63,19 @@ def test_emission_matches_oracle(gpu, name):
         assert np.allclose(Pg.get(k), Ph.get(k), rtol=1e-12, atol=0), k
 
 
+@pytest.mark.parametrize("precision", [abi.PREC_FP64, abi.PREC_FP64_BATCHED])
 @pytest.mark.parametrize("name", ["c1_grey", "c2_line"])
-def test_propagate_fp64_matches_oracle(gpu, name):
+def test_propagate_fp64_matches_oracle(gpu, name, precision):
+    """The plain FP64 drop-in and its batched launch (persistent lanes,
+    register / LDS tally sums) follow the oracle packet for packet."""
     n = 20000 if name == "c1_grey" else 4000
     over = {"nz": 64} if name == "c2_line" else {}
     Pg, Ph, ob = setup_pair(name, n, gpu, **over)
+    Pg.tparams.precision = precision
     gpu.emit(Pg, n)
     ob.emit(Ph, n)
     cnt = Pg.space.zeros(10, "i64")
-    print("case propagate fp64", name, n, flush=True)
+    print("case propagate fp64", name, n, "precision", precision, flush=True)
     gpu.propagate(Pg, n, cnt)
     gpu.sync()
     ob.propagate(Ph, n)
