@@ -126,7 +126,9 @@ class Problem:
         S = space
         a = self.a = {}
         lo, hi, nbin = cfg["nu"]
-        self.nu_np = W.nu_log_grid(lo, hi, nbin)
+        # transport_nu_grid {nu_1, nu_2, del} with nu_1 == nu_2 (the default
+        # {1,1,1}) is locate_array's catch-all single bin: grey transport
+        self.nu_np = W.lin_init_n(lo, hi, 1) if lo == hi else W.nu_log_grid(lo, hi, nbin)
         self.nu, a["nu_x"] = locate_struct(S, self.nu_np)
         self.n_nu = nn = self.nu_np["n"]
         if cfg["grid"] == "1d":

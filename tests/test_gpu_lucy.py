@@ -1,0 +1,110 @@
+"""The Lucy (2005) grey supernova on the GPU, time-dependent, against the
+reference's own fixtures (tests/lucy_supernova/1D: the model of
+models/make_mod.py, the run of param.lua, the criteria of run_test.py):
+
+* model: 100 shells of uniform density, 1.4 Msun (m_sun = 1.99e33) to
+  v_max = 1e9 cm/s at t = 1 d, pure 56Ni inside 0.5 Msun tapering linearly
+  to none at 0.75 Msun, 28Si outside, T = 2e5 K; grey kappa = 0.1 cm^2/g,
+  epsilon = 1, radiative equilibrium, the default single catch-all
+  frequency bin (transport_nu_grid = {1,1,1});
+* steps of dt = min(0.5 d, 0.05 t) (tstep_max_dt, tstep_max_delta;
+  SedonaClass.cpp:321-328), the homologous step before transport, census
+  packets persisting, the escape spectrum in 0.5-day bins from -0.5 d;
+* light curve: the escaping luminosity per time bin against
+  comparefiles/lucy_lc.dat, and the gamma-ray + positron deposition
+  luminosity against comparefiles/lucy_gr.dat, over 3 d < t < 55 d: max
+  relative error 25 %, mean 10 % (run_test.py:41-47);
+* radiation temperature T_rad = (e_rad / a)^(1/4) after steps 15, 30 and
+  60 against comparefiles/plt_files/plt_000{15,30,60}.dat over
+  1e8 < v < 9.8e8 cm/s: max 50 %, mean 2 % (run_test.py:61-80; the .dat
+  twins of the .h5 files the script reads).
+
+The fixture files are copied unchanged under tests/golden/lucy/.  Run for
+the mixed form and the batched FP64 form; 3e4 radioactive packets per step
+(the reference's test uses 1e4).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from pubsed_amd import abi, runtime, workloads as W
+
+pytestmark = pytest.mark.gpu
+DAY = 86400.0
+A_RAD = 7.5657e-15
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "lucy")
+
+
+def lucy_model(P, t):
+    """tests/lucy_supernova/models/make_mod.py (1D part) at time t."""
+    nz = P.nz
+    vmax, msun, mass, pi = 1e9, 1.99e33, 1.4 * 1.99e33, 3.14159
+    v = vmax * np.arange(1, nz + 1) / nz
+    r_out = v * t
+    rho0 = mass / (4.0 * pi / 3.0 * (vmax * t) ** 3)
+    vol = 4 * pi / 3 * (r_out ** 3 - np.concatenate([[0.0], r_out[:-1]]) ** 3)
+    m_enc = 4.0 * pi / 3.0 * (v * t) ** 3 * rho0 / msun
+    ni = np.where(m_enc < 0.5, 1.0, np.where(m_enc < 0.75, (0.75 - m_enc) / 0.25, 0.0))
+    X = np.zeros((nz, 4))
+    X[:, 0] = ni          # elements of the grey configuration: 56Ni, 56Co, 56Fe, 28Si
+    X[:, 3] = 1 - ni
+    P.set("r_out", r_out)
+    P.set("vol", vol)
+    P.set("rho", np.full(nz, rho0))
+    P.set("X", X.ravel())
+    P.set("T", np.full(nz, 2.0e5))
+    P.grid.t_now = P.t_now = P.tparams.t_now = P.op_params.t_now = t
+    return v
+
+
+def rel_errors(x, y, xc, yc, lo, hi):
+    use = (x > lo) & (x < hi)
+    yi = np.interp(x[use], xc, yc)
+    e = np.abs(y[use] - yi) / yi
+    return float(e.max()), float(e.mean())
+
+
+@pytest.mark.parametrize("precision", [abi.PREC_MIXED, abi.PREC_FP64_BATCHED])
+def test_lucy_supernova_light_curve(gpu, precision):
+    n_emit = 30000
+    P = runtime.Problem(runtime.Space("cuda:0"), "c1_grey", nu=(1, 1, 1), grey=0.1, epsilon=1.0, steady=0,
+                        radiative_eq=1, spec_t0=-0.5 * DAY, spec_t1=100 * DAY, spec_dt=0.5 * DAY)
+    v = lucy_model(P, 1.0 * DAY)
+    P.tparams.precision = precision
+    cap = 150 * n_emit
+    P.alloc_packets(cap)
+    scr = P.alloc_packets(cap, name="scr")
+    t, k = 1.0 * DAY, 0
+    dep_t, dep_L, trad = [], [], {}
+    print("case lucy precision=%d" % precision, flush=True)
+    while t < 58 * DAY:
+        dt = min(0.5 * DAY, 0.05 * t)
+        got, alive, esc = runtime.time_step(gpu, P, dt, n_emit, scr, k)
+        assert got == n_emit and alive <= cap - n_emit
+        k += 1
+        t += dt
+        dep_t.append((t - 0.5 * dt) / DAY)
+        dep_L.append(float((P.get("L_radio_dep") * P.get("vol")).sum()))
+        if k in (15, 30, 60):
+            trad[k] = (P.get("e_rad") / A_RAD) ** 0.25
+    gpu.sync()
+    print("steps", k, "packets on the grid", P.n_packets, flush=True)
+    # escaping luminosity per 0.5-day bin of t_obs
+    tg = P.spec_grids["time"]
+    edges = np.concatenate([[tg["min"]], tg["x"]])
+    tc = 0.5 * (edges[1:] + edges[:-1]) / DAY
+    L = P.get("opt_flux").reshape(P.spec_shape).sum(axis=(1, 2, 3)) / np.diff(edges)
+    tl, Ll = np.loadtxt(os.path.join(GOLD, "lucy_lc.dat"), unpack=True)
+    mx, mn = rel_errors(tc, L, tl, Ll, 3, 55)
+    print("light curve: max %.3f mean %.3f" % (mx, mn))
+    assert mx < 0.25 and mn < 0.10, (mx, mn)
+    tg2, Lg = np.loadtxt(os.path.join(GOLD, "lucy_gr.dat"), unpack=True)
+    mx, mn = rel_errors(np.array(dep_t), np.array(dep_L), tg2, Lg, 3, 55)
+    print("deposition: max %.3f mean %.3f" % (mx, mn))
+    assert mx < 0.25 and mn < 0.10, (mx, mn)
+    for s, Tr in trad.items():
+        ref = np.loadtxt(os.path.join(GOLD, "plt_%05d.dat" % s))
+        mx, mn = rel_errors(v, Tr, ref[:, 2], ref[:, 4], 1e8, 9.8e8)
+        print("T_rad after step %d: max %.3f mean %.3f" % (s, mx, mn))
+        assert mx < 0.5 and mn < 0.02, (s, mx, mn)
