@@ -23,6 +23,8 @@
  *                                                             src/opacity/GasState_opacities.cpp:11-136,464-494,564-666
  *   sgpu_emit_radioactive   transport::emit_radioactive       src/transport/emission.cpp:165-243
  *   sgpu_initialize_particles transport::initialize_particles src/transport/emission.cpp:112-159
+ *   sgpu_emit_inner_source  transport::emit_inner_source      src/transport/emission.cpp:370-480
+ *                           (+ setup_core_emission            src/transport/transport_init.cpp:465-548)
  *   sgpu_propagate          the packet loop of transport::step src/transport/transport.cpp:57-88
  *                           + propagate / propagate_monte_carlo :202-484
  *   sgpu_compact            transport::clean_up_particle_vector src/transport/transport.cpp:164-194
@@ -375,6 +377,36 @@ int sgpu_emit_radioactive(sgpu_ctx* ctx,
                           int64_t n_emit, int32_t force_rproc, uint64_t id_base,
                           sgpu_packets* packets, int64_t offset,
                           sgpu_tallies* tallies, int64_t* n_out);
+
+/* Inner boundary ("core") source (transport::setup_core_emission,
+ * src/transport/transport_init.cpp:465-548; parameters core_radius,
+ * core_temperature, core_luminosity, core_photon_frequency). */
+typedef struct {
+  double r_core;           /* cm; 0: packets start at the centre, isotropic     */
+  double T_core;           /* K; 0 with L_core and r_core set: from
+                              L = 4 pi r^2 sigma T^4; still 0: flat spectrum    */
+  double L_core;           /* erg/s; 0: the blackbody's 4 pi^2 r^2 sum B_nu dnu */
+  double core_frequency;   /* Hz; > 0: every packet at this frequency          */
+} sgpu_core_source;
+
+/* transport::emit_inner_source (src/transport/emission.cpp:370-480): n_emit
+ * photon packets into packets[offset .. offset+n_emit), stream ids
+ * id_base + q, on the sphere r_core (1 + 1e-10) at a uniform point with a
+ * Lambertian (cos theta) outward direction -- or at the centre, isotropic,
+ * for r_core = 0 -- energy L_core dt / n_emit, frequency from the core's
+ * spectrum over the nu grid (B_nu(T_core) dnu at the bin centres, or
+ * core_frequency), time t_now + u dt, Lorentz-transformed from the comoving
+ * to the lab frame; zone from the grid.  As the reference, nothing is done
+ * (*n_out = 0) when the packets do not fit the buffer.  The core spectrum
+ * read from a file (core_spectrum_file) is not supported. */
+int sgpu_emit_inner_source(sgpu_ctx* ctx,
+                           const sgpu_grid* grid,
+                           const sgpu_locate* nu_grid,
+                           const sgpu_core_source* core,
+                           const sgpu_transport_params* params,
+                           int64_t n_emit, uint64_t id_base,
+                           sgpu_packets* packets, int64_t offset,
+                           int64_t* n_out);
 
 /* Initial radiation field (transport::initialize_particles,
  * src/transport/emission.cpp:112-159, with create_isotropic_particle

@@ -108,6 +108,10 @@ int      or_initialize_particles(const sgpu_grid* grid, const sgpu_locate* nu_gr
                                  const sgpu_zone_state* zs, const double* e_rad, double* emis_cdf,
                                  const sgpu_transport_params* p, int64_t n_init, uint64_t id_base,
                                  sgpu_packets* pk, int64_t offset, int64_t* n_out);
+int      or_emit_inner_source(const sgpu_grid* grid, const sgpu_locate* nu_grid,
+                              const sgpu_core_source* core, const sgpu_transport_params* p,
+                              int64_t n_emit, uint64_t id_base, sgpu_packets* pk, int64_t offset,
+                              int64_t* n_out);
 int      or_propagate(const sgpu_grid* grid, const sgpu_locate* nu_grid,
                       const sgpu_tables* tab, const sgpu_transport_params* p,
                       sgpu_packets* pk, int64_t n, sgpu_tallies* ta,

@@ -61,6 +61,10 @@ def load():
         lib.or_initialize_particles.argtypes = [P(abi.Grid), P(abi.Locate), P(abi.ZoneState), V, V,
                                                 P(abi.TransportParams), I64, ctypes.c_uint64,
                                                 P(abi.Packets), I64, P(I64)]
+        lib.or_emit_inner_source.restype = I
+        lib.or_emit_inner_source.argtypes = [P(abi.Grid), P(abi.Locate), P(abi.CoreSource),
+                                             P(abi.TransportParams), I64, ctypes.c_uint64, P(abi.Packets),
+                                             I64, P(I64)]
         lib.or_propagate.restype = I
         lib.or_propagate.argtypes = [P(abi.Grid), P(abi.Locate), P(abi.Tables),
                                      P(abi.TransportParams), P(abi.Packets), I64, P(abi.Tallies),
@@ -142,6 +146,13 @@ class OracleBackend:
                                      ctypes.byref(P.tables), ctypes.byref(P.tparams), n_emit,
                                      force_rproc, id_base, ctypes.byref(P.packets), offset,
                                      ctypes.byref(P.tallies), ctypes.byref(nout))
+        return nout.value
+
+    def emit_inner_source(self, P, core, n_emit, id_base=0, offset=0):
+        nout = ctypes.c_int64(0)
+        self.lib.or_emit_inner_source(ctypes.byref(P.grid), ctypes.byref(P.nu), ctypes.byref(core),
+                                      ctypes.byref(P.tparams), n_emit, id_base, ctypes.byref(P.packets),
+                                      offset, ctypes.byref(nout))
         return nout.value
 
     def initialize_particles(self, P, n_init, id_base=0, offset=0):

@@ -878,6 +878,78 @@ int or_initialize_particles(const sgpu_grid* grid, const sgpu_locate* nu_grid,
   return 0;
 }
 
+/* transport::setup_core_emission (transport_init.cpp:465-548) and
+ * transport::emit_inner_source (emission.cpp:370-480).  The core spectrum
+ * is B_nu(T_core) dnu at the bin centres (1 dnu for T_core <= 0), L_core
+ * defaults to its 4 pi r^2 pi sum; every emissivity weight is 1
+ * (transport_init.cpp:321-332).  Packet q draws, in order: for r_core > 0
+ * phi_core, cos theta_core, phi_loc, cos theta_loc = sqrt(u); for r_core = 0
+ * mu, phi; then (without core_frequency) 2 frequency draws; then the time. */
+int or_emit_inner_source(const sgpu_grid* grid, const sgpu_locate* nu_grid, const sgpu_core_source* core,
+                         const sgpu_transport_params* prm, int64_t n_emit, uint64_t id_base,
+                         sgpu_packets* pk, int64_t offset, int64_t* n_out)
+{
+  *n_out = 0;
+  if (n_emit <= 0 || offset + n_emit > pk->capacity) return 0;
+  int nn = nu_grid->n;
+  double r_core = core->r_core, T_core = core->T_core, L_core = core->L_core;
+  if (L_core != 0 && r_core != 0 && T_core == 0) T_core = pow(L_core / (4.0 * OR_PI * r_core * r_core * OR_SB), 0.25);
+  double* cdf = (double*)malloc(sizeof(double) * nn);
+  double L_sum = 0;
+  for (int j = 0; j < nn; j++) {
+    double nu = or_center(nu_grid, j), dnu = or_delta(nu_grid, j);
+    double bb = T_core <= 0 ? 1 : or_blackbody_nu(T_core, nu);
+    cdf[j] = (j == 0 ? 0 : cdf[j - 1]) + bb * dnu;
+    L_sum += 4.0 * OR_PI * r_core * r_core * OR_PI * bb * dnu;
+  }
+  or_cdf_normalize(cdf, nn);
+  if (L_core == 0) L_core = L_sum;
+  double Ep = L_core * prm->dt / n_emit;
+  tctx_t c = {grid, nu_grid, 0, prm, 0, 0, 0, 0, 0, 0};
+  for (int64_t q = 0; q < n_emit; q++) {
+    pkt_t p;
+    memset(&p, 0, sizeof(p));
+    p.id = id_base + (uint64_t)q;
+    p.draw = 0;
+    if (r_core == 0) {
+      double mu = 1 - 2.0 * urand(&c, &p);
+      double phi = 2.0 * OR_PI * urand(&c, &p);
+      double smu = sqrt(1 - mu * mu);
+      p.D[0] = smu * cos(phi); p.D[1] = smu * sin(phi); p.D[2] = mu;
+    } else {
+      double phi_core = 2 * OR_PI * urand(&c, &p);
+      double cosp_core = cos(phi_core), sinp_core = sin(phi_core);
+      double cost_core = 1 - 2.0 * urand(&c, &p);
+      double sint_core = sqrt(1 - cost_core * cost_core);
+      double a_phot = r_core + r_core * 1e-10;
+      p.x[0] = a_phot * sint_core * cosp_core;
+      p.x[1] = a_phot * sint_core * sinp_core;
+      p.x[2] = a_phot * cost_core;
+      double phi_loc = 2 * OR_PI * urand(&c, &p);
+      double cost_loc = sqrt(urand(&c, &p));
+      double sint_loc = sqrt(1 - cost_loc * cost_loc);
+      double D_xl = sint_loc * cos(phi_loc), D_yl = sint_loc * sin(phi_loc), D_zl = cost_loc;
+      p.D[0] = cost_core * cosp_core * D_xl - sinp_core * D_yl + sint_core * cosp_core * D_zl;
+      p.D[1] = cost_core * sinp_core * D_xl + cosp_core * D_yl + sint_core * sinp_core * D_zl;
+      p.D[2] = -sint_core * D_xl + cost_core * D_zl;
+    }
+    p.e = Ep;
+    if (core->core_frequency > 0) p.nu = core->core_frequency;
+    else {
+      int inu = or_cdf_sample(cdf, nn, urand(&c, &p));
+      p.nu = or_sample(nu_grid, inu, urand(&c, &p));
+    }
+    p.ind = or_get_zone(grid, p.x);
+    lorentz_transform(&c, &p, 1);
+    p.t = prm->t_now + urand(&c, &p) * prm->dt;
+    p.type = SGPU_PHOTON;
+    store_pkt(pk, offset + q, &p, SGPU_MOVING);
+  }
+  *n_out = n_emit;
+  free(cdf);
+  return 0;
+}
+
 /* transport::reduce_radiation normalisation, transport_mpi.cpp:415-441 */
 int or_normalize_tallies(const sgpu_grid* grid, const sgpu_locate* nu,
                          const sgpu_transport_params* prm, sgpu_tallies* ta)

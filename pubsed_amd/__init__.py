@@ -47,6 +47,9 @@ def _declare(lib):
         "sgpu_initialize_particles": (I, [V, P(abi.Grid), P(abi.Locate), P(abi.ZoneState), V, V,
                                           P(abi.TransportParams), I64, ctypes.c_uint64, P(abi.Packets),
                                           I64, P(I64)]),
+        "sgpu_emit_inner_source": (I, [V, P(abi.Grid), P(abi.Locate), P(abi.CoreSource),
+                                       P(abi.TransportParams), I64, ctypes.c_uint64, P(abi.Packets), I64,
+                                       P(I64)]),
         "sgpu_propagate": (I, [V, P(abi.Grid), P(abi.Locate), P(abi.Tables),
                                P(abi.TransportParams), P(abi.Packets), I64, P(abi.Tallies), V]),
         "sgpu_compact": (I, [V, P(abi.Packets), I64, P(abi.Packets), P(I64), P(I64)]),

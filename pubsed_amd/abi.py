@@ -75,6 +75,10 @@ class Packets(C.Structure):
                 ("fate", P), ("draw", P), ("id", P)]
 
 
+class CoreSource(C.Structure):
+    _fields_ = [("r_core", f64), ("T_core", f64), ("L_core", f64), ("core_frequency", f64)]
+
+
 class Tables(C.Structure):
     _fields_ = [("abs", P), ("scat", P), ("emis_cdf", P), ("eps_imc", P),
                 ("compton_opac", P), ("photoion_opac", P)]
@@ -113,7 +117,7 @@ PY_STRUCTS = {
     "sgpu_atoms": Atoms, "sgpu_opacity_params": OpacityParams, "sgpu_opacity_out": OpacityOut,
     "sgpu_packets": Packets, "sgpu_tables": Tables, "sgpu_spectrum": Spectrum,
     "sgpu_tallies": Tallies, "sgpu_transport_params": TransportParams,
-    "sgpu_counters": Counters,
+    "sgpu_counters": Counters, "sgpu_core_source": CoreSource,
 }
 
 # every extern "C" entry point declared in include/sedona_gpu.h
@@ -121,7 +125,8 @@ EXPORTED_SYMBOLS = [
     "sgpu_create", "sgpu_destroy", "sgpu_set_stream", "sgpu_get_stream", "sgpu_synchronize",
     "sgpu_last_error", "sgpu_alloc", "sgpu_free", "sgpu_memcpy_h2d", "sgpu_memcpy_d2h",
     "sgpu_last_kernel_ms", "sgpu_enable_timing", "sgpu_opacity_build",
-    "sgpu_emit_radioactive", "sgpu_initialize_particles", "sgpu_propagate", "sgpu_compact",
+    "sgpu_emit_radioactive", "sgpu_initialize_particles", "sgpu_emit_inner_source", "sgpu_propagate",
+    "sgpu_compact",
     "sgpu_compact_noalloc",
     "sgpu_wipe_radiation",
     "sgpu_normalize_tallies", "sgpu_nlte_solve", "sgpu_nlte_batched", "sgpu_nlte_autotune",

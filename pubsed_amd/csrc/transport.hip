@@ -1267,6 +1267,95 @@ __global__ void __launch_bounds__(256) init_particles_kernel(InitArgs A)
 }
 
 // ------------------------------------------------------------------
+// inner boundary source (transport::setup_core_emission,
+// transport_init.cpp:465-548; transport::emit_inner_source, emission.cpp:370-480)
+// ------------------------------------------------------------------
+// the core spectrum's CDF over the nu grid and the packet energy
+// L_core dt / n_emit (L_core from the spectrum when not given); one workgroup
+__global__ void __launch_bounds__(INIT_THREADS) core_prep_kernel(sgpu_locate nu, double r_core, double T_core,
+                                                                 double L_core, double dt, int64_t n_emit,
+                                                                 double* cdf, double* Ep)
+{
+  __shared__ double wsum[INIT_THREADS / WAVE];
+  const int nn = nu.n;
+  for (int j = threadIdx.x; j < nn; j += INIT_THREADS) {
+    const double nu_m = loc_center(nu, j);
+    double bb = 1;
+    if (T_core > 0) {
+      const double zeta = H_PLANCK * nu_m / K_BOLTZ / T_core;
+      bb = 2.0 * nu_m * nu_m * nu_m * H_PLANCK / C_LIGHT / C_LIGHT / (exp(zeta) - 1);
+    }
+    cdf[j] = bb * loc_delta(nu, j);
+  }
+  __syncthreads();
+  const double tot = block_cdf<INIT_THREADS>(cdf, nn, wsum);
+  if (threadIdx.x == 0) {
+    // L_sum = sum 4 pi r^2 pi B dnu, in the reference's factor order
+    const double L = L_core != 0 ? L_core : 4.0 * PI * r_core * r_core * PI * tot;
+    *Ep = L * dt / (double)n_emit;
+  }
+}
+
+struct CoreArgs {
+  GridView<double> g;
+  Axis<double> nu;
+  const double* cdf;
+  const double* Ep;
+  double r_core, core_frequency, t_now, dt;
+  uint64_t seed, id_base;
+  int64_t n_emit, offset;
+  sgpu_packets pk;
+};
+
+__global__ void __launch_bounds__(256) core_emit_kernel(CoreArgs A)
+{
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= A.n_emit) return;
+  Pkt<double> p;
+  p.rng = make_rng(A.seed, A.id_base + (uint64_t)q, 0);
+  if (A.r_core == 0) {
+    p.x[0] = p.x[1] = p.x[2] = 0;
+    isotropic_dir(p, p.D);
+  } else {
+    // a uniform point of the photosphere, a cos-theta outward direction
+    // about its normal, rotated into the grid frame
+    const double phi_core = 2 * PI * p.rng.uniform();
+    const double cosp_core = cos(phi_core), sinp_core = sin(phi_core);
+    const double cost_core = 1 - 2.0 * p.rng.uniform();
+    const double sint_core = sqrt(1 - cost_core * cost_core);
+    const double a_phot = A.r_core + A.r_core * 1e-10;
+    p.x[0] = a_phot * sint_core * cosp_core;
+    p.x[1] = a_phot * sint_core * sinp_core;
+    p.x[2] = a_phot * cost_core;
+    const double phi_loc = 2 * PI * p.rng.uniform();
+    const double cost_loc = sqrt(p.rng.uniform());
+    const double sint_loc = sqrt(1 - cost_loc * cost_loc);
+    const double D_xl = sint_loc * cos(phi_loc), D_yl = sint_loc * sin(phi_loc), D_zl = cost_loc;
+    p.D[0] = cost_core * cosp_core * D_xl - sinp_core * D_yl + sint_core * cosp_core * D_zl;
+    p.D[1] = cost_core * sinp_core * D_xl + cosp_core * D_yl + sint_core * sinp_core * D_zl;
+    p.D[2] = -sint_core * D_xl + cost_core * D_zl;
+  }
+  p.e = *A.Ep;
+  if (A.core_frequency > 0) p.nu = A.core_frequency;
+  else {
+    const int nn = A.nu.n;
+    const int inu = min(cdf_sample(A.cdf, nn, p.rng.uniform()), nn - 1);
+    p.nu = ax_sample(A.nu, inu, p.rng.uniform());
+  }
+  p.ind = get_zone(A.g, p.x);
+  lorentz_transform(A.g, p, 1);
+  p.t = A.t_now + p.rng.uniform() * A.dt;
+  p.type = SGPU_PHOTON;
+  const int64_t o = A.offset + q;
+  const sgpu_packets& k = A.pk;
+  k.x[o] = p.x[0]; k.y[o] = p.x[1]; k.z[o] = p.x[2];
+  k.Dx[o] = p.D[0]; k.Dy[o] = p.D[1]; k.Dz[o] = p.D[2];
+  k.t[o] = p.t; k.e[o] = p.e; k.nu[o] = p.nu;
+  k.ind[o] = p.ind; k.type[o] = p.type; k.fate[o] = SGPU_MOVING;
+  k.draw[o] = p.rng.draw; k.id[o] = A.id_base + (uint64_t)q;
+}
+
+// ------------------------------------------------------------------
 // K6: stream compaction (clean_up_particle_vector), order-keeping
 // ------------------------------------------------------------------
 constexpr int CMP_THREADS = 256;
@@ -1762,6 +1851,47 @@ int compact_args(sgpu_ctx* ctx, sgpu_packets* packets, int64_t n, sgpu_packets* 
   return SGPU_OK;
 }
 }  // namespace
+
+extern "C" int sgpu_emit_inner_source(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_locate* nu_grid,
+                                      const sgpu_core_source* core, const sgpu_transport_params* params,
+                                      int64_t n_emit, uint64_t id_base, sgpu_packets* packets, int64_t offset,
+                                      int64_t* n_out)
+{
+  SGPU_REQUIRE(ctx && nu_grid && core && params && n_out, "null argument");
+  int st = check_grid(grid);
+  if (st) return st;
+  st = check_packets(packets);
+  if (st) return st;
+  SGPU_REQUIRE(nu_grid->n >= 1 && nu_grid->x, "bad frequency grid");
+  SGPU_REQUIRE(n_emit >= 0 && offset >= 0, "negative count or offset");
+  SGPU_REQUIRE(core->r_core >= 0, "core radius must be >= 0");
+  *n_out = 0;
+  // emission.cpp:375, 391-392: nothing to emit, or no room
+  if (n_emit == 0 || offset + n_emit > packets->capacity) return SGPU_OK;
+  double T_core = core->T_core;
+  // transport_init.cpp:478-479: the blackbody of L and R
+  if (core->L_core != 0 && core->r_core != 0 && T_core == 0)
+    T_core = pow(core->L_core / (4.0 * PI * core->r_core * core->r_core * SB), 0.25);
+  const int nn = nu_grid->n;
+  double* cdf = (double*)scratch(ctx, "core_cdf", sizeof(double) * ((size_t)nn + 1), &st);
+  if (st) return st;
+  double* ep = cdf + nn;
+  timer_begin(ctx, "emit");
+  hipLaunchKernelGGL(core_prep_kernel, dim3(1), dim3(INIT_THREADS), 0, ctx->stream, *nu_grid, core->r_core, T_core,
+                     core->L_core, params->dt, n_emit, cdf, ep);
+  SGPU_LAUNCH_CHECK();
+  CoreArgs A;
+  A.g = make_grid<double>(*grid, grid->r_out.x, grid->x_out.x, grid->z_out.x);
+  A.nu = make_axis<double>(*nu_grid, nu_grid->x);
+  A.cdf = cdf; A.Ep = ep; A.r_core = core->r_core; A.core_frequency = core->core_frequency;
+  A.t_now = params->t_now; A.dt = params->dt; A.seed = params->seed; A.id_base = id_base;
+  A.n_emit = n_emit; A.offset = offset; A.pk = *packets;
+  hipLaunchKernelGGL(core_emit_kernel, dim3(grid_blocks(n_emit, 256)), dim3(256), 0, ctx->stream, A);
+  SGPU_LAUNCH_CHECK();
+  timer_end(ctx, "emit");
+  *n_out = n_emit;
+  return SGPU_OK;
+}
 
 extern "C" int sgpu_compact(sgpu_ctx* ctx, sgpu_packets* packets, int64_t n, sgpu_packets* scr,
                             int64_t* n_alive, int64_t* n_escaped)

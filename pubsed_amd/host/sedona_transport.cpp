@@ -118,6 +118,7 @@ int transport::init(const Params& p, const Grid1D& g, const AtomicData* atoms, i
   grid_.kind = SGPU_GRID_1D_SPHERE;
   grid_.n_zones = nz_;
   grid_.t_now = g.t_now;
+  grid_.r_core = p.core_radius;
   grid_.r_out = {nz_, SGPU_LOC_FLEX, g.r_min, 0.0, d_r_out_};
   grid_.vol = d_vol_;
 
@@ -268,6 +269,15 @@ int transport::emit_particles(double dt)
                                  n_particles_, &ta_, &got);
   if (rc < 0) return fail(rc, "emit_particles");
   n_particles_ += got;
+  // emission.cpp:15-22: the inner boundary source after the radioactive one
+  if (p_.core_n_emit > 0) {
+    const sgpu_core_source core = {p_.core_radius, p_.core_temperature, p_.core_luminosity,
+                                   p_.core_photon_frequency};
+    rc = sgpu_emit_inner_source(ctx_, &grid_, &nu_, &core, &tp_, p_.core_n_emit, (step_index_ << 32) | (1ull << 31),
+                                &pk_, n_particles_, &got);
+    if (rc < 0) return fail(rc, "emit_inner_source");
+    n_particles_ += got;
+  }
   return SGPU_OK;
 }
 

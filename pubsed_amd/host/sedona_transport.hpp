@@ -47,6 +47,12 @@ struct Params {
   double limits_temp_max = 1e8;
   int64_t particles_n_initialize = 0;        // initial radiation field packets (initialize_particles)
   int64_t particles_n_emit_radioactive = 0;
+  // inner boundary source (setup_core_emission / emit_inner_source)
+  int64_t core_n_emit = 0;
+  double core_radius = 0;                      // also the grid's inner absorbing boundary
+  double core_temperature = 0;
+  double core_luminosity = 0;
+  double core_photon_frequency = 0;
   int64_t particles_max_total = 10000000;
   uint64_t transport_rng_seed = 20261203;
   int precision = SGPU_PREC_MIXED;

@@ -128,7 +128,14 @@ class Problem:
         lo, hi, nbin = cfg["nu"]
         # transport_nu_grid {nu_1, nu_2, del} with nu_1 == nu_2 (the default
         # {1,1,1}) is locate_array's catch-all single bin: grey transport
-        self.nu_np = W.lin_init_n(lo, hi, 1) if lo == hi else W.nu_log_grid(lo, hi, nbin)
+        # {nu_1, nu_2, del, 1} with a given del: locate_array::log_init's own edges
+        if lo == hi:
+            self.nu_np = W.lin_init_n(lo, hi, 1)
+        elif cfg.get("nu_log_del"):
+            x = W.log_init(lo, hi, cfg["nu_log_del"])
+            self.nu_np = dict(n=len(x), type=abi.LOC_LOG, min=lo, del_=cfg["nu_log_del"], x=x)
+        else:
+            self.nu_np = W.nu_log_grid(lo, hi, nbin)
         self.nu, a["nu_x"] = locate_struct(S, self.nu_np)
         self.n_nu = nn = self.nu_np["n"]
         if cfg["grid"] == "1d":
@@ -391,6 +398,16 @@ class GpuBackend:
                                              ctypes.byref(P.tparams), n_emit, force_rproc, id_base,
                                              ctypes.byref(P.packets), offset, ctypes.byref(P.tallies),
                                              ctypes.byref(nout)), "sgpu_emit_radioactive")
+        return nout.value
+
+    def emit_inner_source(self, P, core, n_emit, id_base=0, offset=0):
+        """sgpu_emit_inner_source with core an abi.CoreSource."""
+        from . import check
+        nout = ctypes.c_int64(0)
+        check(self.lib.sgpu_emit_inner_source(self.ctx, ctypes.byref(P.grid), ctypes.byref(P.nu),
+                                              ctypes.byref(core), ctypes.byref(P.tparams), n_emit, id_base,
+                                              ctypes.byref(P.packets), offset, ctypes.byref(nout)),
+              "sgpu_emit_inner_source")
         return nout.value
 
     def initialize_particles(self, P, n_init, id_base=0, offset=0):
