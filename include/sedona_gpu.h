@@ -330,6 +330,26 @@ void      sgpu_destroy(sgpu_ctx* ctx);
 int  sgpu_set_stream(sgpu_ctx* ctx, void* hip_stream);
 void* sgpu_get_stream(sgpu_ctx* ctx);
 int  sgpu_synchronize(sgpu_ctx* ctx);
+/* Back to the context's own (created, blocking) stream after sgpu_set_stream. */
+int  sgpu_use_own_stream(sgpu_ctx* ctx);
+
+/* HIP graph of a whole step (no reference counterpart: the reference has no
+ * GPU).  sgpu_graph_begin starts capturing the context's stream (a created
+ * stream, not the null stream); the sgpu_* calls that follow are recorded,
+ * not run -- they must make no host round trip (the *_dev entry points below
+ * keep the packet counts on the device; sgpu_opacity_build with
+ * zone_status NULL) and allocate nothing (every call runs once uncaptured
+ * first, at the same sizes, so that its scratch exists).  sgpu_graph_end
+ * ends the capture, updates the context's executable graph in place when the
+ * topology matches the previous capture's (otherwise instantiates it anew)
+ * and, with launch != 0, launches it.  Kernel arguments such as the step's
+ * times are fixed when captured, so a time-dependent run captures every
+ * step.  sgpu_graph_launch replays the last graph; sgpu_graph_nodes gives
+ * its node count. */
+int  sgpu_graph_begin(sgpu_ctx* ctx);
+int  sgpu_graph_end(sgpu_ctx* ctx, int32_t launch);
+int  sgpu_graph_launch(sgpu_ctx* ctx);
+int64_t sgpu_graph_nodes(sgpu_ctx* ctx);
 /* Copy the last error message of this thread into buf (NUL-terminated). */
 int  sgpu_last_error(char* buf, size_t n);
 int  sgpu_alloc(sgpu_ctx* ctx, void** ptr, size_t bytes);
@@ -466,6 +486,29 @@ int sgpu_compact(sgpu_ctx* ctx, sgpu_packets* packets, int64_t n,
  * Calls that overlap on one GPU share that array and race. */
 int sgpu_compact_noalloc(sgpu_ctx* ctx, sgpu_packets* packets, int64_t n, sgpu_packets* scr,
                          int64_t* n_alive, int64_t* n_escaped);
+
+/* Device-resident packet counts (no reference counterpart), for a step
+ * enqueued without host round trips, e.g. captured in a HIP graph:
+ *   sgpu_emit_radioactive_dev: as sgpu_emit_radioactive with the offset read
+ *     from the device int64 *d_count, which it then advances by n_emit when
+ *     packets were emitted (non-zero decay luminosity, room in the buffer:
+ *     otherwise nothing is emitted, as the reference's "Out of particle
+ *     space" case);
+ *   sgpu_propagate_dev: as sgpu_propagate over packets [0, *d_n), the
+ *     launches sized for the buffer's capacity;
+ *   sgpu_compact_dev: as sgpu_compact over [0, *d_count), leaving the
+ *     survivor count in *d_count and the escapes in *d_escaped (device; may
+ *     be NULL); scr must be at least as large as packets. */
+int sgpu_emit_radioactive_dev(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_locate* nu_grid,
+                              const sgpu_zone_state* zones, const sgpu_tables* tables,
+                              const sgpu_transport_params* params, int64_t n_emit, int32_t force_rproc,
+                              uint64_t id_base, sgpu_packets* packets, int64_t* d_count, sgpu_tallies* tallies);
+int sgpu_propagate_dev(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_locate* nu_grid,
+                       const sgpu_tables* tables, const sgpu_transport_params* params,
+                       sgpu_packets* packets, const int64_t* d_n, sgpu_tallies* tallies,
+                       sgpu_counters* counters);
+int sgpu_compact_dev(sgpu_ctx* ctx, sgpu_packets* packets, int64_t* d_count, sgpu_packets* scr,
+                     int64_t* d_escaped);
 
 /* Zero the zone and spectrum tallies (transport::wipe_radiation; the spectra
  * are wiped only when wipe_spectra != 0). */

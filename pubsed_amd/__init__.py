@@ -54,6 +54,17 @@ def _declare(lib):
                                P(abi.TransportParams), P(abi.Packets), I64, P(abi.Tallies), V]),
         "sgpu_compact": (I, [V, P(abi.Packets), I64, P(abi.Packets), P(I64), P(I64)]),
         "sgpu_compact_noalloc": (I, [V, P(abi.Packets), I64, P(abi.Packets), P(I64), P(I64)]),
+        "sgpu_use_own_stream": (I, [V]),
+        "sgpu_graph_begin": (I, [V]),
+        "sgpu_graph_end": (I, [V, I32]),
+        "sgpu_graph_launch": (I, [V]),
+        "sgpu_graph_nodes": (I64, [V]),
+        "sgpu_emit_radioactive_dev": (I, [V, P(abi.Grid), P(abi.Locate), P(abi.ZoneState), P(abi.Tables),
+                                          P(abi.TransportParams), I64, I32, ctypes.c_uint64, P(abi.Packets), V,
+                                          P(abi.Tallies)]),
+        "sgpu_propagate_dev": (I, [V, P(abi.Grid), P(abi.Locate), P(abi.Tables), P(abi.TransportParams),
+                                   P(abi.Packets), V, P(abi.Tallies), V]),
+        "sgpu_compact_dev": (I, [V, P(abi.Packets), V, P(abi.Packets), V]),
         "sgpu_wipe_radiation": (I, [V, I32, I32, P(abi.Tallies), I32]),
         "sgpu_normalize_tallies": (I, [V, P(abi.Grid), P(abi.Locate), P(abi.TransportParams),
                                        P(abi.Tallies)]),

@@ -29,6 +29,14 @@ void* scratch(sgpu_ctx* ctx, const char* name, size_t bytes, int* status)
 {
   sgpu_scratch& s = ctx->scratch[name];
   if (s.bytes >= bytes && s.ptr) { *status = SGPU_OK; return s.ptr; }
+  if (ctx->capturing) {
+    // no allocation inside a captured graph: the same calls must run once
+    // uncaptured first, at the same sizes
+    set_error("scratch '%s' would grow to %zu bytes during graph capture; run the step once uncaptured first",
+              name, bytes);
+    *status = SGPU_ERR_ARG;
+    return nullptr;
+  }
   if (s.ptr) {
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) { *status = hip_fail(e, "sync before scratch regrow", __FILE__, __LINE__); return nullptr; }
@@ -45,7 +53,7 @@ void* scratch(sgpu_ctx* ctx, const char* name, size_t bytes, int* status)
 
 void timer_begin(sgpu_ctx* ctx, const char* phase)
 {
-  if (!ctx->timing) return;
+  if (!ctx->timing || ctx->capturing) return;
   sgpu_phase_timer& t = ctx->timers[phase];
   if (!t.start) { (void)hipEventCreate(&t.start); (void)hipEventCreate(&t.stop); }
   (void)hipEventRecord(t.start, ctx->stream);
@@ -53,7 +61,7 @@ void timer_begin(sgpu_ctx* ctx, const char* phase)
 
 void timer_end(sgpu_ctx* ctx, const char* phase)
 {
-  if (!ctx->timing) return;
+  if (!ctx->timing || ctx->capturing) return;
   sgpu_phase_timer& t = ctx->timers[phase];
   (void)hipEventRecord(t.stop, ctx->stream);
   t.pending = true;
@@ -87,8 +95,61 @@ void sgpu_destroy(sgpu_ctx* ctx)
     if (kv.second.start) (void)hipEventDestroy(kv.second.start);
     if (kv.second.stop) (void)hipEventDestroy(kv.second.stop);
   }
+  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
+}
+
+int sgpu_graph_begin(sgpu_ctx* ctx)
+{
+  SGPU_REQUIRE(ctx && !ctx->capturing, "null context, or a capture already open");
+  SGPU_REQUIRE(ctx->stream != nullptr, "graph capture needs a created stream, not the null stream");
+  SGPU_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  ctx->capturing = 1;
+  return SGPU_OK;
+}
+
+int sgpu_graph_end(sgpu_ctx* ctx, int32_t launch)
+{
+  SGPU_REQUIRE(ctx && ctx->capturing, "no capture open");
+  ctx->capturing = 0;
+  hipGraph_t g = nullptr;
+  SGPU_HIP(hipStreamEndCapture(ctx->stream, &g));
+  size_t nodes = 0;
+  (void)hipGraphGetNodes(g, nullptr, &nodes);
+  ctx->graph_nodes = nodes;
+  if (ctx->graph_exec) {
+    // the same topology as the last step's graph: update its parameters
+    hipGraphNode_t err_node = nullptr;
+    hipGraphExecUpdateResult res;
+    if (hipGraphExecUpdate(ctx->graph_exec, g, &err_node, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(ctx->graph_exec);
+      ctx->graph_exec = nullptr;
+    }
+  }
+  hipError_t e = hipSuccess;
+  if (!ctx->graph_exec) e = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
+  if (e == hipSuccess && launch) e = hipGraphLaunch(ctx->graph_exec, ctx->stream);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return sgpu::hip_fail(e, "graph instantiate / launch", __FILE__, __LINE__);
+  return SGPU_OK;
+}
+
+int sgpu_graph_launch(sgpu_ctx* ctx)
+{
+  SGPU_REQUIRE(ctx && ctx->graph_exec && !ctx->capturing, "no graph to launch");
+  SGPU_HIP(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+  return SGPU_OK;
+}
+
+int64_t sgpu_graph_nodes(sgpu_ctx* ctx) { return ctx ? (int64_t)ctx->graph_nodes : -1; }
+
+int sgpu_use_own_stream(sgpu_ctx* ctx)
+{
+  SGPU_REQUIRE(ctx, "null context");
+  ctx->stream = ctx->own_stream;
+  return SGPU_OK;
 }
 
 int sgpu_set_stream(sgpu_ctx* ctx, void* s)

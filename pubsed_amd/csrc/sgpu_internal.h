@@ -34,6 +34,10 @@ struct sgpu_ctx {
   std::map<std::string, sgpu_phase_timer> timers;
   std::map<std::string, sgpu_scratch> scratch;
   nlte_tuning nlte;
+  // HIP graph of a captured step (sgpu_graph_begin / sgpu_graph_end)
+  int capturing = 0;
+  hipGraphExec_t graph_exec = nullptr;
+  size_t graph_nodes = 0;
 };
 
 namespace sgpu {

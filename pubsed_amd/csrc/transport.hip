@@ -70,6 +70,7 @@ struct PropArgs {
   sgpu_transport_params p;
   sgpu_packets pk;
   int64_t n;
+  const int64_t* n_dev;   // when set: the packet count, read on the device
   sgpu_tallies ta;
   sgpu_counters* cnt;
   int lds_tallies;   // zone scalars accumulated in LDS
@@ -851,6 +852,7 @@ __global__ void __launch_bounds__(PROP_THREADS) __attribute__((amdgpu_waves_per_
 propagate_kernel(PropArgs<R> A0)
 {
   PropArgs<R> A = A0;
+  if (A0.n_dev) A.n = *A0.n_dev;
   if constexpr (KIND != 0) A.g.kind = KIND;
   const int nz = A.g.n_zones;
   const bool use_lds = A.lds_tallies;
@@ -1096,6 +1098,8 @@ struct EmitArgs {
   double t_now, dt;
   uint64_t seed, id_base;
   int64_t n_emit, offset;
+  const int64_t* offset_dev;   // when set: the offset, read on the device; nothing is
+                               // emitted if the packets would pass the capacity
   sgpu_packets pk;
   double* L_dep;
   int lds_dep;
@@ -1111,7 +1115,13 @@ __global__ void __launch_bounds__(256) emit_kernel(EmitArgs A)
   }
   const double L_tot = *A.L_tot;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < A.n_emit && L_tot != 0) {
+  int64_t offset = A.offset;
+  bool fits = true;
+  if (A.offset_dev) {
+    offset = *A.offset_dev;
+    fits = offset + A.n_emit <= A.pk.capacity;
+  }
+  if (q < A.n_emit && L_tot != 0 && fits) {
     const double E_p = L_tot * A.dt / (1.0 * A.n_emit);
     Pkt<double> p;
     p.rng = make_rng(A.seed, A.id_base + (uint64_t)q, 0);
@@ -1138,7 +1148,7 @@ __global__ void __launch_bounds__(256) emit_kernel(EmitArgs A)
     p.e = E_p;
     lorentz_transform(A.g, p, 1);
     p.t = t;
-    const int64_t o = A.offset + q;
+    const int64_t o = offset + q;
     const sgpu_packets& k = A.pk;
     k.x[o] = p.x[0]; k.y[o] = p.x[1]; k.z[o] = p.x[2];
     k.Dx[o] = p.D[0]; k.Dy[o] = p.D[1]; k.Dz[o] = p.D[2];
@@ -1355,14 +1365,24 @@ __global__ void __launch_bounds__(256) core_emit_kernel(CoreArgs A)
   k.draw[o] = p.rng.draw; k.id[o] = A.id_base + (uint64_t)q;
 }
 
+// the device packet count after an emission: + n_emit when packets were
+// emitted (non-zero luminosity, room in the buffer); one thread
+__global__ void emit_count_kernel(const double* L_tot, int64_t n_emit, int64_t capacity, int64_t* count)
+{
+  const int64_t c = *count;
+  if (*L_tot != 0 && c + n_emit <= capacity) *count = c + n_emit;
+}
+
 // ------------------------------------------------------------------
 // K6: stream compaction (clean_up_particle_vector), order-keeping
 // ------------------------------------------------------------------
 constexpr int CMP_THREADS = 256;
 
-__global__ void keep_count_kernel(const int32_t* fate, int64_t n, int32_t* blk_keep, int32_t* blk_esc)
+__global__ void keep_count_kernel(const int32_t* fate, int64_t n, int32_t* blk_keep, int32_t* blk_esc,
+                                  const int64_t* n_dev)
 {
   __shared__ int ks[CMP_THREADS / WAVE], es[CMP_THREADS / WAVE];
+  if (n_dev) n = *n_dev;
   int64_t i = (int64_t)blockIdx.x * CMP_THREADS + threadIdx.x;
   int f = i < n ? fate[i] : SGPU_ESCAPED + 100;
   int keep = (i < n) && f != SGPU_ESCAPED && f != SGPU_ABSORBED;
@@ -1412,9 +1432,11 @@ __global__ void __launch_bounds__(1024) scan_blocks_kernel(int32_t* blk_keep, co
   }
 }
 
-__global__ void scatter_keep_kernel(sgpu_packets src, sgpu_packets dst, int64_t n, const int32_t* blk_off)
+__global__ void scatter_keep_kernel(sgpu_packets src, sgpu_packets dst, int64_t n, const int32_t* blk_off,
+                                    const int64_t* n_dev)
 {
   __shared__ int woff[CMP_THREADS / WAVE];
+  if (n_dev) n = *n_dev;
   int64_t i = (int64_t)blockIdx.x * CMP_THREADS + threadIdx.x;
   int f = i < n ? src.fate[i] : SGPU_ESCAPED;
   int keep = (i < n) && f != SGPU_ESCAPED && f != SGPU_ABSORBED;
@@ -1440,6 +1462,25 @@ __global__ void copy_kernel(const T* __restrict__ s, T* __restrict__ d, int64_t 
 {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) d[i] = s[i];
+}
+
+// the first tot[0] packets of src into dst, every field (one launch)
+__global__ void copy_packets_kernel(sgpu_packets src, sgpu_packets dst, const long long* tot)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= *tot) return;
+  dst.x[i] = src.x[i]; dst.y[i] = src.y[i]; dst.z[i] = src.z[i];
+  dst.Dx[i] = src.Dx[i]; dst.Dy[i] = src.Dy[i]; dst.Dz[i] = src.Dz[i];
+  dst.t[i] = src.t[i]; dst.e[i] = src.e[i]; dst.nu[i] = src.nu[i];
+  dst.ind[i] = src.ind[i]; dst.type[i] = src.type[i]; dst.fate[i] = src.fate[i];
+  dst.draw[i] = src.draw[i]; dst.id[i] = src.id[i];
+}
+
+// the device counts after a compaction
+__global__ void compact_counts_kernel(const long long* tot, int64_t* n_alive, int64_t* n_escaped)
+{
+  *n_alive = tot[0];
+  if (n_escaped) *n_escaped = tot[1];
 }
 
 __global__ void fill_kernel(double* p, int64_t n, double v)
@@ -1565,10 +1606,12 @@ unsigned grid_blocks(int64_t n, int threads)
 
 }  // namespace
 
-extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_locate* nu_grid,
-                              const sgpu_tables* tables, const sgpu_transport_params* params,
-                              sgpu_packets* packets, int64_t n, sgpu_tallies* tallies,
-                              sgpu_counters* counters)
+namespace {
+// the propagation of packets [0, n), or of [0, *n_dev) read on the device
+// when n_dev is set (the grids are then sized for n, the buffer's capacity)
+int propagate_impl(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_locate* nu_grid, const sgpu_tables* tables,
+                   const sgpu_transport_params* params, sgpu_packets* packets, int64_t n, const int64_t* n_dev,
+                   sgpu_tallies* tallies, sgpu_counters* counters)
 {
   SGPU_REQUIRE(ctx && nu_grid && tables && params && tallies, "null argument");
   int st = check_grid(grid);
@@ -1598,7 +1641,7 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
     A.nu = make_axis<double>(*nu_grid, nu_grid->x);
     A.tab = {tables->abs, tables->scat, tables->emis_cdf};
     A.eps_imc = tables->eps_imc; A.compton = tables->compton_opac; A.photoion = tables->photoion_opac;
-    A.p = *params; A.pk = *packets; A.n = n; A.ta = *tallies; A.cnt = counters;
+    A.p = *params; A.pk = *packets; A.n = n; A.n_dev = n_dev; A.ta = *tallies; A.cnt = counters;
     A.lds_tallies = nz <= LDS_ZONE_MAX;
     const size_t lds = A.lds_tallies ? sizeof(double) * T_NFIELDS * nz : 0;
     const char* ew = getenv("SGPU_PROP_WG_PER_CU");
@@ -1617,7 +1660,8 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
     A.nu = make_axis<double>(*nu_grid, nu_grid->x);
     A.tab = {tables->abs, tables->scat, tables->emis_cdf};
     A.eps_imc = tables->eps_imc; A.compton = tables->compton_opac; A.photoion = tables->photoion_opac;
-    A.p = *params; A.pk = *packets; A.n = n; A.ta = *tallies; A.cnt = counters; A.lds_tallies = 0;
+    A.p = *params; A.pk = *packets; A.n = n; A.n_dev = n_dev; A.ta = *tallies; A.cnt = counters;
+    A.lds_tallies = 0;
     timer_begin(ctx, "propagate");
     if (params->use_ddmc == 2)
       hipLaunchKernelGGL((propagate_kernel<double, false, false, 1, 0, true>), dim3(grid_blocks(n, PROP_THREADS)),
@@ -1662,7 +1706,7 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
     A.nu = make_axis<float>(*nu_grid, nu32);
     A.tab = {ke, alias};
     A.eps_imc = tables->eps_imc; A.compton = tables->compton_opac; A.photoion = tables->photoion_opac;
-    A.p = *params; A.pk = *packets; A.n = n; A.ta = *tallies; A.cnt = counters;
+    A.p = *params; A.pk = *packets; A.n = n; A.n_dev = n_dev; A.ta = *tallies; A.cnt = counters;
     // launch shape (tuning knobs for sweeps, defaults below):
     //   SGPU_PROP_SHAPE  0 persistent lanes + register tallies (default)
     //                    1 one packet per thread + register tallies
@@ -1709,24 +1753,35 @@ extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_l
   timer_end(ctx, "propagate");
   return SGPU_OK;
 }
+}  // namespace
 
-extern "C" int sgpu_emit_radioactive(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_locate* nu_grid,
-                                     const sgpu_zone_state* zones, const sgpu_tables* tables,
-                                     const sgpu_transport_params* params, int64_t n_emit,
-                                     int32_t force_rproc, uint64_t id_base, sgpu_packets* packets,
-                                     int64_t offset, sgpu_tallies* tallies, int64_t* n_out)
+extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_locate* nu_grid,
+                              const sgpu_tables* tables, const sgpu_transport_params* params,
+                              sgpu_packets* packets, int64_t n, sgpu_tallies* tallies,
+                              sgpu_counters* counters)
 {
-  SGPU_REQUIRE(ctx && nu_grid && zones && tables && params && tallies && n_out, "null argument");
-  int st = check_grid(grid);
-  if (st) return st;
-  st = check_packets(packets);
-  if (st) return st;
-  SGPU_REQUIRE(zones->n_zones == grid->n_zones, "zone count mismatch");
-  SGPU_REQUIRE(n_emit >= 0 && offset >= 0 && offset + n_emit <= packets->capacity,
-               "emission range [%lld, %lld) exceeds capacity %lld", (long long)offset,
-               (long long)(offset + n_emit), (long long)packets->capacity);
-  SGPU_REQUIRE(tables->emis_cdf && tallies->L_radio_dep && tallies->L_radio_emit, "missing arrays");
-  *n_out = 0;
+  return propagate_impl(ctx, grid, nu_grid, tables, params, packets, n, nullptr, tallies, counters);
+}
+
+extern "C" int sgpu_propagate_dev(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_locate* nu_grid,
+                                  const sgpu_tables* tables, const sgpu_transport_params* params,
+                                  sgpu_packets* packets, const int64_t* d_n, sgpu_tallies* tallies,
+                                  sgpu_counters* counters)
+{
+  SGPU_REQUIRE(d_n && packets, "null argument");
+  return propagate_impl(ctx, grid, nu_grid, tables, params, packets, packets->capacity, d_n, tallies, counters);
+}
+
+namespace {
+// the radioactive emission's kernels; the offset from the host (d_count
+// null) or from the device count, which the emission then advances
+int emit_radioactive_impl(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_locate* nu_grid,
+                          const sgpu_zone_state* zones, const sgpu_tables* tables,
+                          const sgpu_transport_params* params, int64_t n_emit, int32_t force_rproc,
+                          uint64_t id_base, sgpu_packets* packets, int64_t offset, int64_t* d_count,
+                          sgpu_tallies* tallies, double** ltot_out)
+{
+  int st = 0;
   const int nz = grid->n_zones;
   double* zcdf = (double*)scratch(ctx, "emit_zcdf", sizeof(double) * (2 * (size_t)nz + 1), &st);
   if (st) return st;
@@ -1742,13 +1797,70 @@ extern "C" int sgpu_emit_radioactive(sgpu_ctx* ctx, const sgpu_grid* grid, const
     A.nu = make_axis<double>(*nu_grid, nu_grid->x);
     A.emis_cdf = tables->emis_cdf; A.zcdf = zcdf; A.gfrac = gfr; A.L_tot = ltot;
     A.t_now = params->t_now; A.dt = params->dt; A.seed = params->seed; A.id_base = id_base;
-    A.n_emit = n_emit; A.offset = offset; A.pk = *packets; A.L_dep = tallies->L_radio_dep;
+    A.n_emit = n_emit; A.offset = offset; A.offset_dev = d_count; A.pk = *packets;
+    A.L_dep = tallies->L_radio_dep;
     A.lds_dep = nz <= 4096;
     hipLaunchKernelGGL(emit_kernel, dim3(grid_blocks(n_emit, 256)), dim3(256),
                        A.lds_dep ? sizeof(double) * nz : 0, ctx->stream, A);
     SGPU_LAUNCH_CHECK();
+    if (d_count) {
+      hipLaunchKernelGGL(emit_count_kernel, dim3(1), dim3(1), 0, ctx->stream, ltot, n_emit, packets->capacity,
+                         d_count);
+      SGPU_LAUNCH_CHECK();
+    }
   }
   timer_end(ctx, "emit");
+  *ltot_out = ltot;
+  return SGPU_OK;
+}
+
+int emit_args(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_locate* nu_grid, const sgpu_zone_state* zones,
+              const sgpu_tables* tables, const sgpu_transport_params* params, const sgpu_packets* packets,
+              const sgpu_tallies* tallies, int64_t n_emit)
+{
+  SGPU_REQUIRE(ctx && nu_grid && zones && tables && params && tallies, "null argument");
+  int st = check_grid(grid);
+  if (st) return st;
+  st = check_packets(packets);
+  if (st) return st;
+  SGPU_REQUIRE(zones->n_zones == grid->n_zones, "zone count mismatch");
+  SGPU_REQUIRE(n_emit >= 0, "negative packet count");
+  SGPU_REQUIRE(tables->emis_cdf && tallies->L_radio_dep && tallies->L_radio_emit, "missing arrays");
+  return SGPU_OK;
+}
+}  // namespace
+
+extern "C" int sgpu_emit_radioactive_dev(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_locate* nu_grid,
+                                         const sgpu_zone_state* zones, const sgpu_tables* tables,
+                                         const sgpu_transport_params* params, int64_t n_emit,
+                                         int32_t force_rproc, uint64_t id_base, sgpu_packets* packets,
+                                         int64_t* d_count, sgpu_tallies* tallies)
+{
+  int st = emit_args(ctx, grid, nu_grid, zones, tables, params, packets, tallies, n_emit);
+  if (st) return st;
+  SGPU_REQUIRE(d_count, "device count missing");
+  double* ltot;
+  return emit_radioactive_impl(ctx, grid, nu_grid, zones, tables, params, n_emit, force_rproc, id_base, packets,
+                               0, d_count, (sgpu_tallies*)tallies, &ltot);
+}
+
+extern "C" int sgpu_emit_radioactive(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_locate* nu_grid,
+                                     const sgpu_zone_state* zones, const sgpu_tables* tables,
+                                     const sgpu_transport_params* params, int64_t n_emit,
+                                     int32_t force_rproc, uint64_t id_base, sgpu_packets* packets,
+                                     int64_t offset, sgpu_tallies* tallies, int64_t* n_out)
+{
+  int st = emit_args(ctx, grid, nu_grid, zones, tables, params, packets, tallies, n_emit);
+  if (st) return st;
+  SGPU_REQUIRE(n_out, "null argument");
+  SGPU_REQUIRE(offset >= 0 && offset + n_emit <= packets->capacity,
+               "emission range [%lld, %lld) exceeds capacity %lld", (long long)offset,
+               (long long)(offset + n_emit), (long long)packets->capacity);
+  *n_out = 0;
+  double* ltot;
+  if ((st = emit_radioactive_impl(ctx, grid, nu_grid, zones, tables, params, n_emit, force_rproc, id_base, packets,
+                                  offset, nullptr, tallies, &ltot)))
+    return st;
   double hl = 0;
   SGPU_HIP(hipMemcpyAsync(&hl, ltot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   SGPU_HIP(hipStreamSynchronize(ctx->stream));
@@ -1805,9 +1917,10 @@ int compact_with(sgpu_ctx* ctx, sgpu_packets* packets, int64_t n, sgpu_packets* 
 {
   const unsigned nb = grid_blocks(n, CMP_THREADS);
   hipLaunchKernelGGL(keep_count_kernel, dim3(nb), dim3(CMP_THREADS), 0, ctx->stream, packets->fate, n, blk,
-                     blk + nb);
+                     blk + nb, nullptr);
   hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(1024), 0, ctx->stream, blk, blk + nb, (int)nb, tot);
-  hipLaunchKernelGGL(scatter_keep_kernel, dim3(nb), dim3(CMP_THREADS), 0, ctx->stream, *packets, *scr, n, blk);
+  hipLaunchKernelGGL(scatter_keep_kernel, dim3(nb), dim3(CMP_THREADS), 0, ctx->stream, *packets, *scr, n, blk,
+                     nullptr);
   SGPU_LAUNCH_CHECK();
   long long ht[2];
   SGPU_HIP(hipMemcpyAsync(ht, tot, sizeof(ht), hipMemcpyDeviceToHost, ctx->stream));
@@ -1890,6 +2003,34 @@ extern "C" int sgpu_emit_inner_source(sgpu_ctx* ctx, const sgpu_grid* grid, cons
   SGPU_LAUNCH_CHECK();
   timer_end(ctx, "emit");
   *n_out = n_emit;
+  return SGPU_OK;
+}
+
+extern "C" int sgpu_compact_dev(sgpu_ctx* ctx, sgpu_packets* packets, int64_t* d_count, sgpu_packets* scr,
+                                int64_t* d_escaped)
+{
+  SGPU_REQUIRE(ctx && d_count, "null argument");
+  int st = check_packets(packets);
+  if (st) return st;
+  st = check_packets(scr);
+  if (st) return st;
+  SGPU_REQUIRE(scr->capacity >= packets->capacity, "staging buffer smaller than the packet buffer");
+  const int64_t cap = packets->capacity;
+  if (cap == 0) return SGPU_OK;
+  const unsigned nb = grid_blocks(cap, CMP_THREADS);
+  int32_t* blk = (int32_t*)scratch(ctx, "cmpd_blk", sizeof(int32_t) * 2 * (size_t)nb + 64, &st);
+  if (st) return st;
+  long long* tot = (long long*)scratch(ctx, "cmpd_tot", sizeof(long long) * 2, &st);
+  if (st) return st;
+  hipLaunchKernelGGL(keep_count_kernel, dim3(nb), dim3(CMP_THREADS), 0, ctx->stream, packets->fate, cap, blk,
+                     blk + nb, (const int64_t*)d_count);
+  hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(1024), 0, ctx->stream, blk, blk + nb, (int)nb, tot);
+  hipLaunchKernelGGL(scatter_keep_kernel, dim3(nb), dim3(CMP_THREADS), 0, ctx->stream, *packets, *scr, cap, blk,
+                     (const int64_t*)d_count);
+  hipLaunchKernelGGL(copy_packets_kernel, dim3(grid_blocks(cap, 256)), dim3(256), 0, ctx->stream, *scr, *packets,
+                     tot);
+  hipLaunchKernelGGL(compact_counts_kernel, dim3(1), dim3(1), 0, ctx->stream, tot, d_count, d_escaped);
+  SGPU_LAUNCH_CHECK();
   return SGPU_OK;
 }
 

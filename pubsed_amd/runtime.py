@@ -351,6 +351,44 @@ def time_step(backend, P, dt, n_emit, scratch_pk, step_index, counters=None):
     return got, alive, esc
 
 
+def time_step_dev(backend, P, dt, n_emit, scratch_pk, step_index, d_cnt, counters=None, graph=False):
+    """time_step with the packet counts kept on the device -- d_cnt, a device
+    int64 pair: [0] packets on the grid (in and out), [1] escapes of the step
+    -- so that the step is enqueued without a host round trip; with
+    graph=True it is captured into a HIP graph and launched (the backend must
+    run on its own stream, and a run's first step uncaptured, so that every
+    call's scratch exists).  LTE opacities only (the NLTE gas state iterates
+    with the host)."""
+    backend.homologous_step(P, dt)
+    P.tparams.dt = dt
+    P.op_params.dt = dt
+    P.op_params.first_step = 1 if step_index == 0 else 0
+    if graph:
+        backend.graph_begin()
+    try:
+        backend.opacity_build(P, zone_status=False)
+        backend.wipe(P, wipe_spectra=int(P.cfg["steady"]))
+        backend.emit_dev(P, n_emit, d_cnt, id_base=(step_index + 1) << 32)
+        backend.propagate_dev(P, d_cnt, counters)
+        backend.compact_dev(P, scratch_pk, d_cnt, d_cnt[1:])
+        backend.normalize(P)
+        if P.cfg["radiative_eq"]:
+            backend.solve_eq_temperature(P)
+    except Exception:
+        if graph:
+            try:
+                backend.graph_end(launch=False)
+            except Exception:
+                pass
+        raise
+    if graph:
+        backend.graph_end(launch=True)
+    P.t_now += dt
+    P.tparams.t_now = P.t_now
+    P.op_params.t_now = P.t_now
+    P.op_params.first_step = 0
+
+
 class GpuBackend:
     """Runs the reference's step functions through libsedona_gpu.so."""
 
@@ -383,12 +421,54 @@ class GpuBackend:
         from . import check
         check(self.lib.sgpu_synchronize(self.ctx), "sgpu_synchronize")
 
-    def opacity_build(self, P):
+    def opacity_build(self, P, zone_status=True):
+        """zone_status=False leaves the per-zone solve status unwritten, so
+        the call makes no host round trip (it can then be graph-captured)."""
         from . import check
+        out = P.op_out
+        if not zone_status:
+            out = abi.OpacityOut(*[getattr(P.op_out, f) for f, _ in abi.OpacityOut._fields_])
+            out.zone_status = None
         return check(self.lib.sgpu_opacity_build(self.ctx, ctypes.byref(P.nu), ctypes.byref(P.zs),
                                                  ctypes.byref(P.atoms) if P.atoms else None,
-                                                 ctypes.byref(P.op_params), ctypes.byref(P.op_out)),
+                                                 ctypes.byref(P.op_params), ctypes.byref(out)),
                      "sgpu_opacity_build")
+
+    # device-resident packet counts and HIP graphs of a step
+    def use_own_stream(self):
+        from . import check
+        check(self.lib.sgpu_use_own_stream(self.ctx), "sgpu_use_own_stream")
+
+    def graph_begin(self):
+        from . import check
+        check(self.lib.sgpu_graph_begin(self.ctx), "sgpu_graph_begin")
+
+    def graph_end(self, launch=True):
+        from . import check
+        check(self.lib.sgpu_graph_end(self.ctx, 1 if launch else 0), "sgpu_graph_end")
+
+    def graph_nodes(self):
+        return int(self.lib.sgpu_graph_nodes(self.ctx))
+
+    def emit_dev(self, P, n_emit, d_count, id_base=0, force_rproc=0):
+        from . import check
+        check(self.lib.sgpu_emit_radioactive_dev(self.ctx, ctypes.byref(P.grid), ctypes.byref(P.nu),
+                                                 ctypes.byref(P.zs), ctypes.byref(P.tables),
+                                                 ctypes.byref(P.tparams), n_emit, force_rproc, id_base,
+                                                 ctypes.byref(P.packets), abi.ptr(d_count),
+                                                 ctypes.byref(P.tallies)), "sgpu_emit_radioactive_dev")
+
+    def propagate_dev(self, P, d_n, counters=None):
+        from . import check
+        check(self.lib.sgpu_propagate_dev(self.ctx, ctypes.byref(P.grid), ctypes.byref(P.nu),
+                                          ctypes.byref(P.tables), ctypes.byref(P.tparams),
+                                          ctypes.byref(P.packets), abi.ptr(d_n), ctypes.byref(P.tallies),
+                                          abi.ptr(counters)), "sgpu_propagate_dev")
+
+    def compact_dev(self, P, scratch_pk, d_count, d_escaped=None):
+        from . import check
+        check(self.lib.sgpu_compact_dev(self.ctx, ctypes.byref(P.packets), abi.ptr(d_count),
+                                        ctypes.byref(scratch_pk), abi.ptr(d_escaped)), "sgpu_compact_dev")
 
     def emit(self, P, n_emit, id_base=0, offset=0, force_rproc=0):
         from . import check
