@@ -325,51 +325,68 @@ def bench_nlte_state(gpu, P, torch, abi, np):
     return out
 
 
-def bench_time_dependent(a, gpu, steps, warmup, packets, brief=False):
+def bench_time_dependent(a, gpu, steps, warmup, packets, brief=False, graph=True):
     """Time-dependent configuration (the 2D ellipsoidal outflow): every step
     is the homologous hydro step, opacity build, wipe, emission of N new
     packets after the census, propagation to t + dt, compaction and
-    normalisation (runtime.time_step).  Census packets persist, so the timed
-    step (after the warm-up steps) propagates the census plus N.  `value`
-    is packets propagated per second of the whole step."""
+    normalisation.  Census packets persist, so a timed step (after the
+    warm-up steps) propagates the census plus N.  With graph=True the packet
+    counts stay on the device and every timed step is captured in a HIP
+    graph and launched (runtime.time_step_dev); the warm-up steps run
+    uncaptured, the last of them with per-kernel timing for the roofline.
+    `value` is packets propagated per second of the whole step."""
     import numpy as np
     import torch
     from pubsed_amd import runtime, abi, workloads as W
     cfg = runtime.CONFIGS[a.config]
     dt = cfg.get("dt", 0.1 * W.DAY)
     cnt = torch.zeros(10, dtype=torch.int64, device="cuda:0")
+    cnt_w = torch.zeros(10, dtype=torch.int64, device="cuda:0")
 
-    def run(precision, n_emit, steps, warm):
+    def run(precision, n_emit, steps, warm, graph):
         P = runtime.Problem(runtime.Space("cuda:0"), a.config)
         cap = n_emit * (steps + warm)
         P.alloc_packets(cap)
         scr = P.alloc_packets(cap, name="scr")
         P.tparams.precision = precision
-        for k in range(warm):
-            runtime.time_step(gpu, P, dt, n_emit, scr, k)
-            log("warm-up step %d: %d packets on the grid" % (k, P.n_packets))
-        torch.cuda.synchronize()
-        cnt.zero_()
-        n_prop, prop_ms, op_ms = 0, [], []
-        t0 = time.perf_counter()
-        for k in range(warm, warm + steps):
-            n_prop += P.n_packets + n_emit
-            runtime.time_step(gpu, P, dt, n_emit, scr, k, cnt)
-            prop_ms.append(gpu.last_ms("propagate"))
-            op_ms.append(gpu.last_ms("opacity"))
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t0
+        d_cnt = torch.zeros(2, dtype=torch.int64, device="cuda:0")
+        gpu.use_own_stream()
+        try:
+            prop_w = op_w = None
+            for k in range(warm):
+                last = k == warm - 1
+                if last:
+                    cnt_w.zero_()
+                    torch.cuda.synchronize()
+                runtime.time_step_dev(gpu, P, dt, n_emit, scr, k, d_cnt, cnt_w if last else None)
+                if last:
+                    prop_w, op_w = gpu.last_ms("propagate"), gpu.last_ms("opacity")
+                log("warm-up step %d: %d packets on the grid" % (k, int(d_cnt[0])))
+            torch.cuda.synchronize()
+            cnt.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for k in range(warm, warm + steps):
+                runtime.time_step_dev(gpu, P, dt, n_emit, scr, k, d_cnt, cnt, graph=graph and k > 0)
+            gpu.sync()
+            el = time.perf_counter() - t0
+        finally:
+            gpu.use_torch_stream()
+        P.n_packets = int(d_cnt[0])
         c = dict(zip(abi.COUNTER_FIELDS, cnt.cpu().tolist()))
-        return P, el, n_prop, prop_ms, op_ms, c
+        cw = dict(zip(abi.COUNTER_FIELDS, cnt_w.cpu().tolist()))
+        return P, el, c["n_packets"], [prop_w], [op_w], c, cw
 
     N = packets or cfg["n_packets"]
-    log("config %s, %d new packets per step, dt %.3g s, mixed precision" % (a.config, N, dt))
-    P, el, n_prop, prop_ms, op_ms, c = run(abi.PREC_MIXED, N, steps, warmup)
+    log("config %s, %d new packets per step, dt %.3g s, mixed precision%s" %
+        (a.config, N, dt, ", one HIP graph per step" if graph else ""))
+    P, el, n_prop, prop_ms, op_ms, c, cw = run(abi.PREC_MIXED, N, steps, warmup, graph)
     value = n_prop / el
     prop_avg = float(np.mean(prop_ms))
-    bpk = packet_bytes(c, P.n_nu, 1)
-    achieved = bpk * (n_prop / steps) / (prop_avg * 1e-3) / 1e9
-    log("mixed: %.3f s/step, propagate %.1f ms, %d packets propagated" % (el / steps, prop_avg, n_prop))
+    bpk = packet_bytes(cw, P.n_nu, 1)
+    achieved = bpk * cw["n_packets"] / (prop_avg * 1e-3) / 1e9
+    log("mixed: %.3f s/step, %d packets propagated; last warm-up step: propagate %.1f ms for %d packets" %
+        (el / steps, n_prop, prop_avg, cw["n_packets"]))
     per = {k: c[k] / max(c["n_packets"], 1) for k in ("n_events", "n_binshift", "n_redist", "n_escaped",
                                                        "n_stopped", "n_scatter")}
     spec = P.get("opt_click")
@@ -385,10 +402,13 @@ def bench_time_dependent(a, gpu, steps, warmup, packets, brief=False):
                    "new_packets_per_step": N, "dt_s": dt, "zones": P.nz, "nu_bins": P.n_nu,
                    "packets_on_grid_after": P.n_packets},
         "per_packet": per, "spectrum_packets": float(spec.sum()),
+        "hip_graph_per_step": bool(graph), "graph_nodes": gpu.graph_nodes() if graph else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": PROP_KERNEL % P.grid.kind, "kernel_ms": prop_avg,
-                     "bytes_per_packet": bpk},
+                     "bytes_per_packet": bpk,
+                     "measured_on": "the last warm-up step, run uncaptured with HIP event timing "
+                                    "(%d packets)" % cw["n_packets"]},
         "opacity_build": {"kernel_ms": float(np.mean(op_ms)), "zones": P.nz,
                           "lines": P.cfg.get("n_lines", 0)},
     }
@@ -398,7 +418,7 @@ def bench_time_dependent(a, gpu, steps, warmup, packets, brief=False):
         return res
     Np = min(N, 10_000_000)
     log("plain FP64: %d new packets per step" % Np)
-    _, elp, n_prop_p, _, _, cp = run(abi.PREC_FP64, Np, 1, warmup)
+    _, elp, n_prop_p, _, _, cp, _ = run(abi.PREC_FP64, Np, 1, max(warmup, 1), False)
     res["plain_fp64_packets_per_s"] = n_prop_p / elp
     res["plain_new_packets_per_step"] = Np
     res["mixed_over_plain"] = value / res["plain_fp64_packets_per_s"]
