@@ -615,13 +615,23 @@ int sgpu_nlte_populations(sgpu_ctx* ctx, const sgpu_atoms* atoms, int32_t nsys,
                           const double* b, const double* lte, double* lev_n, double* ion_frac);
 
 /* Choose launch shapes for sgpu_nlte_batched by timing candidates on the
- * given batch (project-written autotuner).  The choice is kept in the
- * context and used by later sgpu_nlte_batched calls. */
+ * given batch (project-written autotuner): workgroup size {128, 256, 512} x
+ * LU panel width {8, 16} x workgroups per CU {1, 2, 4}, for each
+ * precision, keeping only shapes whose workgroups can share a CU (LDS and
+ * registers).  The choice is kept in the context and used by later
+ * sgpu_nlte_batched calls. */
 int sgpu_nlte_autotune(sgpu_ctx* ctx, int32_t nsys, const int32_t* n,
                        const int64_t* rate_off, const int64_t* vec_off,
                        const double* rates, const double* lte,
                        double* b, double* pop, int32_t* iters,
                        char* report, size_t report_len);
+
+/* Pin the launch shape of sgpu_nlte_batched for one precision
+ * (SGPU_NLTE_FP64 or SGPU_NLTE_FP32_REFINE) instead of autotuning:
+ * threads a multiple of 64 in [128, 512], panel 8 or 16, wg_per_cu in [1, 16]
+ * (project-written; no reference counterpart). */
+int sgpu_nlte_set_tuning(sgpu_ctx* ctx, int32_t precision, int32_t threads, int32_t panel,
+                         int32_t wg_per_cu);
 
 /* ------------------------------------------------------------------ */
 /* radiative-equilibrium gas temperature                               */

@@ -130,7 +130,7 @@ EXPORTED_SYMBOLS = [
     "sgpu_compact_noalloc", "sgpu_use_own_stream", "sgpu_graph_begin", "sgpu_graph_end", "sgpu_graph_launch",
     "sgpu_graph_nodes", "sgpu_emit_radioactive_dev", "sgpu_propagate_dev", "sgpu_compact_dev",
     "sgpu_wipe_radiation",
-    "sgpu_normalize_tallies", "sgpu_nlte_solve", "sgpu_nlte_batched", "sgpu_nlte_autotune",
+    "sgpu_normalize_tallies", "sgpu_nlte_solve", "sgpu_nlte_batched", "sgpu_nlte_autotune", "sgpu_nlte_set_tuning",
     "sgpu_homologous_step", "sgpu_adjoint_forward", "sgpu_adjoint_backward",
     "sgpu_nlte_set_rates", "sgpu_nlte_populations", "sgpu_solve_eq_temperature",
     "sgpu_nlte_radiative_rates", "sgpu_opacity_build_nlte", "sgpu_alias_tables",

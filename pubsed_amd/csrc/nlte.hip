@@ -12,14 +12,24 @@
 //                       workgroup: the drop-in for one solve_nlte call.
 //   sgpu_nlte_batched   every system in one persistent launch.  Workgroups
 //                       pull systems (largest first) from a device queue.
-//     FP64 mode         FP64 LU in a per-workgroup device-memory slot.
+//                       Blocked right-looking LU in LDS (row stride n | 1):
+//                       one wavefront factors each panel and runs the
+//                       triangular solves (DPP pivot search, readlane
+//                       broadcasts, no workgroup barrier), every wavefront
+//                       takes part in the panel's row solve and the trailing
+//                       update -- three workgroup barriers per panel.
+//     FP64 mode         FP64 LU in LDS for n <= 143 (164 KB), in a
+//                       per-workgroup device-memory slot above that.
 //     FP32_REFINE mode  row/column power-of-two equilibration, FP32 LU in LDS
-//                       (n <= 202: 163 KB), trailing updates on MFMA
+//                       (n <= 201), trailing updates on MFMA
 //                       (v_mfma_f32_32x32x2f32) for the bulk of the matrix,
 //                       FP64 residuals r = e - M b straight from the FP64
-//                       rates, FP32 correction solves, until
-//                       max|db| <= tol max|b|; systems that do not converge
-//                       (or are too large for LDS) are re-solved in FP64.
+//                       rates, FP64 correction solves through the FP32
+//                       factors, until max|db| <= tol max|b|; systems that do
+//                       not converge (or are too large for LDS) are re-solved
+//                       in FP64.
+//   sgpu_nlte_autotune  times workgroup size x panel width x workgroups per
+//                       CU (those whose LDS can share a CU) for both modes.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstdio>
@@ -29,8 +39,8 @@
 
 namespace sgpu {
 
-constexpr int NLTE_THREADS = 256;
-constexpr int NLTE_LDS_MAX_N = 202;          // 202*202*4 = 163,216 B of LDS
+constexpr int NLTE_THREADS = 256;       // default workgroup size
+constexpr int NLTE_MAX_THREADS = 512;   // largest the batched kernels accept
 constexpr int NLTE_MAX_ITERS = 12;
 
 // ------------------------------------------------------------------
@@ -249,9 +259,9 @@ __device__ __forceinline__ int next_system(const BatchArgs& A, int* sh)
   return k < A.nsys ? A.order[k] : -1;
 }
 
-__global__ void __launch_bounds__(NLTE_THREADS) nlte_batched_f64_kernel(BatchArgs A)
+__global__ void __launch_bounds__(NLTE_MAX_THREADS) nlte_batched_f64_kernel(BatchArgs A, size_t /*lds_bytes*/)
 {
-  __shared__ ArgMax red[NLTE_THREADS / WAVE];
+  __shared__ ArgMax red[NLTE_MAX_THREADS / WAVE];
   __shared__ int perm[1024];
   __shared__ int singular, sh;
   double* M = A.slots + (size_t)blockIdx.x * A.slot_stride;
@@ -272,152 +282,656 @@ __global__ void __launch_bounds__(NLTE_THREADS) nlte_batched_f64_kernel(BatchArg
 }
 
 // ------------------------------------------------------------------
-// FP32 LU in LDS + FP64 refinement
+// LU in LDS.  One wavefront factors each panel (DPP pivot search, readlane
+// broadcasts, no workgroup barrier inside the panel); every wavefront takes
+// part in the panel's row solve and in the trailing update, which runs on
+// MFMA for whole and partial tiles.  The row stride is odd (ld = n | 1), so
+// that a column (lane-strided by ld) and a row both read LDS without bank
+// conflicts.  The loads of a step are issued together, ahead of its stores:
+// one LDS round trip per step instead of one per element.
 // ------------------------------------------------------------------
-// Scaled system: As = Dr M Dc with Dr, Dc powers of two; M b = e becomes
-// As y = Dr e, b = Dc y.  LDS holds As (row stride ld = n), the pivot
-// permutation lives in a per-workgroup device slot.
-struct MixedLds {
-  float a[NLTE_LDS_MAX_N * NLTE_LDS_MAX_N];
-};
+template <class T> struct LdsCap;
+template <> struct LdsCap<float> { static constexpr int n = 201; };    // 201 * 201 * 4 = 161,604 B
+template <> struct LdsCap<double> { static constexpr int n = 143; };   // 143 * 143 * 8 = 163,592 B
+constexpr size_t NLTE_LDS_BYTES = 163840;    // LDS of a CU (gfx950)
+// the largest matrices and the kernels' static shared memory (< 192 B) fit
+static_assert(sizeof(float) * 201 * 201 + 192 <= NLTE_LDS_BYTES, "FP32 LDS capacity");
+static_assert(sizeof(double) * 143 * 143 + 192 <= NLTE_LDS_BYTES, "FP64 LDS capacity");
+constexpr int NLTE_SLOTS = 4;   // rows per lane in the one-wavefront steps (n <= 256)
 
-// trailing update A[r0.., c0..] -= L[r0.., j0..j0+nb) * U[j0..j0+nb, c0..]
-// on MFMA 32x32x2 f32 tiles; the remainder rows/columns use VALU.
-__device__ void trailing_update_mfma(float* a, int ld, int j0, int nb, int r0, int n)
+__host__ __device__ __forceinline__ int lds_ld(int n) { return n | 1; }
+
+// dynamic LDS of a batched launch whose largest LDS system has cap levels:
+// the matrix, and at least two staged FP64 rows for the equilibration
+template <class T>
+__host__ __device__ inline size_t lds_bytes_for(int cap)
 {
-  const int m = n - r0;            // rows / cols of the trailing block
-  const int tiles = m / 32;        // full 32x32 tiles per dimension
-  const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-  const int nwaves = blockDim.x / WAVE;
-  // MFMA tiles
-  for (int t = wave; t < tiles * tiles; t += nwaves) {
-    const int ti = t / tiles, tj = t % tiles;
-    const int rb = r0 + ti * 32, cb = r0 + tj * 32;
-    typedef float f32x16 __attribute__((ext_vector_type(16)));
-    f32x16 acc;
-    // accumulator layout of v_mfma_f32_32x32x2f32: lane l holds column
-    // (l % 32), rows 8*(i/4) + 4*(l/32)... (i = 0..15): row = (i/4)*8 + (l/32)*4 + i%4
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-      int row = (i / 4) * 8 + (lane / 32) * 4 + (i % 4);
-      acc[i] = a[(size_t)(rb + row) * ld + cb + (lane % 32)];
-    }
-    for (int k = 0; k < nb; k += 2) {
-      // A operand: lane l supplies L[rb + l%32][j0 + k + l/32]
-      float av = -a[(size_t)(rb + lane % 32) * ld + j0 + k + lane / 32];
-      // B operand: lane l supplies U[j0 + k + l/32][cb + l%32]
-      float bv = a[(size_t)(j0 + k + lane / 32) * ld + cb + lane % 32];
-      if (k + lane / 32 >= nb) { av = 0.f; bv = 0.f; }
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-      int row = (i / 4) * 8 + (lane / 32) * 4 + (i % 4);
-      a[(size_t)(rb + row) * ld + cb + (lane % 32)] = acc[i];
-    }
-  }
-  // VALU remainder: rows >= r0 + 32*tiles or cols >= r0 + 32*tiles
-  const int edge = r0 + tiles * 32;
-  for (int idx = threadIdx.x; idx < m * m; idx += blockDim.x) {
-    int i = r0 + idx / m, c = r0 + idx % m;
-    if (i < edge && c < edge) continue;
-    float s = a[(size_t)i * ld + c];
-    for (int k = 0; k < nb; ++k) s -= a[(size_t)i * ld + j0 + k] * a[(size_t)(j0 + k) * ld + c];
-    a[(size_t)i * ld + c] = s;
-  }
+  const size_t m = sizeof(T) * (size_t)cap * lds_ld(cap), st = sizeof(double) * 2 * (size_t)lds_ld(cap);
+  return m > st ? m : st;
 }
 
-// blocked right-looking partial-pivot LU in LDS (panel width nb)
-__device__ int lu_f32_lds(float* a, int n, int* perm, int nb, ArgMax* red)
+// order LDS accesses of one wavefront (its LDS operations execute in order;
+// this keeps the compiler from moving them across)
+__device__ __forceinline__ void wave_fence()
 {
-  const int tid = threadIdx.x, NT = blockDim.x;
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <class V> struct PivotOf { V v; int i; };
+
+template <class V>
+__device__ __forceinline__ PivotOf<V> better_p(PivotOf<V> a, PivotOf<V> b)
+{
+  // larger magnitude wins; the smaller row index wins ties (first maximum)
+  return (b.v > a.v || (b.v == a.v && b.i < a.i)) ? b : a;
+}
+
+template <int CTRL>
+__device__ __forceinline__ int dpp(int v) { return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false); }
+
+template <int CTRL>
+__device__ __forceinline__ PivotOf<float> dpp_better(PivotOf<float> m)
+{
+  PivotOf<float> o;
+  o.v = __int_as_float(dpp<CTRL>(__float_as_int(m.v)));
+  o.i = dpp<CTRL>(m.i);
+  return better_p(m, o);
+}
+
+template <int CTRL>
+__device__ __forceinline__ PivotOf<double> dpp_better(PivotOf<double> m)
+{
+  PivotOf<double> o;
+  const long long b = __double_as_longlong(m.v);
+  const int lo = dpp<CTRL>((int)(b & 0xffffffff)), hi = dpp<CTRL>((int)(b >> 32));
+  o.v = __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+  o.i = dpp<CTRL>(m.i);
+  return better_p(m, o);
+}
+
+__device__ __forceinline__ float readlane(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+__device__ __forceinline__ double readlane(double v, int l)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// the wavefront's best (largest |value|, then smallest row), uniform result:
+// xor 1 and xor 2 inside quads, half-row and row mirrors (every lane then
+// holds its 16-lane row's best), the four rows through readlane
+template <class V>
+__device__ __forceinline__ PivotOf<V> wave_argmax(PivotOf<V> m)
+{
+  m = dpp_better<0xB1>(m);    // quad_perm [1,0,3,2]
+  m = dpp_better<0x4E>(m);    // quad_perm [2,3,0,1]
+  m = dpp_better<0x141>(m);   // row_half_mirror
+  m = dpp_better<0x140>(m);   // row_mirror
+  PivotOf<V> r = {readlane(m.v, 0), __builtin_amdgcn_readlane(m.i, 0)};
+  r = better_p(r, PivotOf<V>{readlane(m.v, 16), __builtin_amdgcn_readlane(m.i, 16)});
+  r = better_p(r, PivotOf<V>{readlane(m.v, 32), __builtin_amdgcn_readlane(m.i, 32)});
+  r = better_p(r, PivotOf<V>{readlane(m.v, 48), __builtin_amdgcn_readlane(m.i, 48)});
+  return r;
+}
+
+template <class V>
+__device__ __forceinline__ V wave_max(V v)
+{
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, WAVE));
+  return v;
+}
+
+// Panel j0 .. j0+jb of the partial-pivot LU, by one wavefront.  Per column
+// j: the pivot (first largest |a_ij|, i >= j) comes from registers -- the
+// candidates were formed by the previous column's update; one round trip
+// reads rows j and ip (interchanged over all n columns), the pivot and the
+// pivot row's panel segment; a second reads each lane's rows below j, which
+// are scaled by the pivot and updated in the panel's remaining columns.
+// Returns 1 when a pivot is exactly zero (that column is then left as is).
+template <class T, int NB>
+__device__ int panel_factor(T* a, int ld, int n, int j0, int jb, int* perm, int lane)
+{
+  const int ns = (n + WAVE - 1) / WAVE;
   int singular = 0;
-  for (int i = tid; i < n; i += NT) perm[i] = i;
-  __syncthreads();
-  for (int j0 = 0; j0 < n; j0 += nb) {
-    const int jb = min(nb, n - j0);
-    // panel factorisation (columns j0..j0+jb), updating the full rows
-    for (int j = j0; j < j0 + jb; ++j) {
-      ArgMax m = {-1.0, n};
-      for (int i = j + tid; i < n; i += NT) {
-        ArgMax t = {(double)fabsf(a[(size_t)i * n + j]), i};
-        m = better(m, t);
+  PivotOf<T> m = {(T)-1, n};
+#pragma unroll
+  for (int r = 0; r < NLTE_SLOTS; ++r) {
+    const int i = lane + WAVE * r;
+    if (r < ns && i >= j0 && i < n) m = better_p(m, PivotOf<T>{(T)fabs(a[i * ld + j0]), i});
+  }
+  for (int t = 0; t < jb; ++t) {
+    const int j = j0 + t;
+    const int mc = jb - t - 1;
+    const int ip = wave_argmax(m).i;
+    const bool swap = ip != j && ip < n;
+    const int pr = (ip < n ? ip : j) * ld;
+    T x[NLTE_SLOTS], y[NLTE_SLOTS], u[NB];
+    if (swap) {
+#pragma unroll
+      for (int r = 0; r < NLTE_SLOTS; ++r) {
+        const int k = lane + WAVE * r;
+        if (r < ns && k < n) { x[r] = a[j * ld + k]; y[r] = a[ip * ld + k]; }
       }
-      ArgMax p = block_argmax(m, red);
-      const int ip = p.i;
-      if (ip != j) {
-        for (int k = tid; k < n; k += NT) {
-          float t = a[(size_t)j * n + k];
-          a[(size_t)j * n + k] = a[(size_t)ip * n + k];
-          a[(size_t)ip * n + k] = t;
-        }
-        if (tid == 0) { int t = perm[j]; perm[j] = perm[ip]; perm[ip] = t; }
-      }
-      __syncthreads();
-      const float ajj = a[(size_t)j * n + j];
-      if (ajj == 0.f) { singular = 1; continue; }
-      const float inv = 1.f / ajj;
-      for (int i = j + 1 + tid; i < n; i += NT) a[(size_t)i * n + j] *= inv;
-      __syncthreads();
-      // update the rest of the panel columns only
-      const int mr = n - j - 1, mc = j0 + jb - j - 1;
-      if (mc > 0)
-        for (int idx = tid; idx < mr * mc; idx += NT) {
-          int i = j + 1 + idx / mc, k = j + 1 + idx % mc;
-          a[(size_t)i * n + k] -= a[(size_t)i * n + j] * a[(size_t)j * n + k];
-        }
-      __syncthreads();
     }
-    const int r0 = j0 + jb;
-    if (r0 >= n) break;
-    // U12 = L11^-1 A12 (unit lower triangular solve, columns in parallel)
-    for (int c = r0 + tid; c < n; c += NT)
-      for (int i = j0 + 1; i < j0 + jb; ++i) {
-        float s = a[(size_t)i * n + c];
-        for (int k = j0; k < i; ++k) s -= a[(size_t)i * n + k] * a[(size_t)k * n + c];
-        a[(size_t)i * n + c] = s;
+    const T piv = a[pr + j];
+#pragma unroll
+    for (int c = 0; c < NB - 1; ++c)
+      if (c < mc) u[c] = a[pr + j + 1 + c];
+    if (swap) {
+#pragma unroll
+      for (int r = 0; r < NLTE_SLOTS; ++r) {
+        const int k = lane + WAVE * r;
+        if (r < ns && k < n) { a[j * ld + k] = y[r]; a[ip * ld + k] = x[r]; }
       }
-    __syncthreads();
-    trailing_update_mfma(a, n, j0, jb, r0, n);
-    __syncthreads();
+      if (lane == 0) { const int q = perm[j]; perm[j] = perm[ip]; perm[ip] = q; }
+    }
+    wave_fence();
+    m = PivotOf<T>{(T)-1, n};
+    if (piv == (T)0) {
+      singular = 1;
+      if (mc > 0) {
+#pragma unroll
+        for (int r = 0; r < NLTE_SLOTS; ++r) {
+          const int i = lane + WAVE * r;
+          if (r < ns && i > j && i < n) m = better_p(m, PivotOf<T>{(T)fabs(a[i * ld + j + 1]), i});
+        }
+      }
+      continue;
+    }
+    const T inv = (T)1 / piv;
+#pragma unroll
+    for (int r = 0; r < NLTE_SLOTS; ++r) {
+      const int i = lane + WAVE * r;
+      if (r < ns && i > j && i < n) {
+        T sg[NB];
+        T l = a[i * ld + j];
+#pragma unroll
+        for (int c = 0; c < NB - 1; ++c)
+          if (c < mc) sg[c] = a[i * ld + j + 1 + c];
+        l *= inv;
+        a[i * ld + j] = l;
+#pragma unroll
+        for (int c = 0; c < NB - 1; ++c)
+          if (c < mc) {
+            sg[c] -= l * u[c];
+            a[i * ld + j + 1 + c] = sg[c];
+          }
+        if (mc > 0) m = better_p(m, PivotOf<T>{(T)fabs(sg[0]), i});
+      }
+    }
+    wave_fence();
   }
   return singular;
 }
 
-// y = U^-1 L^-1 (P rhs) with the FP32 factors, rhs/y FP64 in device memory
-__device__ void lu_f32_apply(const float* a, int n, const int* perm, const double* rhs, double* y, double* tmp)
+// The same for a full panel (jb == NB) with every column's shape known at
+// compile time: no per-element guards, so one wavefront issues about
+// 40 + 5 (NB - t) instructions per row slot and column instead of a guarded
+// sequence per element.
+template <class T, int NB>
+__device__ int panel_factor_full(T* a, int ld, int n, int j0, int* perm, int lane)
 {
-  const int tid = threadIdx.x, NT = blockDim.x;
-  for (int i = tid; i < n; i += NT) tmp[i] = rhs[perm[i]];
-  __syncthreads();
-  for (int k = 0; k < n; ++k) {
-    const double v = tmp[k];
-    for (int i = k + 1 + tid; i < n; i += NT) tmp[i] -= (double)a[(size_t)i * n + k] * v;
-    __syncthreads();
+  int singular = 0;
+  int rowi[NLTE_SLOTS];
+  bool inb[NLTE_SLOTS];
+#pragma unroll
+  for (int r = 0; r < NLTE_SLOTS; ++r) {
+    const int i = lane + WAVE * r;
+    inb[r] = i < n;
+    rowi[r] = (i < n ? i : n - 1) * ld;
   }
-  for (int k = n - 1; k >= 0; --k) {
-    if (tid == 0) tmp[k] = tmp[k] / (double)a[(size_t)k * n + k];
-    __syncthreads();
-    const double v = tmp[k];
-    for (int i = tid; i < k; i += NT) tmp[i] -= (double)a[(size_t)i * n + k] * v;
-    __syncthreads();
+  PivotOf<T> m = {(T)-1, n};
+#pragma unroll
+  for (int r = 0; r < NLTE_SLOTS; ++r) {
+    const int i = lane + WAVE * r;
+    if (inb[r] && i >= j0) m = better_p(m, PivotOf<T>{(T)fabs(a[rowi[r] + j0]), i});
   }
-  for (int i = tid; i < n; i += NT) y[i] = tmp[i];
-  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < NB; ++t) {
+    const int j = j0 + t;
+    const int ip = wave_argmax(m).i;
+    const bool swap = ip != j && ip < n;
+    const int pr = (ip < n ? ip : j) * ld;
+    T x[NLTE_SLOTS], y[NLTE_SLOTS], u[NB];
+    if (swap) {
+#pragma unroll
+      for (int r = 0; r < NLTE_SLOTS; ++r) {
+        const int k = lane + WAVE * r;
+        if (k < n) { x[r] = a[j * ld + k]; y[r] = a[ip * ld + k]; }
+      }
+    }
+    const T piv = a[pr + j];
+#pragma unroll
+    for (int c = 0; c < NB - 1 - t; ++c) u[c] = a[pr + j + 1 + c];
+    if (swap) {
+#pragma unroll
+      for (int r = 0; r < NLTE_SLOTS; ++r) {
+        const int k = lane + WAVE * r;
+        if (k < n) { a[j * ld + k] = y[r]; a[ip * ld + k] = x[r]; }
+      }
+      if (lane == 0) { const int q = perm[j]; perm[j] = perm[ip]; perm[ip] = q; }
+    }
+    wave_fence();
+    m = PivotOf<T>{(T)-1, n};
+    if (piv == (T)0) {
+      singular = 1;
+      if (t + 1 < NB) {
+#pragma unroll
+        for (int r = 0; r < NLTE_SLOTS; ++r) {
+          const int i = lane + WAVE * r;
+          if (inb[r] && i > j) m = better_p(m, PivotOf<T>{(T)fabs(a[rowi[r] + j + 1]), i});
+        }
+      }
+      continue;
+    }
+    const T inv = (T)1 / piv;
+#pragma unroll
+    for (int r = 0; r < NLTE_SLOTS; ++r) {
+      const int i = lane + WAVE * r;
+      if (inb[r] && i > j) {
+        T sg[NB];
+        T l = a[rowi[r] + j];
+#pragma unroll
+        for (int c = 0; c < NB - 1 - t; ++c) sg[c] = a[rowi[r] + j + 1 + c];
+        l *= inv;
+        a[rowi[r] + j] = l;
+#pragma unroll
+        for (int c = 0; c < NB - 1 - t; ++c) {
+          sg[c] -= l * u[c];
+          a[rowi[r] + j + 1 + c] = sg[c];
+        }
+        if (t + 1 < NB) m = better_p(m, PivotOf<T>{(T)fabs(sg[0]), i});
+      }
+    }
+    wave_fence();
+  }
+  return singular;
 }
 
-__global__ void __launch_bounds__(NLTE_THREADS) nlte_batched_mixed_kernel(BatchArgs A, int nb)
+// U12 = L11^-1 A12 (unit lower triangular), one column per thread, the
+// column held in registers
+template <class T, int NB>
+__device__ void panel_row_solve(T* a, int ld, int n, int j0, int jb, int r0)
 {
-  extern __shared__ float lds_a[];
-  __shared__ ArgMax red[NLTE_THREADS / WAVE];
-  __shared__ double dred[NLTE_THREADS / WAVE];
-  __shared__ int sh, sing_f64;
-  double* slot = A.slots + (size_t)blockIdx.x * A.slot_stride;
+  for (int c = r0 + (int)threadIdx.x; c < n; c += blockDim.x) {
+    T col[NB];
+#pragma unroll
+    for (int t = 0; t < NB; ++t)
+      if (t < jb) col[t] = a[(j0 + t) * ld + c];
+#pragma unroll
+    for (int i = 1; i < NB; ++i) {
+      if (i < jb) {
+        T acc = col[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) acc -= a[(j0 + i) * ld + j0 + k] * col[k];
+        col[i] = acc;
+      }
+    }
+#pragma unroll
+    for (int t = 1; t < NB; ++t)
+      if (t < jb) a[(j0 + t) * ld + c] = col[t];
+  }
+}
+
+// trailing update A22 -= L21 U12 in FP32 on v_mfma_f32_32x32x2f32 tiles;
+// rows / columns past n are read as zero and not written, so partial tiles
+// run on MFMA too.  Lane l supplies A[l%32][k + l/32] and B[k + l/32][l%32];
+// it holds accumulator column l%32, rows 8(i/4) + 4(l/32) + i%4, i < 16.
+template <int NB>
+__device__ void trailing_update(float* a, int ld, int j0, int jb, int r0, int n)
+{
+  typedef float f32x16 __attribute__((ext_vector_type(16)));
+  const int m = n - r0;
+  const int tiles = (m + 31) / 32;
+  const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  const int nwaves = blockDim.x / WAVE;
+  const int lr = lane % 32, lk = lane / 32;
+  for (int t = wave; t < tiles * tiles; t += nwaves) {
+    const int ti = t / tiles, tj = t - ti * tiles;
+    const int rb = r0 + ti * 32, cb = r0 + tj * 32;
+    const int col = cb + lr, arow = rb + lr;
+    const bool colok = col < n, arowok = arow < n;
+    float av[NB / 2], bv[NB / 2];
+#pragma unroll
+    for (int k2 = 0; k2 < NB / 2; ++k2) {
+      const int kk = 2 * k2 + lk;
+      const bool kok = kk < jb;
+      av[k2] = (kok && arowok) ? -a[arow * ld + j0 + kk] : 0.f;
+      bv[k2] = (kok && colok) ? a[(j0 + kk) * ld + col] : 0.f;
+    }
+    f32x16 acc;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int row = rb + (i / 4) * 8 + lk * 4 + (i % 4);
+      acc[i] = (row < n && colok) ? a[row * ld + col] : 0.f;
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < NB / 2; ++k2)
+      if (2 * k2 < jb) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[k2], bv[k2], acc, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int row = rb + (i / 4) * 8 + lk * 4 + (i % 4);
+      if (row < n && colok) a[row * ld + col] = acc[i];
+    }
+  }
+}
+
+// the same in FP64 on v_mfma_f64_16x16x4f64 tiles: lane l supplies
+// A[l%16][k + l/16] and B[k + l/16][l%16]; it holds accumulator column l%16,
+// rows l/16 + 4v, v < 4
+template <int NB>
+__device__ void trailing_update(double* a, int ld, int j0, int jb, int r0, int n)
+{
+  typedef double f64x4 __attribute__((ext_vector_type(4)));
+  const int m = n - r0;
+  const int tiles = (m + 15) / 16;
+  const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  const int nwaves = blockDim.x / WAVE;
+  const int lr = lane % 16, lk = lane / 16;
+  for (int t = wave; t < tiles * tiles; t += nwaves) {
+    const int ti = t / tiles, tj = t - ti * tiles;
+    const int rb = r0 + ti * 16, cb = r0 + tj * 16;
+    const int col = cb + lr, arow = rb + lr;
+    const bool colok = col < n, arowok = arow < n;
+    double av[NB / 4], bv[NB / 4];
+#pragma unroll
+    for (int k4 = 0; k4 < NB / 4; ++k4) {
+      const int kk = 4 * k4 + lk;
+      const bool kok = kk < jb;
+      av[k4] = (kok && arowok) ? -a[arow * ld + j0 + kk] : 0.0;
+      bv[k4] = (kok && colok) ? a[(j0 + kk) * ld + col] : 0.0;
+    }
+    f64x4 acc;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int row = rb + lk + 4 * v;
+      acc[v] = (row < n && colok) ? a[row * ld + col] : 0.0;
+    }
+#pragma unroll
+    for (int k4 = 0; k4 < NB / 4; ++k4)
+      if (4 * k4 < jb) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[k4], bv[k4], acc, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int row = rb + lk + 4 * v;
+      if (row < n && colok) a[row * ld + col] = acc[v];
+    }
+  }
+}
+
+// blocked right-looking partial-pivot LU in LDS (panel width NB), three
+// workgroup barriers per panel; returns 1 when a pivot is exactly zero
+template <class T, int NB>
+__device__ int lu_lds(T* a, int ld, int n, int* perm, int* sh_flag)
+{
   const int tid = threadIdx.x, NT = blockDim.x;
+  const int wave = tid / WAVE, lane = tid % WAVE;
+  for (int i = tid; i < n; i += NT) perm[i] = i;
+  if (tid == 0) *sh_flag = 0;
+  __syncthreads();
+  for (int j0 = 0; j0 < n; j0 += NB) {
+    const int jb = min(NB, n - j0);
+    if (wave == 0) {
+      const int sing = jb == NB ? panel_factor_full<T, NB>(a, ld, n, j0, perm, lane)
+                                : panel_factor<T, NB>(a, ld, n, j0, jb, perm, lane);
+      if (lane == 0 && sing) *sh_flag = 1;
+    }
+    __syncthreads();
+    const int r0 = j0 + jb;
+    if (r0 >= n) break;
+    panel_row_solve<T, NB>(a, ld, n, j0, jb, r0);
+    __syncthreads();
+    trailing_update<NB>(a, ld, j0, jb, r0, n);
+    __syncthreads();
+  }
+  const int sing = *sh_flag;
+  __syncthreads();
+  return sing;
+}
+
+// y = U^-1 L^-1 (P rhs) with the factors in LDS, by one wavefront, in the
+// factors' precision: the right-hand side is scaled by a power of two into
+// range first.  Lane l keeps entries l, l + 64, l + 128, l + 192 in
+// registers.  The 64-step blocks are unrolled over the slot index, so each
+// step is a readlane, one masked update of its own slot and a plain update
+// of the later (earlier, going back) slots; the next column's LDS reads are
+// issued before the current update.
+template <class T>
+__device__ void tri_solve_wave(const T* a, int ld, int n, const int* perm, const double* rhs, double* y, int lane)
+{
+  const int ns = (n + WAVE - 1) / WAVE;
+  double rm = 0;
+  for (int i = lane; i < n; i += WAVE) rm = fmax(rm, fabs(rhs[i]));
+  rm = wave_max(rm);
+  int e = 0;
+  if (rm > 0 && rm < HUGE_VAL) frexp(rm, &e);
+  const double sc = ldexp(1.0, -e), usc = ldexp(1.0, e);
+  constexpr bool recip = sizeof(T) == sizeof(float);
+  T v[NLTE_SLOTS], cur[NLTE_SLOTS], nxt[NLTE_SLOTS], dg[NLTE_SLOTS];
+  int row[NLTE_SLOTS];
+#pragma unroll
+  for (int r = 0; r < NLTE_SLOTS; ++r) {
+    const int i = lane + WAVE * r;
+    const int ic = i < n ? i : n - 1;        // rows past n read row n-1; never used
+    row[r] = ic * ld;
+    v[r] = i < n ? (T)(rhs[perm[i]] * sc) : (T)0;
+    dg[r] = a[row[r] + ic];
+  }
+  if (recip) {
+    // FP32: the pivots' reciprocals, once (the refinement absorbs their rounding)
+#pragma unroll
+    for (int r = 0; r < NLTE_SLOTS; ++r) dg[r] = (T)1 / dg[r];
+  }
+#pragma unroll
+  for (int rk = 0; rk < NLTE_SLOTS; ++rk) {   // forward, unit lower
+    if (rk < ns) {
+      const int k0 = rk * WAVE, cnt = min(WAVE, n - k0);
+#pragma unroll
+      for (int r = rk; r < NLTE_SLOTS; ++r) cur[r] = a[row[r] + k0];
+      for (int src = 0; src < cnt; ++src) {
+        const int kn = min(k0 + src + 1, n - 1);
+#pragma unroll
+        for (int r = rk; r < NLTE_SLOTS; ++r) nxt[r] = a[row[r] + kn];
+        const T xk = readlane(v[rk], src);
+        const T u = v[rk] - cur[rk] * xk;
+        v[rk] = lane > src ? u : v[rk];
+#pragma unroll
+        for (int r = rk + 1; r < NLTE_SLOTS; ++r) v[r] -= cur[r] * xk;
+#pragma unroll
+        for (int r = rk; r < NLTE_SLOTS; ++r) cur[r] = nxt[r];
+      }
+    }
+  }
+#pragma unroll
+  for (int rk = NLTE_SLOTS - 1; rk >= 0; --rk) {   // backward
+    if (rk < ns) {
+      const int k0 = rk * WAVE, cnt = min(WAVE, n - k0);
+#pragma unroll
+      for (int r = 0; r <= rk; ++r) cur[r] = a[row[r] + k0 + cnt - 1];
+      for (int src = cnt - 1; src >= 0; --src) {
+        const int kn = max(k0 + src - 1, 0);
+#pragma unroll
+        for (int r = 0; r <= rk; ++r) nxt[r] = a[row[r] + kn];
+        T xk;
+        if (recip) xk = readlane(v[rk] * dg[rk], src);
+        else xk = readlane(v[rk], src) / readlane(dg[rk], src);
+        const T u = v[rk] - cur[rk] * xk;
+        v[rk] = lane < src ? u : (lane == src ? xk : v[rk]);
+#pragma unroll
+        for (int r = 0; r < rk; ++r) v[r] -= cur[r] * xk;
+#pragma unroll
+        for (int r = 0; r <= rk; ++r) cur[r] = nxt[r];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NLTE_SLOTS; ++r) {
+    const int i = lane + WAVE * r;
+    if (i < n) y[i] = (double)v[r] * usc;
+  }
+}
+
+// Global-memory passes over a system's rates, NT threads, loads issued 16
+// at a time so that each thread keeps 16 in flight.
+constexpr int NLTE_UNROLL = 16;
+
+// diag_i = -sum_j R_ij, summed in j order (one thread per row)
+__device__ void row_sums(int n, const double* R, double* diag)
+{
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double* row = R + (size_t)i * n;
+    double acc = 0.0;
+    int j = 0;
+    for (; j + NLTE_UNROLL <= n; j += NLTE_UNROLL) {
+      double t[NLTE_UNROLL];
+#pragma unroll
+      for (int u = 0; u < NLTE_UNROLL; ++u) t[u] = row[j + u];
+#pragma unroll
+      for (int u = 0; u < NLTE_UNROLL; ++u) acc += t[u];
+    }
+    for (; j < n; ++j) acc += row[j];
+    diag[i] = -1 * acc;
+  }
+}
+
+__device__ __forceinline__ double pow2_scale(double mx)
+{
+  int e;
+  frexp(mx > 0 ? mx : 1.0, &e);
+  return ldexp(1.0, -e);
+}
+
+// rs_i = 2^-e(max_j |M_ij|): M_ii = diag_i, M_ij = R_ji, M_{n-1,j} = lte_j
+// (one thread per row of M = column of R: coalesced reads)
+__device__ void row_scales(int n, const double* R, const double* lte, const double* diag, double* rs)
+{
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double mx = 0;
+    if (i == n - 1) {
+      for (int j = 0; j < n; ++j) mx = fmax(mx, fabs(lte[j]));
+    } else {
+      int j = 0;
+      for (; j + NLTE_UNROLL <= n; j += NLTE_UNROLL) {
+        double t[NLTE_UNROLL];
+#pragma unroll
+        for (int u = 0; u < NLTE_UNROLL; ++u) t[u] = R[(size_t)(j + u) * n + i];
+#pragma unroll
+        for (int u = 0; u < NLTE_UNROLL; ++u) mx = fmax(mx, j + u == i ? 0.0 : fabs(t[u]));
+      }
+      for (; j < n; ++j) mx = fmax(mx, j == i ? 0.0 : fabs(R[(size_t)j * n + i]));
+      mx = fmax(mx, fabs(diag[i]));
+    }
+    rs[i] = pow2_scale(mx);
+  }
+}
+
+// cs_j = 2^-e(max_i |M_ij| rs_i) (one thread per column of M = row of R)
+__device__ void col_scales(int n, const double* R, const double* lte, const double* diag, const double* rs,
+                           double* cs)
+{
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const double* row = R + (size_t)j * n;
+    double mx = fmax(fabs(lte[j]) * rs[n - 1], j < n - 1 ? fabs(diag[j]) * rs[j] : 0.0);
+    int i = 0;
+    for (; i + NLTE_UNROLL <= n - 1; i += NLTE_UNROLL) {
+      double t[NLTE_UNROLL], w[NLTE_UNROLL];
+#pragma unroll
+      for (int u = 0; u < NLTE_UNROLL; ++u) { t[u] = row[i + u]; w[u] = rs[i + u]; }
+#pragma unroll
+      for (int u = 0; u < NLTE_UNROLL; ++u) mx = fmax(mx, i + u == j ? 0.0 : fabs(t[u]) * w[u]);
+    }
+    for (; i < n - 1; ++i) mx = fmax(mx, i == j ? 0.0 : fabs(row[i]) * rs[i]);
+    cs[j] = pow2_scale(mx);
+  }
+}
+
+// M (scaled by rs_i cs_j when given) into LDS at row stride ld, T precision,
+// transposed from the rows of R: thread k takes the flat indices k, k + NT,
+// ... of R (coalesced), 8 loads in flight
+template <class T>
+__device__ void fill_lds(int n, const double* R, const double* lte, const double* diag, const double* rs,
+                         const double* cs, T* a, int ld)
+{
+  const int NT = blockDim.x, nn = n * n;
+  int jj = threadIdx.x / n, ii = threadIdx.x - jj * n;   // R[jj][ii] = M[ii][jj]
+  const int dj = NT / n, di = NT - dj * n;
+  for (int base = threadIdx.x; base < nn; base += 8 * NT) {
+    double t[8];
+    int ri[8], rj[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      ri[u] = ii;
+      rj[u] = jj;
+      t[u] = base + u * NT < nn ? R[(size_t)jj * n + ii] : 0.0;
+      ii += di;
+      jj += dj;
+      if (ii >= n) { ii -= n; ++jj; }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (base + u * NT < nn) {
+        const int i = ri[u], j = rj[u];
+        double v = (i == n - 1) ? lte[j] : (i == j ? diag[i] : t[u]);
+        if (rs) v = v * rs[i] * cs[j];
+        a[i * ld + j] = (T)v;
+      }
+    }
+  }
+}
+
+// r_i = (e_{n-1} - M x)_i rs_i, one thread per row of M (coalesced reads of
+// the columns of R), 16 loads in flight
+__device__ void residual(int n, const double* R, const double* lte, const double* diag, const double* rs,
+                        const double* x, double* r)
+{
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double acc = 0;
+    int j = 0;
+    if (i == n - 1) {
+      for (; j + NLTE_UNROLL <= n; j += NLTE_UNROLL) {
+        double t[NLTE_UNROLL], w[NLTE_UNROLL];
+#pragma unroll
+        for (int u = 0; u < NLTE_UNROLL; ++u) { t[u] = lte[j + u]; w[u] = x[j + u]; }
+#pragma unroll
+        for (int u = 0; u < NLTE_UNROLL; ++u) acc += t[u] * w[u];
+      }
+      for (; j < n; ++j) acc += lte[j] * x[j];
+      acc = 1.0 - acc;
+    } else {
+      acc = diag[i] * x[i];
+      for (; j + NLTE_UNROLL <= n; j += NLTE_UNROLL) {
+        double t[NLTE_UNROLL], w[NLTE_UNROLL];
+#pragma unroll
+        for (int u = 0; u < NLTE_UNROLL; ++u) { t[u] = R[(size_t)(j + u) * n + i]; w[u] = x[j + u]; }
+#pragma unroll
+        for (int u = 0; u < NLTE_UNROLL; ++u) acc += j + u == i ? 0.0 : t[u] * w[u];
+      }
+      for (; j < n; ++j) if (j != i) acc += R[(size_t)j * n + i] * x[j];
+      acc = -acc;
+    }
+    r[i] = acc * rs[i];
+  }
+}
+
+// Scaled system: As = Dr M Dc with Dr, Dc powers of two; M b = e becomes
+// As y = Dr e, b = Dc y.  The FP32 factors of As sit in LDS (row stride
+// n | 1); the pivot permutation and the FP64 vectors in the workgroup's slot.
+template <int NB>
+__global__ void __launch_bounds__(NLTE_MAX_THREADS) nlte_batched_mixed_kernel(BatchArgs A, size_t lds_bytes)
+{
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  float* lds_a = (float*)lds_raw;
+  __shared__ ArgMax red[NLTE_MAX_THREADS / WAVE];
+  __shared__ int sh, sing_f64, flag;
+  double* slot = A.slots + (size_t)blockIdx.x * A.slot_stride;
+  const int tid = threadIdx.x;
+  const int wave = tid / WAVE, lane = tid % WAVE;
+  (void)lds_bytes;
   while (true) {
     const int s = next_system(A, &sh);
     if (s < 0) break;
     const int n = A.n[s];
+    const int ld = lds_ld(n);
     const double* R = A.rates + A.rate_off[s];
     const double* lte = A.lte + A.vec_off[s];
     double* bout = A.b + A.vec_off[s];
@@ -431,81 +945,55 @@ __global__ void __launch_bounds__(NLTE_THREADS) nlte_batched_mixed_kernel(BatchA
     double* d = r + n;
     double* tmp = d + n;
     int* perm = (int*)(tmp + n);
-    double* M64 = (double*)(perm + 2 * ((n + 1) / 2)) ;
+    double* M64 = (double*)(perm + 2 * ((n + 1) / 2));
     int it = 0;
     bool ok = false;
-    if (n <= NLTE_LDS_MAX_N) {
-      // diagonal and equilibration: rs_i = 2^-e(max_j |M_ij|)
-      for (int i = tid; i < n; i += NT) {
-        double Rout = 0.0;
-        for (int j = 0; j < n; ++j) Rout += R[(size_t)i * n + j];
-        diag[i] = -1 * Rout;
-      }
+    if (n <= LdsCap<float>::n) {
+      // equilibration: rs_i = 2^-e(max_j |M_ij|), cs_j = 2^-e(max_i |M_ij| rs_i)
+      row_sums(n, R, diag);
       __syncthreads();
-      for (int i = tid; i < n; i += NT) {
-        double mx = 0;
-        if (i == n - 1) for (int j = 0; j < n; ++j) mx = fmax(mx, fabs(lte[j]));
-        else {
-          for (int j = 0; j < n; ++j) mx = fmax(mx, j == i ? fabs(diag[i]) : fabs(R[(size_t)j * n + i]));
-        }
-        int e;
-        frexp(mx > 0 ? mx : 1.0, &e);
-        rs[i] = ldexp(1.0, -e);
-      }
+      row_scales(n, R, lte, diag, rs);
       __syncthreads();
-      for (int j = tid; j < n; j += NT) {
-        double mx = 0;
-        for (int i = 0; i < n; ++i) {
-          double v = (i == n - 1) ? lte[j] : (i == j ? diag[i] : R[(size_t)j * n + i]);
-          mx = fmax(mx, fabs(v) * rs[i]);
-        }
-        int e;
-        frexp(mx > 0 ? mx : 1.0, &e);
-        cs[j] = ldexp(1.0, -e);
-      }
+      col_scales(n, R, lte, diag, rs, cs);
       __syncthreads();
-      for (int idx = tid; idx < n * n; idx += NT) {
-        int i = idx / n, j = idx - i * n;
-        double v = (i == n - 1) ? lte[j] : (i == j ? diag[i] : R[(size_t)j * n + i]);
-        lds_a[idx] = (float)(v * rs[i] * cs[j]);
-      }
+      fill_lds<float>(n, R, lte, diag, rs, cs, lds_a, ld);
       __syncthreads();
-      int sing = lu_f32_lds(lds_a, n, perm, nb, red);
+      const int sing = lu_lds<float, NB>(lds_a, ld, n, perm, &flag);
       if (!sing) {
         // x = Dc As^-1 Dr e_{n-1}
-        for (int i = tid; i < n; i += NT) r[i] = (i == n - 1) ? rs[i] : 0.0;
-        __syncthreads();
-        lu_f32_apply(lds_a, n, perm, r, d, tmp);
-        for (int i = tid; i < n; i += NT) x[i] = d[i] * cs[i];
+        if (wave == 0) {
+          for (int i = lane; i < n; i += WAVE) r[i] = (i == n - 1) ? rs[i] : 0.0;
+          wave_fence();
+          tri_solve_wave(lds_a, ld, n, perm, r, d, lane);
+          wave_fence();
+          for (int i = lane; i < n; i += WAVE) x[i] = d[i] * cs[i];
+        }
         __syncthreads();
         for (it = 1; it <= NLTE_MAX_ITERS; ++it) {
           // FP64 residual r = e_{n-1} - M x, scaled by Dr
-          for (int i = tid; i < n; i += NT) {
-            double acc;
-            if (i == n - 1) {
-              acc = 0;
-              for (int j = 0; j < n; ++j) acc += lte[j] * x[j];
-              acc = 1.0 - acc;
-            } else {
-              acc = diag[i] * x[i];
-              for (int j = 0; j < n; ++j) if (j != i) acc += R[(size_t)j * n + i] * x[j];
-              acc = -acc;
+          residual(n, R, lte, diag, rs, x, r);
+          __syncthreads();
+          if (wave == 0) {
+            tri_solve_wave(lds_a, ld, n, perm, r, d, lane);
+            wave_fence();
+            double dmax = 0, xmax = 0;
+            for (int i = lane; i < n; i += WAVE) {
+              const double dx = d[i] * cs[i];
+              const double xi = x[i] + dx;
+              x[i] = xi;
+              dmax = fmax(dmax, fabs(dx));
+              xmax = fmax(xmax, fabs(xi));
             }
-            r[i] = acc * rs[i];
+            dmax = wave_max(dmax);
+            xmax = wave_max(xmax);
+            // 1: converged, 2: NaN (give up on FP32), 0: continue
+            if (lane == 0) flag = (!(dmax == dmax) || !(xmax == xmax)) ? 2 : (dmax <= A.tol * xmax ? 1 : 0);
           }
           __syncthreads();
-          lu_f32_apply(lds_a, n, perm, r, d, tmp);
-          double dmax = 0, xmax = 0;
-          for (int i = tid; i < n; i += NT) {
-            double dx = d[i] * cs[i];
-            x[i] += dx;
-            dmax = fmax(dmax, fabs(dx));
-            xmax = fmax(xmax, fabs(x[i]));
-          }
-          dmax = block_max(dmax, dred);
-          xmax = block_max(xmax, dred);
-          if (!(dmax == dmax) || !(xmax == xmax)) break;   // NaN: give up on FP32
-          if (dmax <= A.tol * xmax) { ok = true; break; }
+          const int f = flag;
+          __syncthreads();
+          if (f == 2) break;
+          if (f == 1) { ok = true; break; }
         }
       }
     }
@@ -520,6 +1008,55 @@ __global__ void __launch_bounds__(NLTE_THREADS) nlte_batched_mixed_kernel(BatchA
       write_pops(n, x, lte, bout, pout);
       if (A.iters && tid == 0) A.iters[s] = -1;
     }
+    __syncthreads();
+  }
+}
+
+// FP64 LU in LDS (n <= 143) for the FP64 mode; larger systems in the
+// workgroup's device slot
+template <int NB>
+__global__ void __launch_bounds__(NLTE_MAX_THREADS) nlte_batched_f64_lds_kernel(BatchArgs A, size_t lds_bytes)
+{
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double* lds_d = (double*)lds_raw;
+  __shared__ ArgMax red[NLTE_MAX_THREADS / WAVE];
+  __shared__ int sh, sing_f64, flag;
+  double* slot = A.slots + (size_t)blockIdx.x * A.slot_stride;
+  const int tid = threadIdx.x;
+  const int wave = tid / WAVE, lane = tid % WAVE;
+  (void)lds_bytes;
+  while (true) {
+    const int s = next_system(A, &sh);
+    if (s < 0) break;
+    const int n = A.n[s];
+    const double* R = A.rates + A.rate_off[s];
+    const double* lte = A.lte + A.vec_off[s];
+    // slot layout: diag[n] | x[n] | r[n] | perm (ints) | M64 (larger systems)
+    double* diag = slot;
+    double* x = diag + n;
+    double* r = x + n;
+    int* perm = (int*)(r + n);
+    double* M64 = (double*)(perm + 2 * ((n + 1) / 2));
+    if (n <= LdsCap<double>::n) {
+      const int ld = lds_ld(n);
+      row_sums(n, R, diag);
+      __syncthreads();
+      fill_lds<double>(n, R, lte, diag, nullptr, nullptr, lds_d, ld);
+      __syncthreads();
+      lu_lds<double, NB>(lds_d, ld, n, perm, &flag);
+      if (wave == 0) {
+        for (int i = lane; i < n; i += WAVE) r[i] = (i == n - 1) ? 1.0 : 0.0;
+        wave_fence();
+        tri_solve_wave(lds_d, ld, n, perm, r, x, lane);
+      }
+      __syncthreads();
+    } else {
+      if (tid == 0) sing_f64 = 0;
+      assemble<double>(n, R, lte, M64, n);
+      lu_solve_f64(n, M64, perm, x, red, &sing_f64);
+    }
+    write_pops(n, x, lte, A.b + A.vec_off[s], A.pop ? A.pop + A.vec_off[s] : nullptr);
+    if (A.iters && tid == 0) A.iters[s] = 0;
     __syncthreads();
   }
 }
@@ -542,6 +1079,22 @@ extern "C" int sgpu_nlte_solve(sgpu_ctx* ctx, int32_t n, const double* rates, co
 
 namespace {
 
+// the batched kernel for a precision / panel width / largest system, and
+// its dynamic LDS (0 for the device-memory FP64 kernel)
+const void* batched_kernel(int precision, int panel, int max_n, size_t* lds)
+{
+  if (precision == SGPU_NLTE_FP64) {
+    if (max_n > LdsCap<double>::n) {
+      *lds = 0;
+      return (const void*)nlte_batched_f64_kernel;
+    }
+    *lds = lds_bytes_for<double>(max_n);
+    return panel == 8 ? (const void*)nlte_batched_f64_lds_kernel<8> : (const void*)nlte_batched_f64_lds_kernel<16>;
+  }
+  *lds = lds_bytes_for<float>(std::min(max_n, LdsCap<float>::n));
+  return panel == 8 ? (const void*)nlte_batched_mixed_kernel<8> : (const void*)nlte_batched_mixed_kernel<16>;
+}
+
 int batched_impl(sgpu_ctx* ctx, int32_t nsys, const int32_t* n, const int64_t* rate_off, const int64_t* vec_off,
                  const double* rates, const double* lte, double* b, double* pop, int32_t precision, double tol,
                  int32_t* iters, int threads, int panel, int wg_per_cu)
@@ -563,6 +1116,10 @@ int batched_impl(sgpu_ctx* ctx, int32_t nsys, const int32_t* n, const int64_t* r
   SGPU_HIP(hipStreamSynchronize(ctx->stream));
   SGPU_REQUIRE(hmm[0] >= 1 && hmm[1] <= 1024, "system sizes must lie in [1, 1024] (got %d..%d)", hmm[0], hmm[1]);
   const int max_n = hmm[1];
+  SGPU_REQUIRE(threads >= 2 * WAVE && threads <= NLTE_MAX_THREADS && threads % WAVE == 0,
+               "workgroup size %d must be a multiple of %d in [%d, %d]", threads, WAVE, 2 * WAVE, NLTE_MAX_THREADS);
+  SGPU_REQUIRE((panel == 8 || panel == 16) && wg_per_cu >= 1,
+               "panel width %d (8 or 16) / workgroups per CU %d", panel, wg_per_cu);
   const unsigned nwg = (unsigned)std::min<int64_t>((int64_t)ctx->n_cu * wg_per_cu, nsys);
   // slot: 7 vectors + perm + an FP64 matrix for the fallback / FP64 mode
   const size_t slot = (size_t)max_n * max_n + 8 * (size_t)max_n + 64;
@@ -572,14 +1129,11 @@ int batched_impl(sgpu_ctx* ctx, int32_t nsys, const int32_t* n, const int64_t* r
   SGPU_LAUNCH_CHECK();
   BatchArgs A{nsys, n, rate_off, vec_off, rates, lte, b, pop, iters, order, counter, slots, slot, precision, tol};
   timer_begin(ctx, "nlte");
-  if (precision == SGPU_NLTE_FP64) {
-    hipLaunchKernelGGL(nlte_batched_f64_kernel, dim3(nwg), dim3(threads), 0, ctx->stream, A);
-  } else {
-    const size_t lds = sizeof(float) * NLTE_LDS_MAX_N * NLTE_LDS_MAX_N;
-    SGPU_HIP(hipFuncSetAttribute((const void*)nlte_batched_mixed_kernel,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(nlte_batched_mixed_kernel, dim3(nwg), dim3(threads), lds, ctx->stream, A, panel);
-  }
+  size_t lds = 0;
+  const void* fn = batched_kernel(precision, panel, max_n, &lds);
+  if (lds) SGPU_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  void* args[] = {&A, &lds};
+  SGPU_HIP(hipLaunchKernel(fn, dim3(nwg), dim3(threads), args, lds, ctx->stream));
   SGPU_LAUNCH_CHECK();
   timer_end(ctx, "nlte");
   return SGPU_OK;
@@ -592,52 +1146,99 @@ extern "C" int sgpu_nlte_batched(sgpu_ctx* ctx, int32_t nsys, const int32_t* n, 
                                  double* pop, int32_t precision, double tol, int32_t* iters)
 {
   SGPU_REQUIRE(ctx, "null context");
-  const nlte_tuning& t = ctx->nlte;
+  const nlte_tuning& t = ctx->nlte[precision == SGPU_NLTE_FP32_REFINE ? 1 : 0];
   return batched_impl(ctx, nsys, n, rate_off, vec_off, rates, lte, b, pop, precision, tol, iters,
                       t.valid ? t.threads : NLTE_THREADS, t.valid ? t.panel : 16, t.valid ? t.wg_per_cu : 1);
+}
+
+extern "C" int sgpu_nlte_set_tuning(sgpu_ctx* ctx, int32_t precision, int32_t threads, int32_t panel,
+                                    int32_t wg_per_cu)
+{
+  SGPU_REQUIRE(ctx, "null context");
+  SGPU_REQUIRE(precision == SGPU_NLTE_FP64 || precision == SGPU_NLTE_FP32_REFINE, "unknown precision %d",
+               precision);
+  SGPU_REQUIRE(threads >= 2 * WAVE && threads <= NLTE_MAX_THREADS && threads % WAVE == 0,
+               "workgroup size %d must be a multiple of %d in [%d, %d]", threads, WAVE, 2 * WAVE, NLTE_MAX_THREADS);
+  SGPU_REQUIRE((panel == 8 || panel == 16) && wg_per_cu >= 1 && wg_per_cu <= 16,
+               "panel width %d (8 or 16) / workgroups per CU %d (1..16)", panel, wg_per_cu);
+  nlte_tuning& t = ctx->nlte[precision == SGPU_NLTE_FP32_REFINE ? 1 : 0];
+  t.valid = 1;
+  t.threads = threads;
+  t.panel = panel;
+  t.wg_per_cu = wg_per_cu;
+  return SGPU_OK;
 }
 
 extern "C" int sgpu_nlte_autotune(sgpu_ctx* ctx, int32_t nsys, const int32_t* n, const int64_t* rate_off,
                                   const int64_t* vec_off, const double* rates, const double* lte, double* b,
                                   double* pop, int32_t* iters, char* report, size_t report_len)
 {
-  SGPU_REQUIRE(ctx, "null context");
-  const int panels[] = {8, 16, 32};
-  const int wgs[] = {1, 2};
+  SGPU_REQUIRE(ctx && n, "null argument");
+  if (nsys <= 0) return SGPU_OK;
+  // the largest system decides the LDS a workgroup takes, and so how many
+  // workgroups can share a CU: candidates that cannot co-reside are dropped
+  int max_n = 0;
+  {
+    int st = 0;
+    int32_t* mm = (int32_t*)scratch(ctx, "nlte_mm", 64, &st);
+    if (st) return st;
+    hipLaunchKernelGGL(size_range_kernel, dim3(1), dim3(1024), 0, ctx->stream, n, nsys, mm);
+    SGPU_LAUNCH_CHECK();
+    int32_t hmm[2];
+    SGPU_HIP(hipMemcpyAsync(hmm, mm, sizeof(hmm), hipMemcpyDeviceToHost, ctx->stream));
+    SGPU_HIP(hipStreamSynchronize(ctx->stream));
+    max_n = hmm[1];
+  }
+  const int threads_c[] = {128, 256, 512};
+  const int panels[] = {8, 16};
+  const int wgs[] = {1, 2, 4};
+  const int precs[] = {SGPU_NLTE_FP64, SGPU_NLTE_FP32_REFINE};
   hipEvent_t e0, e1;
   SGPU_HIP(hipEventCreate(&e0));
   SGPU_HIP(hipEventCreate(&e1));
-  float best = 1e30f;
-  int bp = 16, bw = 1;
   std::string rep;
-  for (int w : wgs)
-    for (int p : panels) {
-      // one warm-up, then the timed run
-      int st = batched_impl(ctx, nsys, n, rate_off, vec_off, rates, lte, b, pop, SGPU_NLTE_FP32_REFINE, 1e-12,
-                            iters, NLTE_THREADS, p, w);
-      if (st) return st;
-      SGPU_HIP(hipEventRecord(e0, ctx->stream));
-      st = batched_impl(ctx, nsys, n, rate_off, vec_off, rates, lte, b, pop, SGPU_NLTE_FP32_REFINE, 1e-12, iters,
-                        NLTE_THREADS, p, w);
-      if (st) return st;
-      SGPU_HIP(hipEventRecord(e1, ctx->stream));
-      SGPU_HIP(hipEventSynchronize(e1));
-      float ms = 0;
-      SGPU_HIP(hipEventElapsedTime(&ms, e0, e1));
-      char line[128];
-      snprintf(line, sizeof(line), "panel=%d wg_per_cu=%d: %.4f ms\n", p, w, ms);
-      rep += line;
-      if (ms < best) { best = ms; bp = p; bw = w; }
-    }
+  for (int pi = 0; pi < 2; ++pi) {
+    const int prec = precs[pi];
+    float best = 1e30f;
+    int bt = NLTE_THREADS, bp = 16, bw = 1;
+    for (int th : threads_c)
+      for (int p : panels)
+        for (int w : wgs) {
+          // only shapes whose workgroups can share a CU (LDS and registers)
+          size_t lds = 0;
+          const void* fn = batched_kernel(prec, p, max_n, &lds);
+          if (lds) SGPU_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+          int resident = 0;
+          SGPU_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, fn, th, lds));
+          if (w > resident) continue;
+          // one warm-up, then the timed run
+          int st = batched_impl(ctx, nsys, n, rate_off, vec_off, rates, lte, b, pop, prec, 1e-12, iters, th, p, w);
+          if (st) return st;
+          SGPU_HIP(hipEventRecord(e0, ctx->stream));
+          st = batched_impl(ctx, nsys, n, rate_off, vec_off, rates, lte, b, pop, prec, 1e-12, iters, th, p, w);
+          if (st) return st;
+          SGPU_HIP(hipEventRecord(e1, ctx->stream));
+          SGPU_HIP(hipEventSynchronize(e1));
+          float ms = 0;
+          SGPU_HIP(hipEventElapsedTime(&ms, e0, e1));
+          char line[160];
+          snprintf(line, sizeof(line), "%s threads=%d panel=%d wg_per_cu=%d: %.4f ms\n",
+                   prec == SGPU_NLTE_FP64 ? "fp64" : "fp32_refine", th, p, w, ms);
+          rep += line;
+          if (ms < best) { best = ms; bt = th; bp = p; bw = w; }
+        }
+    nlte_tuning& t = ctx->nlte[pi];
+    t.valid = 1;
+    t.threads = bt;
+    t.panel = bp;
+    t.wg_per_cu = bw;
+    char line[160];
+    snprintf(line, sizeof(line), "chosen %s: threads=%d panel=%d wg_per_cu=%d (%.4f ms)\n",
+             prec == SGPU_NLTE_FP64 ? "fp64" : "fp32_refine", bt, bp, bw, best);
+    rep += line;
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  ctx->nlte.valid = 1;
-  ctx->nlte.threads = NLTE_THREADS;
-  ctx->nlte.panel = bp;
-  ctx->nlte.wg_per_cu = bw;
-  char line[128];
-  snprintf(line, sizeof(line), "chosen: panel=%d wg_per_cu=%d (%.4f ms)\n", bp, bw, best);
-  rep += line;
   if (report && report_len) {
     strncpy(report, rep.c_str(), report_len - 1);
     report[report_len - 1] = 0;

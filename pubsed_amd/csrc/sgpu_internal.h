@@ -33,7 +33,7 @@ struct sgpu_ctx {
   int timing = 0;
   std::map<std::string, sgpu_phase_timer> timers;
   std::map<std::string, sgpu_scratch> scratch;
-  nlte_tuning nlte;
+  nlte_tuning nlte[2];   // per sgpu_nlte_batched precision (FP64, FP32_REFINE)
   // HIP graph of a captured step (sgpu_graph_begin / sgpu_graph_end)
   int capturing = 0;
   hipGraphExec_t graph_exec = nullptr;

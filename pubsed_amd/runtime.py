@@ -576,6 +576,11 @@ class GpuBackend:
                                                 abi.ptr(b), abi.ptr(pop), precision, tol,
                                                 abi.ptr(iters)), "sgpu_nlte_batched")
 
+    def nlte_set_tuning(self, precision, threads, panel, wg_per_cu):
+        from . import check
+        return check(self.lib.sgpu_nlte_set_tuning(self.ctx, precision, threads, panel, wg_per_cu),
+                     "sgpu_nlte_set_tuning")
+
     def nlte_autotune(self, nsys, n, rate_off, vec_off, rates, lte, b, pop, iters):
         from . import check
         buf = ctypes.create_string_buffer(4096)

@@ -68,9 +68,11 @@ def test_single_system_drop_in(gpu):
         assert np.allclose(p, np.maximum(r * l.cpu().numpy(), 1e-30), rtol=1e-10, atol=0)
 
 
-@pytest.mark.parametrize("precision,bar", [(abi.NLTE_FP64, 1e-11), (abi.NLTE_FP32_REFINE, 1e-8)])
-def test_batched(gpu, precision, bar):
-    sy = W.nlte_systems(n_sys=96, seed=5)
+@pytest.mark.parametrize("precision,bar,n_max", [(abi.NLTE_FP64, 1e-11, 200), (abi.NLTE_FP64, 1e-11, 140),
+                                                 (abi.NLTE_FP32_REFINE, 1e-8, 200)])
+def test_batched(gpu, precision, bar, n_max):
+    """n_max 200: FP64 LU in device memory; 140: FP64 LU in LDS (n <= 143)."""
+    sy = W.nlte_systems(n_sys=96, n_max=n_max, seed=5)
     ref = oracle_solutions(sy)
     d = to_dev(sy)
     print("case nlte batched", precision, flush=True)
@@ -86,9 +88,36 @@ def test_batched(gpu, precision, bar):
         assert (it > 0).all(), "some systems fell back to FP64: %s" % it
 
 
+@pytest.mark.parametrize("precision,n_max,bar", [(abi.NLTE_FP64, 140, 1e-11), (abi.NLTE_FP32_REFINE, 201, 1e-8)])
+def test_every_launch_shape(gpu, precision, n_max, bar):
+    """Every shape the autotuner may pick (workgroup size x panel width x
+    workgroups per CU) solves the same uneven batch to the oracle's answer;
+    sizes straddle the panel widths and the MFMA tile edges."""
+    sy = W.nlte_systems(n_sys=40, n_min=2, n_max=n_max, seed=11)
+    ref = oracle_solutions(sy)
+    d = to_dev(sy)
+    try:
+        for threads in (128, 256, 512):
+            for panel in (8, 16):
+                for wg in (1, 2):
+                    print("case nlte shape", precision, threads, panel, wg, flush=True)
+                    gpu.nlte_set_tuning(precision, threads, panel, wg)
+                    d["b"].zero_()
+                    gpu.nlte_batched(sy["n_sys"], d["n"], d["rate_off"], d["vec_off"], d["rates"], d["lte"],
+                                     d["b"], None, precision, 1e-13, d["iters"])
+                    gpu.sync()
+                    errs = normwise(sy, d["b"].cpu().numpy(), ref)
+                    assert errs.max() <= bar, (threads, panel, wg, errs.max())
+                    if precision == abi.NLTE_FP32_REFINE:
+                        assert (d["iters"].cpu().numpy() > 0).all(), (threads, panel, wg)
+    finally:
+        gpu.nlte_set_tuning(precision, 256, 16, 1)
+
+
 def test_uneven_sizes_edge(gpu):
-    """Tiny, panel-straddling and LDS-limit sizes in one batch (n = 2 ... 202,
-    plus 230 which exceeds the LDS tile and takes the FP64 path)."""
+    """Tiny, panel-straddling and LDS-limit sizes in one batch (n = 2 ... 201,
+    the largest FP32 matrix with an odd row stride in 160 KiB of LDS), plus 202
+    and 230, which exceed it and take the FP64 path."""
     rng = np.random.default_rng(3)
     sizes = [2, 3, 31, 32, 33, 64, 65, 97, 128, 160, 201, 202, 230]
     sy = W.nlte_systems(n_sys=len(sizes), seed=9)
@@ -109,8 +138,8 @@ def test_uneven_sizes_edge(gpu):
     it = d["iters"].cpu().numpy()
     print("errs", errs, "iters", it)
     assert errs.max() <= 1e-8
-    assert it[-1] == -1            # n = 230 > LDS limit: FP64 path
-    assert (it[:-1] > 0).all()
+    assert (it[-2:] == -1).all()    # n = 202, 230 > LDS limit: FP64 path
+    assert (it[:-2] > 0).all()
 
 
 def rate_case(nz=6, near_lte=False, seed=5):
