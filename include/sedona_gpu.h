@@ -537,7 +537,10 @@ int sgpu_nlte_solve(sgpu_ctx* ctx, int32_t n, const double* rates,
 /* All systems in one persistent launch.  System s has size n[s]; its rates
  * start at rates + rate_off[s] (n[s]^2 doubles, row-major) and its lte/b/pop
  * at vec_off[s].  precision: SGPU_NLTE_FP64 or SGPU_NLTE_FP32_REFINE.
- * tol: refinement stops when max|dx|/max|x| <= tol (FP32 mode).
+ * FP32 mode: refinement stops when max|dx|/max|x| <= tol, or, once the
+ * corrections no longer halve from one iteration to the next (the FP64
+ * residual's rounding floor), when max|dx|/max|x| <= 16 tol; at most 12
+ * corrections.
  * iters[s] (device, may be NULL) receives the refinement iterations used, or
  * -1 when the FP32 solve did not converge and the system was re-solved in FP64.
  * n[], rate_off[], vec_off[] are device arrays. */

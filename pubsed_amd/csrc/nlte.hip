@@ -302,11 +302,12 @@ constexpr int NLTE_SLOTS = 4;   // rows per lane in the one-wavefront steps (n <
 __host__ __device__ __forceinline__ int lds_ld(int n) { return n | 1; }
 
 // dynamic LDS of a batched launch whose largest LDS system has cap levels:
-// the matrix, and at least two staged FP64 rows for the equilibration
+// the matrix, and at least 4 cap FP64 partial results for the
+// equilibration passes (which run before the matrix is filled)
 template <class T>
 __host__ __device__ inline size_t lds_bytes_for(int cap)
 {
-  const size_t m = sizeof(T) * (size_t)cap * lds_ld(cap), st = sizeof(double) * 2 * (size_t)lds_ld(cap);
+  const size_t m = sizeof(T) * (size_t)cap * lds_ld(cap), st = sizeof(double) * 4 * (size_t)cap;
   return m > st ? m : st;
 }
 
@@ -382,6 +383,19 @@ __device__ __forceinline__ V wave_max(V v)
   return v;
 }
 
+// The pivot permutation lives in registers of the factoring wavefront:
+// lane l keeps perm[l + 64 r] in pv[r].  Interchange of entries j and ip
+// (uniform): two readlanes and a select per slot.
+__device__ __forceinline__ void perm_swap(int* pv, int j, int ip, int lane)
+{
+  const int sj = j / WAVE, lj = j % WAVE, sp = ip / WAVE, lp = ip % WAVE;
+  const int vj = __builtin_amdgcn_readlane(sj == 0 ? pv[0] : sj == 1 ? pv[1] : sj == 2 ? pv[2] : pv[3], lj);
+  const int vp = __builtin_amdgcn_readlane(sp == 0 ? pv[0] : sp == 1 ? pv[1] : sp == 2 ? pv[2] : pv[3], lp);
+#pragma unroll
+  for (int r = 0; r < NLTE_SLOTS; ++r)
+    pv[r] = (r == sj && lane == lj) ? vp : ((r == sp && lane == lp) ? vj : pv[r]);
+}
+
 // Panel j0 .. j0+jb of the partial-pivot LU, by one wavefront.  Per column
 // j: the pivot (first largest |a_ij|, i >= j) comes from registers -- the
 // candidates were formed by the previous column's update; one round trip
@@ -390,7 +404,7 @@ __device__ __forceinline__ V wave_max(V v)
 // are scaled by the pivot and updated in the panel's remaining columns.
 // Returns 1 when a pivot is exactly zero (that column is then left as is).
 template <class T, int NB>
-__device__ int panel_factor(T* a, int ld, int n, int j0, int jb, int* perm, int lane)
+__device__ __forceinline__ int panel_factor(T* a, int ld, int n, int j0, int jb, int* pv, int lane)
 {
   const int ns = (n + WAVE - 1) / WAVE;
   int singular = 0;
@@ -424,7 +438,7 @@ __device__ int panel_factor(T* a, int ld, int n, int j0, int jb, int* perm, int 
         const int k = lane + WAVE * r;
         if (r < ns && k < n) { a[j * ld + k] = y[r]; a[ip * ld + k] = x[r]; }
       }
-      if (lane == 0) { const int q = perm[j]; perm[j] = perm[ip]; perm[ip] = q; }
+      perm_swap(pv, j, ip, lane);
     }
     wave_fence();
     m = PivotOf<T>{(T)-1, n};
@@ -466,93 +480,108 @@ __device__ int panel_factor(T* a, int ld, int n, int j0, int jb, int* perm, int 
 }
 
 // The same for a full panel (jb == NB) with every column's shape known at
-// compile time: no per-element guards, so one wavefront issues about
-// 40 + 5 (NB - t) instructions per row slot and column instead of a guarded
-// sequence per element.
+// compile time (the column index is a template parameter): no per-element
+// guards, one wavefront issues about 40 + 5 (NB - t) instructions per row
+// slot and column.
 template <class T, int NB>
-__device__ int panel_factor_full(T* a, int ld, int n, int j0, int* perm, int lane)
-{
-  int singular = 0;
+struct LdsPanel {
+  T* a;
+  int ld, n, j0, lane;
   int rowi[NLTE_SLOTS];
   bool inb[NLTE_SLOTS];
+  int singular;
+};
+
+template <class T, int NB, int t>
+__device__ __forceinline__ void lds_panel_col(LdsPanel<T, NB>& S, PivotOf<T> m, int* pv)
+{
+  T* a = S.a;
+  const int ld = S.ld, n = S.n, lane = S.lane, j = S.j0 + t;
+  constexpr int mc = NB - 1 - t;
+  const int ip = wave_argmax(m).i;
+  const bool swap = ip != j && ip < n;
+  const int pr = (ip < n ? ip : j) * ld;
+  T x[NLTE_SLOTS], y[NLTE_SLOTS], u[NB];
+  if (swap) {
 #pragma unroll
-  for (int r = 0; r < NLTE_SLOTS; ++r) {
-    const int i = lane + WAVE * r;
-    inb[r] = i < n;
-    rowi[r] = (i < n ? i : n - 1) * ld;
+    for (int r = 0; r < NLTE_SLOTS; ++r) {
+      const int k = lane + WAVE * r;
+      if (k < n) { x[r] = a[j * ld + k]; y[r] = a[ip * ld + k]; }
+    }
   }
-  PivotOf<T> m = {(T)-1, n};
+  const T piv = a[pr + j];
 #pragma unroll
-  for (int r = 0; r < NLTE_SLOTS; ++r) {
-    const int i = lane + WAVE * r;
-    if (inb[r] && i >= j0) m = better_p(m, PivotOf<T>{(T)fabs(a[rowi[r] + j0]), i});
+  for (int c = 0; c < mc; ++c) u[c] = a[pr + j + 1 + c];
+  if (swap) {
+#pragma unroll
+    for (int r = 0; r < NLTE_SLOTS; ++r) {
+      const int k = lane + WAVE * r;
+      if (k < n) { a[j * ld + k] = y[r]; a[ip * ld + k] = x[r]; }
+    }
+    perm_swap(pv, j, ip, lane);
   }
-#pragma unroll
-  for (int t = 0; t < NB; ++t) {
-    const int j = j0 + t;
-    const int ip = wave_argmax(m).i;
-    const bool swap = ip != j && ip < n;
-    const int pr = (ip < n ? ip : j) * ld;
-    T x[NLTE_SLOTS], y[NLTE_SLOTS], u[NB];
-    if (swap) {
+  wave_fence();
+  PivotOf<T> mn = {(T)-1, n};
+  if (piv == (T)0) {
+    S.singular = 1;
+    if (mc > 0) {
 #pragma unroll
       for (int r = 0; r < NLTE_SLOTS; ++r) {
-        const int k = lane + WAVE * r;
-        if (k < n) { x[r] = a[j * ld + k]; y[r] = a[ip * ld + k]; }
+        const int i = lane + WAVE * r;
+        if (S.inb[r] && i > j) mn = better_p(mn, PivotOf<T>{(T)fabs(a[S.rowi[r] + j + 1]), i});
       }
     }
-    const T piv = a[pr + j];
-#pragma unroll
-    for (int c = 0; c < NB - 1 - t; ++c) u[c] = a[pr + j + 1 + c];
-    if (swap) {
-#pragma unroll
-      for (int r = 0; r < NLTE_SLOTS; ++r) {
-        const int k = lane + WAVE * r;
-        if (k < n) { a[j * ld + k] = y[r]; a[ip * ld + k] = x[r]; }
-      }
-      if (lane == 0) { const int q = perm[j]; perm[j] = perm[ip]; perm[ip] = q; }
-    }
-    wave_fence();
-    m = PivotOf<T>{(T)-1, n};
-    if (piv == (T)0) {
-      singular = 1;
-      if (t + 1 < NB) {
-#pragma unroll
-        for (int r = 0; r < NLTE_SLOTS; ++r) {
-          const int i = lane + WAVE * r;
-          if (inb[r] && i > j) m = better_p(m, PivotOf<T>{(T)fabs(a[rowi[r] + j + 1]), i});
-        }
-      }
-      continue;
-    }
+  } else {
     const T inv = (T)1 / piv;
 #pragma unroll
     for (int r = 0; r < NLTE_SLOTS; ++r) {
       const int i = lane + WAVE * r;
-      if (inb[r] && i > j) {
+      if (S.inb[r] && i > j) {
         T sg[NB];
-        T l = a[rowi[r] + j];
+        T l = a[S.rowi[r] + j];
 #pragma unroll
-        for (int c = 0; c < NB - 1 - t; ++c) sg[c] = a[rowi[r] + j + 1 + c];
+        for (int c = 0; c < mc; ++c) sg[c] = a[S.rowi[r] + j + 1 + c];
         l *= inv;
-        a[rowi[r] + j] = l;
+        a[S.rowi[r] + j] = l;
 #pragma unroll
-        for (int c = 0; c < NB - 1 - t; ++c) {
+        for (int c = 0; c < mc; ++c) {
           sg[c] -= l * u[c];
-          a[rowi[r] + j + 1 + c] = sg[c];
+          a[S.rowi[r] + j + 1 + c] = sg[c];
         }
-        if (t + 1 < NB) m = better_p(m, PivotOf<T>{(T)fabs(sg[0]), i});
+        if (mc > 0) mn = better_p(mn, PivotOf<T>{(T)fabs(sg[0]), i});
       }
     }
-    wave_fence();
   }
-  return singular;
+  wave_fence();
+  if constexpr (t + 1 < NB) lds_panel_col<T, NB, t + 1>(S, mn, pv);
+}
+
+template <class T, int NB>
+__device__ __forceinline__ int panel_factor_full(T* a, int ld, int n, int j0, int* pv, int lane)
+{
+  LdsPanel<T, NB> S;
+  S.a = a;
+  S.ld = ld;
+  S.n = n;
+  S.j0 = j0;
+  S.lane = lane;
+  S.singular = 0;
+  PivotOf<T> m = {(T)-1, n};
+#pragma unroll
+  for (int r = 0; r < NLTE_SLOTS; ++r) {
+    const int i = lane + WAVE * r;
+    S.inb[r] = i < n;
+    S.rowi[r] = (i < n ? i : n - 1) * ld;
+    if (S.inb[r] && i >= j0) m = better_p(m, PivotOf<T>{(T)fabs(a[S.rowi[r] + j0]), i});
+  }
+  lds_panel_col<T, NB, 0>(S, m, pv);
+  return S.singular;
 }
 
 // U12 = L11^-1 A12 (unit lower triangular), one column per thread, the
 // column held in registers
 template <class T, int NB>
-__device__ void panel_row_solve(T* a, int ld, int n, int j0, int jb, int r0)
+__device__ __forceinline__ void panel_row_solve(T* a, int ld, int n, int j0, int jb, int r0)
 {
   for (int c = r0 + (int)threadIdx.x; c < n; c += blockDim.x) {
     T col[NB];
@@ -579,7 +608,7 @@ __device__ void panel_row_solve(T* a, int ld, int n, int j0, int jb, int r0)
 // run on MFMA too.  Lane l supplies A[l%32][k + l/32] and B[k + l/32][l%32];
 // it holds accumulator column l%32, rows 8(i/4) + 4(l/32) + i%4, i < 16.
 template <int NB>
-__device__ void trailing_update(float* a, int ld, int j0, int jb, int r0, int n)
+__device__ __forceinline__ void trailing_update(float* a, int ld, int j0, int jb, int r0, int n)
 {
   typedef float f32x16 __attribute__((ext_vector_type(16)));
   const int m = n - r0;
@@ -621,7 +650,7 @@ __device__ void trailing_update(float* a, int ld, int j0, int jb, int r0, int n)
 // A[l%16][k + l/16] and B[k + l/16][l%16]; it holds accumulator column l%16,
 // rows l/16 + 4v, v < 4
 template <int NB>
-__device__ void trailing_update(double* a, int ld, int j0, int jb, int r0, int n)
+__device__ __forceinline__ void trailing_update(double* a, int ld, int j0, int jb, int r0, int n)
 {
   typedef double f64x4 __attribute__((ext_vector_type(4)));
   const int m = n - r0;
@@ -662,18 +691,19 @@ __device__ void trailing_update(double* a, int ld, int j0, int jb, int r0, int n
 // blocked right-looking partial-pivot LU in LDS (panel width NB), three
 // workgroup barriers per panel; returns 1 when a pivot is exactly zero
 template <class T, int NB>
-__device__ int lu_lds(T* a, int ld, int n, int* perm, int* sh_flag)
+__device__ __forceinline__ int lu_lds(T* a, int ld, int n, int* pv, int* sh_flag)
 {
-  const int tid = threadIdx.x, NT = blockDim.x;
+  const int tid = threadIdx.x;
   const int wave = tid / WAVE, lane = tid % WAVE;
-  for (int i = tid; i < n; i += NT) perm[i] = i;
+#pragma unroll
+  for (int r = 0; r < NLTE_SLOTS; ++r) pv[r] = lane + WAVE * r;
   if (tid == 0) *sh_flag = 0;
   __syncthreads();
   for (int j0 = 0; j0 < n; j0 += NB) {
     const int jb = min(NB, n - j0);
     if (wave == 0) {
-      const int sing = jb == NB ? panel_factor_full<T, NB>(a, ld, n, j0, perm, lane)
-                                : panel_factor<T, NB>(a, ld, n, j0, jb, perm, lane);
+      const int sing = jb == NB ? panel_factor_full<T, NB>(a, ld, n, j0, pv, lane)
+                                : panel_factor<T, NB>(a, ld, n, j0, jb, pv, lane);
       if (lane == 0 && sing) *sh_flag = 1;
     }
     __syncthreads();
@@ -697,8 +727,11 @@ __device__ int lu_lds(T* a, int ld, int n, int* perm, int* sh_flag)
 // of the later (earlier, going back) slots; the next column's LDS reads are
 // issued before the current update.
 template <class T>
-__device__ void tri_solve_wave(const T* a, int ld, int n, const int* perm, const double* rhs, double* y, int lane)
+__device__ __forceinline__ void tri_solve_wave(const T* a, int ld, int n, const int* pv, const double* rhs, double* y,
+                                               int lane)
 {
+  n = __builtin_amdgcn_readfirstlane(n);     // uniform: scalar loop control
+  ld = __builtin_amdgcn_readfirstlane(ld);
   const int ns = (n + WAVE - 1) / WAVE;
   double rm = 0;
   for (int i = lane; i < n; i += WAVE) rm = fmax(rm, fabs(rhs[i]));
@@ -707,14 +740,14 @@ __device__ void tri_solve_wave(const T* a, int ld, int n, const int* perm, const
   if (rm > 0 && rm < HUGE_VAL) frexp(rm, &e);
   const double sc = ldexp(1.0, -e), usc = ldexp(1.0, e);
   constexpr bool recip = sizeof(T) == sizeof(float);
-  T v[NLTE_SLOTS], cur[NLTE_SLOTS], nxt[NLTE_SLOTS], dg[NLTE_SLOTS];
+  T v[NLTE_SLOTS], c0[NLTE_SLOTS], c1[NLTE_SLOTS], dg[NLTE_SLOTS];
   int row[NLTE_SLOTS];
 #pragma unroll
   for (int r = 0; r < NLTE_SLOTS; ++r) {
     const int i = lane + WAVE * r;
     const int ic = i < n ? i : n - 1;        // rows past n read row n-1; never used
     row[r] = ic * ld;
-    v[r] = i < n ? (T)(rhs[perm[i]] * sc) : (T)0;
+    v[r] = i < n ? (T)(rhs[pv[r]] * sc) : (T)0;
     dg[r] = a[row[r] + ic];
   }
   if (recip) {
@@ -722,48 +755,59 @@ __device__ void tri_solve_wave(const T* a, int ld, int n, const int* perm, const
 #pragma unroll
     for (int r = 0; r < NLTE_SLOTS; ++r) dg[r] = (T)1 / dg[r];
   }
+  // one forward step k = k0 + src with column k in cur[], column kn loaded
+  // into nxt[] for the next step
+#define SGPU_FWD(src, cur, nxt)                                                   \
+  {                                                                               \
+    const int kn = min(k0 + (src) + 1, n - 1);                                    \
+    _Pragma("unroll") for (int r = rk; r < NLTE_SLOTS; ++r) nxt[r] = a[row[r] + kn]; \
+    const T xk = readlane(v[rk], (src));                                          \
+    const T u = v[rk] - cur[rk] * xk;                                             \
+    v[rk] = lane > (src) ? u : v[rk];                                             \
+    _Pragma("unroll") for (int r = rk + 1; r < NLTE_SLOTS; ++r) v[r] -= cur[r] * xk; \
+  }
 #pragma unroll
   for (int rk = 0; rk < NLTE_SLOTS; ++rk) {   // forward, unit lower
     if (rk < ns) {
       const int k0 = rk * WAVE, cnt = min(WAVE, n - k0);
 #pragma unroll
-      for (int r = rk; r < NLTE_SLOTS; ++r) cur[r] = a[row[r] + k0];
-      for (int src = 0; src < cnt; ++src) {
-        const int kn = min(k0 + src + 1, n - 1);
-#pragma unroll
-        for (int r = rk; r < NLTE_SLOTS; ++r) nxt[r] = a[row[r] + kn];
-        const T xk = readlane(v[rk], src);
-        const T u = v[rk] - cur[rk] * xk;
-        v[rk] = lane > src ? u : v[rk];
-#pragma unroll
-        for (int r = rk + 1; r < NLTE_SLOTS; ++r) v[r] -= cur[r] * xk;
-#pragma unroll
-        for (int r = rk; r < NLTE_SLOTS; ++r) cur[r] = nxt[r];
+      for (int r = rk; r < NLTE_SLOTS; ++r) c0[r] = a[row[r] + k0];
+      int src = 0;
+      for (; src + 1 < cnt; src += 2) {
+        SGPU_FWD(src, c0, c1)
+        SGPU_FWD(src + 1, c1, c0)
       }
+      if (src < cnt) SGPU_FWD(src, c0, c1)
     }
+  }
+#undef SGPU_FWD
+  // one backward step k = k0 + src, column k in cur[], column kn into nxt[]
+#define SGPU_BWD(src, cur, nxt)                                                   \
+  {                                                                               \
+    const int kn = max(k0 + (src) - 1, 0);                                        \
+    _Pragma("unroll") for (int r = 0; r <= rk; ++r) nxt[r] = a[row[r] + kn];      \
+    T xk;                                                                         \
+    if (recip) xk = readlane(v[rk] * dg[rk], (src));                              \
+    else xk = readlane(v[rk], (src)) / readlane(dg[rk], (src));                   \
+    const T u = v[rk] - cur[rk] * xk;                                             \
+    v[rk] = lane < (src) ? u : (lane == (src) ? xk : v[rk]);                      \
+    _Pragma("unroll") for (int r = 0; r < rk; ++r) v[r] -= cur[r] * xk;           \
   }
 #pragma unroll
   for (int rk = NLTE_SLOTS - 1; rk >= 0; --rk) {   // backward
     if (rk < ns) {
       const int k0 = rk * WAVE, cnt = min(WAVE, n - k0);
 #pragma unroll
-      for (int r = 0; r <= rk; ++r) cur[r] = a[row[r] + k0 + cnt - 1];
-      for (int src = cnt - 1; src >= 0; --src) {
-        const int kn = max(k0 + src - 1, 0);
-#pragma unroll
-        for (int r = 0; r <= rk; ++r) nxt[r] = a[row[r] + kn];
-        T xk;
-        if (recip) xk = readlane(v[rk] * dg[rk], src);
-        else xk = readlane(v[rk], src) / readlane(dg[rk], src);
-        const T u = v[rk] - cur[rk] * xk;
-        v[rk] = lane < src ? u : (lane == src ? xk : v[rk]);
-#pragma unroll
-        for (int r = 0; r < rk; ++r) v[r] -= cur[r] * xk;
-#pragma unroll
-        for (int r = 0; r <= rk; ++r) cur[r] = nxt[r];
+      for (int r = 0; r <= rk; ++r) c0[r] = a[row[r] + k0 + cnt - 1];
+      int src = cnt - 1;
+      for (; src >= 1; src -= 2) {
+        SGPU_BWD(src, c0, c1)
+        SGPU_BWD(src - 1, c1, c0)
       }
+      if (src == 0) SGPU_BWD(0, c0, c1)
     }
   }
+#undef SGPU_BWD
 #pragma unroll
   for (int r = 0; r < NLTE_SLOTS; ++r) {
     const int i = lane + WAVE * r;
@@ -771,9 +815,17 @@ __device__ void tri_solve_wave(const T* a, int ld, int n, const int* perm, const
   }
 }
 
-// Global-memory passes over a system's rates, NT threads, loads issued 16
-// at a time so that each thread keeps 16 in flight.
+// Global-memory passes over a system's rates: loads issued 16 (row sums: 32)
+// at a time so that each thread keeps them in flight, and, where the order
+// of the reduction does not matter, q = min(4, NT / n) threads per row of M
+// whose partial results meet in part[(q - 1) n].
 constexpr int NLTE_UNROLL = 16;
+
+__device__ __forceinline__ int parts_per_row(int n)
+{
+  const int q = (int)blockDim.x / n;
+  return q >= 4 ? 4 : (q >= 2 ? 2 : 1);
+}
 
 // diag_i = -sum_j R_ij, summed in j order (one thread per row)
 __device__ void row_sums(int n, const double* R, double* diag)
@@ -782,12 +834,12 @@ __device__ void row_sums(int n, const double* R, double* diag)
     const double* row = R + (size_t)i * n;
     double acc = 0.0;
     int j = 0;
-    for (; j + NLTE_UNROLL <= n; j += NLTE_UNROLL) {
-      double t[NLTE_UNROLL];
+    for (; j + 2 * NLTE_UNROLL <= n; j += 2 * NLTE_UNROLL) {
+      double t[2 * NLTE_UNROLL];
 #pragma unroll
-      for (int u = 0; u < NLTE_UNROLL; ++u) t[u] = row[j + u];
+      for (int u = 0; u < 2 * NLTE_UNROLL; ++u) t[u] = row[j + u];
 #pragma unroll
-      for (int u = 0; u < NLTE_UNROLL; ++u) acc += t[u];
+      for (int u = 0; u < 2 * NLTE_UNROLL; ++u) acc += t[u];
     }
     for (; j < n; ++j) acc += row[j];
     diag[i] = -1 * acc;
@@ -802,45 +854,63 @@ __device__ __forceinline__ double pow2_scale(double mx)
 }
 
 // rs_i = 2^-e(max_j |M_ij|): M_ii = diag_i, M_ij = R_ji, M_{n-1,j} = lte_j
-// (one thread per row of M = column of R: coalesced reads)
-__device__ void row_scales(int n, const double* R, const double* lte, const double* diag, double* rs)
+// (threads over the rows of M = columns of R: coalesced reads)
+__device__ void row_scales(int n, const double* R, const double* lte, const double* diag, double* rs, double* part)
 {
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  const int q = parts_per_row(n), len = (n + q - 1) / q;
+  for (int t = threadIdx.x; t < q * n; t += blockDim.x) {
+    const int p = t / n, i = t - p * n;
+    const int j1 = min(n, (p + 1) * len);
+    int j = p * len;
     double mx = 0;
     if (i == n - 1) {
-      for (int j = 0; j < n; ++j) mx = fmax(mx, fabs(lte[j]));
+      for (; j < j1; ++j) mx = fmax(mx, fabs(lte[j]));
     } else {
-      int j = 0;
-      for (; j + NLTE_UNROLL <= n; j += NLTE_UNROLL) {
-        double t[NLTE_UNROLL];
+      for (; j + NLTE_UNROLL <= j1; j += NLTE_UNROLL) {
+        double v[NLTE_UNROLL];
 #pragma unroll
-        for (int u = 0; u < NLTE_UNROLL; ++u) t[u] = R[(size_t)(j + u) * n + i];
+        for (int u = 0; u < NLTE_UNROLL; ++u) v[u] = R[(size_t)(j + u) * n + i];
 #pragma unroll
-        for (int u = 0; u < NLTE_UNROLL; ++u) mx = fmax(mx, j + u == i ? 0.0 : fabs(t[u]));
+        for (int u = 0; u < NLTE_UNROLL; ++u) mx = fmax(mx, j + u == i ? 0.0 : fabs(v[u]));
       }
-      for (; j < n; ++j) mx = fmax(mx, j == i ? 0.0 : fabs(R[(size_t)j * n + i]));
-      mx = fmax(mx, fabs(diag[i]));
+      for (; j < j1; ++j) mx = fmax(mx, j == i ? 0.0 : fabs(R[(size_t)j * n + i]));
     }
+    part[p * n + i] = mx;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double mx = part[i];
+    for (int p = 1; p < q; ++p) mx = fmax(mx, part[p * n + i]);
+    if (i < n - 1) mx = fmax(mx, fabs(diag[i]));
     rs[i] = pow2_scale(mx);
   }
 }
 
-// cs_j = 2^-e(max_i |M_ij| rs_i) (one thread per column of M = row of R)
+// cs_j = 2^-e(max_i |M_ij| rs_i) (threads over the columns of M = rows of R)
 __device__ void col_scales(int n, const double* R, const double* lte, const double* diag, const double* rs,
-                           double* cs)
+                           double* cs, double* part)
 {
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+  const int q = parts_per_row(n), len = (n - 1 + q - 1) / q;
+  for (int t = threadIdx.x; t < q * n; t += blockDim.x) {
+    const int p = t / n, j = t - p * n;
     const double* row = R + (size_t)j * n;
-    double mx = fmax(fabs(lte[j]) * rs[n - 1], j < n - 1 ? fabs(diag[j]) * rs[j] : 0.0);
-    int i = 0;
-    for (; i + NLTE_UNROLL <= n - 1; i += NLTE_UNROLL) {
-      double t[NLTE_UNROLL], w[NLTE_UNROLL];
+    const int i1 = min(n - 1, (p + 1) * len);
+    int i = p * len;
+    double mx = 0;
+    for (; i + NLTE_UNROLL <= i1; i += NLTE_UNROLL) {
+      double v[NLTE_UNROLL], w[NLTE_UNROLL];
 #pragma unroll
-      for (int u = 0; u < NLTE_UNROLL; ++u) { t[u] = row[i + u]; w[u] = rs[i + u]; }
+      for (int u = 0; u < NLTE_UNROLL; ++u) { v[u] = row[i + u]; w[u] = rs[i + u]; }
 #pragma unroll
-      for (int u = 0; u < NLTE_UNROLL; ++u) mx = fmax(mx, i + u == j ? 0.0 : fabs(t[u]) * w[u]);
+      for (int u = 0; u < NLTE_UNROLL; ++u) mx = fmax(mx, i + u == j ? 0.0 : fabs(v[u]) * w[u]);
     }
-    for (; i < n - 1; ++i) mx = fmax(mx, i == j ? 0.0 : fabs(row[i]) * rs[i]);
+    for (; i < i1; ++i) mx = fmax(mx, i == j ? 0.0 : fabs(row[i]) * rs[i]);
+    part[p * n + j] = mx;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    double mx = fmax(fabs(lte[j]) * rs[n - 1], j < n - 1 ? fabs(diag[j]) * rs[j] : 0.0);
+    for (int p = 0; p < q; ++p) mx = fmax(mx, part[p * n + j]);
     cs[j] = pow2_scale(mx);
   }
 }
@@ -879,43 +949,52 @@ __device__ void fill_lds(int n, const double* R, const double* lte, const double
   }
 }
 
-// r_i = (e_{n-1} - M x)_i rs_i, one thread per row of M (coalesced reads of
-// the columns of R), 16 loads in flight
+// r_i = (e_{n-1} - M x)_i rs_i (threads over the rows of M: coalesced reads
+// of the columns of R; q threads per row, partial sums in part[(q - 1) n])
 __device__ void residual(int n, const double* R, const double* lte, const double* diag, const double* rs,
-                        const double* x, double* r)
+                         const double* x, double* r, double* part)
 {
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  const int q = parts_per_row(n), len = (n + q - 1) / q;
+  for (int t = threadIdx.x; t < q * n; t += blockDim.x) {
+    const int p = t / n, i = t - p * n;
+    const int j1 = min(n, (p + 1) * len);
+    int j = p * len;
     double acc = 0;
-    int j = 0;
     if (i == n - 1) {
-      for (; j + NLTE_UNROLL <= n; j += NLTE_UNROLL) {
-        double t[NLTE_UNROLL], w[NLTE_UNROLL];
+      for (; j + NLTE_UNROLL <= j1; j += NLTE_UNROLL) {
+        double v[NLTE_UNROLL], w[NLTE_UNROLL];
 #pragma unroll
-        for (int u = 0; u < NLTE_UNROLL; ++u) { t[u] = lte[j + u]; w[u] = x[j + u]; }
+        for (int u = 0; u < NLTE_UNROLL; ++u) { v[u] = lte[j + u]; w[u] = x[j + u]; }
 #pragma unroll
-        for (int u = 0; u < NLTE_UNROLL; ++u) acc += t[u] * w[u];
+        for (int u = 0; u < NLTE_UNROLL; ++u) acc += v[u] * w[u];
       }
-      for (; j < n; ++j) acc += lte[j] * x[j];
-      acc = 1.0 - acc;
+      for (; j < j1; ++j) acc += lte[j] * x[j];
     } else {
-      acc = diag[i] * x[i];
-      for (; j + NLTE_UNROLL <= n; j += NLTE_UNROLL) {
-        double t[NLTE_UNROLL], w[NLTE_UNROLL];
+      for (; j + NLTE_UNROLL <= j1; j += NLTE_UNROLL) {
+        double v[NLTE_UNROLL], w[NLTE_UNROLL];
 #pragma unroll
-        for (int u = 0; u < NLTE_UNROLL; ++u) { t[u] = R[(size_t)(j + u) * n + i]; w[u] = x[j + u]; }
+        for (int u = 0; u < NLTE_UNROLL; ++u) { v[u] = R[(size_t)(j + u) * n + i]; w[u] = x[j + u]; }
 #pragma unroll
-        for (int u = 0; u < NLTE_UNROLL; ++u) acc += j + u == i ? 0.0 : t[u] * w[u];
+        for (int u = 0; u < NLTE_UNROLL; ++u) acc += j + u == i ? 0.0 : v[u] * w[u];
       }
-      for (; j < n; ++j) if (j != i) acc += R[(size_t)j * n + i] * x[j];
-      acc = -acc;
+      for (; j < j1; ++j) if (j != i) acc += R[(size_t)j * n + i] * x[j];
     }
+    if (p == 0) r[i] = acc;
+    else part[(p - 1) * n + i] = acc;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double acc = r[i];
+    for (int p = 1; p < q; ++p) acc += part[(p - 1) * n + i];
+    acc = (i == n - 1) ? 1.0 - acc : -(diag[i] * x[i] + acc);
     r[i] = acc * rs[i];
   }
 }
 
 // Scaled system: As = Dr M Dc with Dr, Dc powers of two; M b = e becomes
 // As y = Dr e, b = Dc y.  The FP32 factors of As sit in LDS (row stride
-// n | 1); the pivot permutation and the FP64 vectors in the workgroup's slot.
+// n | 1), the pivot permutation in wavefront 0's registers, the FP64
+// vectors in the workgroup's slot.
 template <int NB>
 __global__ void __launch_bounds__(NLTE_MAX_THREADS) nlte_batched_mixed_kernel(BatchArgs A, size_t lds_bytes)
 {
@@ -950,31 +1029,34 @@ __global__ void __launch_bounds__(NLTE_MAX_THREADS) nlte_batched_mixed_kernel(Ba
     bool ok = false;
     if (n <= LdsCap<float>::n) {
       // equilibration: rs_i = 2^-e(max_j |M_ij|), cs_j = 2^-e(max_i |M_ij| rs_i)
+      double* part = (double*)lds_raw;   // free until the matrix is filled
       row_sums(n, R, diag);
       __syncthreads();
-      row_scales(n, R, lte, diag, rs);
+      row_scales(n, R, lte, diag, rs, part);
       __syncthreads();
-      col_scales(n, R, lte, diag, rs, cs);
+      col_scales(n, R, lte, diag, rs, cs, part);
       __syncthreads();
       fill_lds<float>(n, R, lte, diag, rs, cs, lds_a, ld);
       __syncthreads();
-      const int sing = lu_lds<float, NB>(lds_a, ld, n, perm, &flag);
+      int pv[NLTE_SLOTS];   // pivot permutation (wavefront 0's registers)
+      const int sing = lu_lds<float, NB>(lds_a, ld, n, pv, &flag);
       if (!sing) {
         // x = Dc As^-1 Dr e_{n-1}
         if (wave == 0) {
           for (int i = lane; i < n; i += WAVE) r[i] = (i == n - 1) ? rs[i] : 0.0;
           wave_fence();
-          tri_solve_wave(lds_a, ld, n, perm, r, d, lane);
+          tri_solve_wave(lds_a, ld, n, pv, r, d, lane);
           wave_fence();
           for (int i = lane; i < n; i += WAVE) x[i] = d[i] * cs[i];
         }
         __syncthreads();
+        double dprev = HUGE_VAL;   // last relative correction (wavefront 0)
         for (it = 1; it <= NLTE_MAX_ITERS; ++it) {
           // FP64 residual r = e_{n-1} - M x, scaled by Dr
-          residual(n, R, lte, diag, rs, x, r);
+          residual(n, R, lte, diag, rs, x, r, M64);
           __syncthreads();
           if (wave == 0) {
-            tri_solve_wave(lds_a, ld, n, perm, r, d, lane);
+            tri_solve_wave(lds_a, ld, n, pv, r, d, lane);
             wave_fence();
             double dmax = 0, xmax = 0;
             for (int i = lane; i < n; i += WAVE) {
@@ -986,8 +1068,12 @@ __global__ void __launch_bounds__(NLTE_MAX_THREADS) nlte_batched_mixed_kernel(Ba
             }
             dmax = wave_max(dmax);
             xmax = wave_max(xmax);
+            const double rel = dmax / xmax;
+            // converged: below tol, or stalled (no longer halving) within 16 tol
+            const bool conv = rel <= A.tol || (rel > 0.5 * dprev && rel <= 16 * A.tol);
+            dprev = rel;
             // 1: converged, 2: NaN (give up on FP32), 0: continue
-            if (lane == 0) flag = (!(dmax == dmax) || !(xmax == xmax)) ? 2 : (dmax <= A.tol * xmax ? 1 : 0);
+            if (lane == 0) flag = (!(dmax == dmax) || !(xmax == xmax)) ? 2 : (conv ? 1 : 0);
           }
           __syncthreads();
           const int f = flag;
@@ -1043,11 +1129,12 @@ __global__ void __launch_bounds__(NLTE_MAX_THREADS) nlte_batched_f64_lds_kernel(
       __syncthreads();
       fill_lds<double>(n, R, lte, diag, nullptr, nullptr, lds_d, ld);
       __syncthreads();
-      lu_lds<double, NB>(lds_d, ld, n, perm, &flag);
+      int pv[NLTE_SLOTS];   // pivot permutation (wavefront 0's registers)
+      lu_lds<double, NB>(lds_d, ld, n, pv, &flag);
       if (wave == 0) {
         for (int i = lane; i < n; i += WAVE) r[i] = (i == n - 1) ? 1.0 : 0.0;
         wave_fence();
-        tri_solve_wave(lds_d, ld, n, perm, r, x, lane);
+        tri_solve_wave(lds_d, ld, n, pv, r, x, lane);
       }
       __syncthreads();
     } else {
