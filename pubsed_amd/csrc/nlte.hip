@@ -205,7 +205,10 @@ __global__ void __launch_bounds__(1024) size_range_kernel(const int32_t* n, int 
   }
 }
 
-// counting sort of system indices by size, descending (single workgroup)
+// bucket sort of system indices by size, descending (single workgroup):
+// histogram, prefix, then every thread places its systems through an
+// atomic cursor (the order within one size is arbitrary; each system's
+// result does not depend on it)
 __global__ void __launch_bounds__(1024) order_kernel(const int32_t* n, int nsys, int32_t* order, int32_t* counter)
 {
   __shared__ int hist[1025];
@@ -223,13 +226,11 @@ __global__ void __launch_bounds__(1024) order_kernel(const int32_t* n, int nsys,
     *counter = 0;
   }
   __syncthreads();
-  // stable within a size class: a single thread scatters in index order
-  if (threadIdx.x == 0)
-    for (int s = 0; s < nsys; s++) {
-      int v = n[s];
-      v = v < 0 ? 0 : (v > 1024 ? 1024 : v);
-      order[hist[1024 - v]++] = s;
-    }
+  for (int s = threadIdx.x; s < nsys; s += blockDim.x) {
+    int v = n[s];
+    v = v < 0 ? 0 : (v > 1024 ? 1024 : v);
+    order[atomicAdd(&hist[1024 - v], 1)] = s;
+  }
 }
 
 struct BatchArgs {
