@@ -206,6 +206,16 @@ def main():
     events_per_s = c["n_events"] / (np.sum(prop_ms) * 1e-3)
     log("mixed: %.3f s/step, propagate %.1f ms, events/packet %.1f, %.3g events/s" %
         (dt / a.steps, prop_avg, per_pkt["n_events"], events_per_s))
+    # ceiling of the per-event J_nu tally: random FP64 atomics into an array
+    # of the J_nu table's size (one per event in the propagation)
+    jbuf = torch.zeros(P.nz * P.n_nu, dtype=torch.float64, device=dev)
+    adds = 1 << 30
+    gpu.atomic_rate(jbuf, adds)                       # warm-up
+    jbuf.zero_()
+    at_ms = gpu.atomic_rate(jbuf, adds)
+    assert float(jbuf.sum().item()) == float(adds)
+    atomic_ceiling = adds / (at_ms * 1e-3)
+    del jbuf
     traffic, tinfo = read_traffic(a.traffic_from, N)
     traffic_src = None
     if tinfo:
@@ -232,6 +242,11 @@ def main():
                 "survey_formula_bytes_per_packet": bpk_survey,
                 "survey_formula_frac": bpk_survey * N / (prop_avg * 1e-3) / 1e9 / HBM_PEAK_GBS,
                 "events_per_s": events_per_s,
+                "jnu_atomic_ceiling_per_s": atomic_ceiling,
+                "events_over_atomic_ceiling": events_per_s / atomic_ceiling,
+                "atomic_note": "one FP64 atomic J_nu[zone][bin] += E per event (transport.cpp "
+                               "J_nu tally); ceiling = sgpu_atomic_rate, 2^30 random FP64 "
+                               "atomic adds into an array of the J_nu table's size",
                 "traffic_source": traffic_src}
     out = {
         "metric": "packets/s & NLTE solves/s: batched/mixed vs plain GPU path at matched MC noise",

@@ -510,6 +510,14 @@ int sgpu_propagate_dev(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_locate* 
 int sgpu_compact_dev(sgpu_ctx* ctx, sgpu_packets* packets, int64_t* d_count, sgpu_packets* scr,
                      int64_t* d_escaped);
 
+/* Throughput calibration of the per-event J_nu tally (no reference
+ * counterpart): `adds` FP64 atomic additions of 1.0 at hash-scattered
+ * indices of buf[0, n) (device), launched on the context's stream and timed
+ * with HIP events; *ms receives the kernel time.  buf then holds the counts
+ * (they sum to `adds`).  The propagation does one such addition per event
+ * into J_nu[zone][bin]. */
+int sgpu_atomic_rate(sgpu_ctx* ctx, double* buf, int64_t n, int64_t adds, double* ms);
+
 /* Zero the zone and spectrum tallies (transport::wipe_radiation; the spectra
  * are wiped only when wipe_spectra != 0). */
 int sgpu_wipe_radiation(sgpu_ctx* ctx, int32_t n_zones, int32_t n_nu,

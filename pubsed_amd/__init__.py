@@ -73,6 +73,7 @@ def _declare(lib):
         "sgpu_nlte_autotune": (I, [V, I32, V, V, V, V, V, V, V, V, ctypes.c_char_p,
                                    ctypes.c_size_t]),
         "sgpu_nlte_set_tuning": (I, [V, I32, I32, I32, I32]),
+        "sgpu_atomic_rate": (I, [V, V, ctypes.c_int64, ctypes.c_int64, P(D)]),
         "sgpu_homologous_step": (I, [V, P(abi.Grid), V, V, V, V, V, D]),
         "sgpu_nlte_set_rates": (I, [V, P(abi.Atoms), I32, V, V, V, V, V, V, V, I32, I32, V, V, V,
                                     V, V]),

@@ -129,6 +129,7 @@ EXPORTED_SYMBOLS = [
     "sgpu_compact",
     "sgpu_compact_noalloc", "sgpu_use_own_stream", "sgpu_graph_begin", "sgpu_graph_end", "sgpu_graph_launch",
     "sgpu_graph_nodes", "sgpu_emit_radioactive_dev", "sgpu_propagate_dev", "sgpu_compact_dev",
+    "sgpu_atomic_rate",
     "sgpu_wipe_radiation",
     "sgpu_normalize_tallies", "sgpu_nlte_solve", "sgpu_nlte_batched", "sgpu_nlte_autotune", "sgpu_nlte_set_tuning",
     "sgpu_homologous_step", "sgpu_adjoint_forward", "sgpu_adjoint_backward",

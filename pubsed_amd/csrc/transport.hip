@@ -393,12 +393,15 @@ __device__ __forceinline__ int do_scatter(const PropArgs<R>& A, Pkt<R>& p, R eps
 {
   int fate = SGPU_MOVING;
   if (p.type == SGPU_PHOTON) {
-    if (p.rng.template u<R>() > eps) isotropic_scatter(A, p, 0, c);
-    else {
+    // one inlined scatter for both outcomes (lanes of a wavefront take
+    // both), the draws in the reference's order
+    int redist = 0;
+    if (!(p.rng.template u<R>() > eps)) {
       R z2 = p.rng.template u<R>();
-      if (A.p.radiative_eq || (z2 > (R)A.eps_imc[p.ind])) isotropic_scatter(A, p, 1, c);
+      if (A.p.radiative_eq || (z2 > (R)A.eps_imc[p.ind])) redist = 1;
       else fate = SGPU_ABSORBED;
     }
+    if (fate == SGPU_MOVING) isotropic_scatter(A, p, redist, c);
   } else if (p.type == SGPU_GAMMARAY) {
     if (p.rng.template u<R>() > eps) compton_scatter(A, p, zs, c);
     else {
@@ -1754,6 +1757,51 @@ int propagate_impl(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_locate* nu_g
   return SGPU_OK;
 }
 }  // namespace
+
+namespace {
+
+__device__ __forceinline__ uint32_t mix32(uint32_t x)
+{
+  x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+  return x;
+}
+
+// `per` FP64 atomic additions of 1.0 per thread at hash-scattered indices
+__global__ void __launch_bounds__(256) atomic_rate_kernel(double* buf, uint32_t n, int per)
+{
+  uint32_t h = mix32(blockIdx.x * blockDim.x + threadIdx.x);
+  for (int k = 0; k < per; ++k) {
+    h = mix32(h + (uint32_t)k);
+    atomic_add_f64(&buf[h % n], 1.0);
+  }
+}
+
+}  // namespace
+
+extern "C" int sgpu_atomic_rate(sgpu_ctx* ctx, double* buf, int64_t n, int64_t adds, double* ms)
+{
+  SGPU_REQUIRE(ctx && buf && ms, "null argument");
+  SGPU_REQUIRE(n >= 1 && n <= 0xffffffffLL, "n = %lld outside [1, 2^32)", (long long)n);
+  const int per = 256;
+  const int64_t threads = (adds + per - 1) / per;
+  SGPU_REQUIRE(adds >= 1 && threads % 256 == 0 && threads / 256 <= 0x7fffffff,
+               "adds = %lld must be a positive multiple of 65536", (long long)adds);
+  hipEvent_t e0, e1;
+  SGPU_HIP(hipEventCreate(&e0));
+  SGPU_HIP(hipEventCreate(&e1));
+  SGPU_HIP(hipEventRecord(e0, ctx->stream));
+  hipLaunchKernelGGL(atomic_rate_kernel, dim3((unsigned)(threads / 256)), dim3(256), 0, ctx->stream, buf,
+                     (uint32_t)n, per);
+  SGPU_LAUNCH_CHECK();
+  SGPU_HIP(hipEventRecord(e1, ctx->stream));
+  SGPU_HIP(hipEventSynchronize(e1));
+  float t = 0;
+  SGPU_HIP(hipEventElapsedTime(&t, e0, e1));
+  *ms = t;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return SGPU_OK;
+}
 
 extern "C" int sgpu_propagate(sgpu_ctx* ctx, const sgpu_grid* grid, const sgpu_locate* nu_grid,
                               const sgpu_tables* tables, const sgpu_transport_params* params,

@@ -576,6 +576,15 @@ class GpuBackend:
                                                 abi.ptr(b), abi.ptr(pop), precision, tol,
                                                 abi.ptr(iters)), "sgpu_nlte_batched")
 
+    def atomic_rate(self, buf, adds):
+        """Kernel time (ms) of `adds` FP64 atomic additions of 1.0 at
+        hash-scattered indices of the float64 device tensor buf."""
+        from . import check
+        ms = ctypes.c_double(0)
+        check(self.lib.sgpu_atomic_rate(self.ctx, abi.ptr(buf), buf.numel(), adds, ctypes.byref(ms)),
+              "sgpu_atomic_rate")
+        return ms.value
+
     def nlte_set_tuning(self, precision, threads, panel, wg_per_cu):
         from . import check
         return check(self.lib.sgpu_nlte_set_tuning(self.ctx, precision, threads, panel, wg_per_cu),
