@@ -305,8 +305,20 @@ def main():
                 left -= min(N, 1_000_000) / rates["fp64_batched"]
             n_leg = N if N / rates[key] <= left else max(int(rates[key] * left), n_cal)
             log("%s: %d packets (estimated %.0f s, %.0f s left)" % (key, n_leg, n_leg / rates[key], left))
-            dtp, prop_p, cp, _ = timed(prec, n_leg, 1, 0)
-            out[key] = {"packets": n_leg, "packets_per_s": n_leg / dtp, "propagate_ms": prop_p[0],
+            # steps of at most ~60 s each (the rate is linear in the count in
+            # this range), each reporting its progress
+            chunk = max(n_cal, int(rates[key] * 60.0))
+            dtp, prop_sum, cp, done = 0.0, 0.0, None, 0
+            while done < n_leg:
+                n_c = min(chunk, n_leg - done)
+                dtc, prop_c, cc, _ = timed(prec, n_c, 1, 0)
+                dtp += dtc
+                prop_sum += prop_c[0]
+                cp = cc if cp is None else {k: cp[k] + cc[k] for k in cp}
+                done += n_c
+                log("%s: %d / %d packets, %.1f s" % (key, done, n_leg, dtp))
+            out[key] = {"packets": n_leg, "packets_per_s": n_leg / dtp, "propagate_ms": prop_sum,
+                        "steps": -(-n_leg // chunk),
                         "escape_fraction": cp["n_escaped"] / max(cp["n_packets"], 1),
                         "events_per_packet": cp["n_events"] / max(cp["n_packets"], 1),
                         "mixed_over_this": value / (n_leg / dtp)}

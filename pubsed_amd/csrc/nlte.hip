@@ -20,8 +20,12 @@
 //                       update -- three workgroup barriers per panel.
 //     FP64 mode         FP64 LU in LDS for n <= 143 (164 KB), in a
 //                       per-workgroup device-memory slot above that.
-//     FP32_REFINE mode  row/column power-of-two equilibration, FP32 LU in LDS
-//                       (n <= 201), trailing updates on MFMA
+//     FP32_REFINE mode  systems whose FP64 matrix fits the launch's LDS
+//                       (sized for its largest system) are factored directly
+//                       in FP64 as in the FP64 mode (iters 0: fewer steps
+//                       than FP32 with refinement at these sizes); larger
+//                       ones: row/column power-of-two equilibration, FP32 LU
+//                       in LDS (n <= 201), trailing updates on MFMA
 //                       (v_mfma_f32_32x32x2f32) for the bulk of the matrix,
 //                       FP64 residuals r = e - M b straight from the FP64
 //                       rates, FP64 correction solves through the FP32
@@ -992,6 +996,30 @@ __device__ void residual(int n, const double* R, const double* lte, const double
   }
 }
 
+// M b = e_{n-1} by FP64 LU in LDS (row stride n | 1, the caller checked
+// that n (n | 1) doubles fit): row sums, M transposed from the rows of R,
+// the blocked LU, one wavefront's triangular solves into x.  diag and r are
+// n-vectors of the workgroup's slot.
+template <int NB>
+__device__ __forceinline__ void solve_f64_lds(int n, const double* R, const double* lte, double* lds_d, double* diag,
+                                              double* r, double* x, int* flag)
+{
+  const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  const int ld = lds_ld(n);
+  row_sums(n, R, diag);
+  __syncthreads();
+  fill_lds<double>(n, R, lte, diag, nullptr, nullptr, lds_d, ld);
+  __syncthreads();
+  int pv[NLTE_SLOTS];   // pivot permutation (wavefront 0's registers)
+  lu_lds<double, NB>(lds_d, ld, n, pv, flag);
+  if (wave == 0) {
+    for (int i = lane; i < n; i += WAVE) r[i] = (i == n - 1) ? 1.0 : 0.0;
+    wave_fence();
+    tri_solve_wave(lds_d, ld, n, pv, r, x, lane);
+  }
+  __syncthreads();
+}
+
 // Scaled system: As = Dr M Dc with Dr, Dc powers of two; M b = e becomes
 // As y = Dr e, b = Dc y.  The FP32 factors of As sit in LDS (row stride
 // n | 1), the pivot permutation in wavefront 0's registers, the FP64
@@ -1006,7 +1034,6 @@ __global__ void __launch_bounds__(NLTE_MAX_THREADS) nlte_batched_mixed_kernel(Ba
   double* slot = A.slots + (size_t)blockIdx.x * A.slot_stride;
   const int tid = threadIdx.x;
   const int wave = tid / WAVE, lane = tid % WAVE;
-  (void)lds_bytes;
   while (true) {
     const int s = next_system(A, &sh);
     if (s < 0) break;
@@ -1028,6 +1055,15 @@ __global__ void __launch_bounds__(NLTE_MAX_THREADS) nlte_batched_mixed_kernel(Ba
     double* M64 = (double*)(perm + 2 * ((n + 1) / 2));
     int it = 0;
     bool ok = false;
+    if (sizeof(double) * (size_t)n * ld <= lds_bytes) {
+      // small enough for an FP64 matrix in this launch's LDS: factored
+      // directly in FP64, no refinement (iters 0)
+      solve_f64_lds<NB>(n, R, lte, (double*)lds_raw, diag, r, x, &flag);
+      write_pops(n, x, lte, bout, pout);
+      if (A.iters && tid == 0) A.iters[s] = 0;
+      __syncthreads();
+      continue;
+    }
     if (n <= LdsCap<float>::n) {
       // equilibration: rs_i = 2^-e(max_j |M_ij|), cs_j = 2^-e(max_i |M_ij| rs_i)
       double* part = (double*)lds_raw;   // free until the matrix is filled
@@ -1110,7 +1146,6 @@ __global__ void __launch_bounds__(NLTE_MAX_THREADS) nlte_batched_f64_lds_kernel(
   __shared__ int sh, sing_f64, flag;
   double* slot = A.slots + (size_t)blockIdx.x * A.slot_stride;
   const int tid = threadIdx.x;
-  const int wave = tid / WAVE, lane = tid % WAVE;
   (void)lds_bytes;
   while (true) {
     const int s = next_system(A, &sh);
@@ -1125,19 +1160,7 @@ __global__ void __launch_bounds__(NLTE_MAX_THREADS) nlte_batched_f64_lds_kernel(
     int* perm = (int*)(r + n);
     double* M64 = (double*)(perm + 2 * ((n + 1) / 2));
     if (n <= LdsCap<double>::n) {
-      const int ld = lds_ld(n);
-      row_sums(n, R, diag);
-      __syncthreads();
-      fill_lds<double>(n, R, lte, diag, nullptr, nullptr, lds_d, ld);
-      __syncthreads();
-      int pv[NLTE_SLOTS];   // pivot permutation (wavefront 0's registers)
-      lu_lds<double, NB>(lds_d, ld, n, pv, &flag);
-      if (wave == 0) {
-        for (int i = lane; i < n; i += WAVE) r[i] = (i == n - 1) ? 1.0 : 0.0;
-        wave_fence();
-        tri_solve_wave(lds_d, ld, n, pv, r, x, lane);
-      }
-      __syncthreads();
+      solve_f64_lds<NB>(n, R, lte, lds_d, diag, r, x, &flag);
     } else {
       if (tid == 0) sing_f64 = 0;
       assemble<double>(n, R, lte, M64, n);
@@ -1172,14 +1195,14 @@ namespace {
 const void* batched_kernel(int precision, int panel, int max_n, size_t* lds)
 {
   if (precision == SGPU_NLTE_FP64) {
-    if (max_n > LdsCap<double>::n) {
-      *lds = 0;
-      return (const void*)nlte_batched_f64_kernel;
-    }
-    *lds = lds_bytes_for<double>(max_n);
+    // systems up to 143 levels in LDS, larger ones in the workgroup's slot
+    *lds = lds_bytes_for<double>(std::min(max_n, LdsCap<double>::n));
     return panel == 8 ? (const void*)nlte_batched_f64_lds_kernel<8> : (const void*)nlte_batched_f64_lds_kernel<16>;
   }
-  *lds = lds_bytes_for<float>(std::min(max_n, LdsCap<float>::n));
+  // the FP32 matrix of the largest system, or the FP64 matrix of the
+  // largest system that can be factored directly in FP64, if larger
+  const int c64 = std::min(max_n, LdsCap<double>::n);
+  *lds = std::max(lds_bytes_for<float>(std::min(max_n, LdsCap<float>::n)), lds_bytes_for<double>(c64));
   return panel == 8 ? (const void*)nlte_batched_mixed_kernel<8> : (const void*)nlte_batched_mixed_kernel<16>;
 }
 

@@ -2,8 +2,10 @@
 
 Bars: FP64 forms (single drop-in, batched FP64) within 1e-11 normwise of
 the oracle's departure coefficients; the FP32-LU + FP64-refinement form
-within 1e-8 normwise, with every system converging in FP32 (iters > 0), so
-the MFMA trailing update is exercised rather than the FP64 fallback.
+within 1e-8 normwise.  In that form a system whose FP64 matrix fits the
+launch's LDS is factored directly in FP64 (iters 0); every larger one must
+converge in FP32 (iters > 0), so the MFMA trailing update is exercised
+rather than the FP64 fallback (iters -1).
 """
 import numpy as np
 import pytest
@@ -12,6 +14,16 @@ from pubsed_amd import workloads as W, abi
 from oracle.oracle_py import OracleBackend
 
 pytestmark = pytest.mark.gpu
+
+
+def fp64_direct(n, max_n):
+    """True where the FP32 + refinement launch factors directly in FP64: the
+    launch's LDS holds the FP32 matrix of min(max_n, 201) levels or the FP64
+    one of min(max_n, 143), whichever is larger (row stride n | 1)."""
+    c32, c64 = min(max_n, 201), min(max_n, 143)
+    lds = max(4 * c32 * (c32 | 1), 32 * c32, 8 * c64 * (c64 | 1), 32 * c64)
+    n = np.asarray(n)
+    return 8 * n * (n | 1) <= lds
 
 
 def oracle_solutions(sy):
@@ -85,7 +97,9 @@ def test_batched(gpu, precision, bar, n_max):
     assert errs.max() <= bar
     if precision == abi.NLTE_FP32_REFINE:
         it = d["iters"].cpu().numpy()
-        assert (it > 0).all(), "some systems fell back to FP64: %s" % it
+        direct = fp64_direct(sy["n"], int(np.max(sy["n"])))
+        assert (it[direct] == 0).all() and direct.any()
+        assert (it[~direct] > 0).all(), "some systems fell back to FP64: %s" % it
 
 
 @pytest.mark.parametrize("precision,n_max,bar", [(abi.NLTE_FP64, 140, 1e-11), (abi.NLTE_FP32_REFINE, 201, 1e-8)])
@@ -109,7 +123,9 @@ def test_every_launch_shape(gpu, precision, n_max, bar):
                     errs = normwise(sy, d["b"].cpu().numpy(), ref)
                     assert errs.max() <= bar, (threads, panel, wg, errs.max())
                     if precision == abi.NLTE_FP32_REFINE:
-                        assert (d["iters"].cpu().numpy() > 0).all(), (threads, panel, wg)
+                        it = d["iters"].cpu().numpy()
+                        direct = fp64_direct(sy["n"], int(np.max(sy["n"])))
+                        assert (it[direct] == 0).all() and (it[~direct] > 0).all(), (threads, panel, wg, it)
     finally:
         gpu.nlte_set_tuning(precision, 256, 16, 1)
 
@@ -117,7 +133,8 @@ def test_every_launch_shape(gpu, precision, n_max, bar):
 def test_uneven_sizes_edge(gpu):
     """Tiny, panel-straddling and LDS-limit sizes in one batch (n = 2 ... 201,
     the largest FP32 matrix with an odd row stride in 160 KiB of LDS), plus 202
-    and 230, which exceed it and take the FP64 path."""
+    and 230, which exceed it and take the FP64 path; up to 143 levels the
+    launch factors in FP64 directly."""
     rng = np.random.default_rng(3)
     sizes = [2, 3, 31, 32, 33, 64, 65, 97, 128, 160, 201, 202, 230]
     sy = W.nlte_systems(n_sys=len(sizes), seed=9)
@@ -139,7 +156,9 @@ def test_uneven_sizes_edge(gpu):
     print("errs", errs, "iters", it)
     assert errs.max() <= 1e-8
     assert (it[-2:] == -1).all()    # n = 202, 230 > LDS limit: FP64 path
-    assert (it[:-2] > 0).all()
+    direct = fp64_direct(sy["n"][:-2], 230)
+    assert (it[:-2][direct] == 0).all() and (it[:-2][~direct] > 0).all()
+    assert (~direct).sum() == 2      # 160 and 201: FP32 + refinement
 
 
 def rate_case(nz=6, near_lte=False, seed=5):
