@@ -80,6 +80,7 @@ def load():
         lib.or_nlte_set_rates.restype = I
         lib.or_nlte_set_rates.argtypes = [P(abi.Atoms), I, D, D, V, V, V, V, I, I, V, V]
         lib.or_nlte_populations.argtypes = [P(abi.Atoms), I, V, V, V, V]
+        lib.or_solve_lte.argtypes = [P(abi.Atoms), I, D, D, V, V, V]
         lib.or_voigt_profile.restype = D
         lib.or_voigt_profile.argtypes = [D, D]
         lib.or_photo_cs.restype = D

@@ -171,8 +171,12 @@ class Problem:
         self.atoms = None
         self.n_ion = self.n_lev = 0
         if cfg["grey"] == 0:
-            at = W.synthetic_atoms(self.nu_np, ej["elem_Z"], n_lines=cfg.get("n_lines", 100000),
-                                   seed=seed)
+            if cfg.get("atom") is not None:      # one hand-built atom (hydrogen_atom's arguments)
+                assert len(ej["elem_Z"]) == 1
+                at = W.hydrogen_atom(self.nu_np, **cfg["atom"])
+            else:
+                at = W.synthetic_atoms(self.nu_np, ej["elem_Z"], n_lines=cfg.get("n_lines", 100000),
+                                       seed=seed)
             self.atoms_np = at
             for k in ("lev_off", "ion_off", "lev_E", "lev_g", "lev_ion", "ion_chi", "atom_zero_eps",
                       "line_nu", "line_gf", "line_El", "line_ion", "line_bin", "line_bin_off",

@@ -204,6 +204,29 @@ def synthetic_atoms(nu_grid, elem_Z, n_lines=100000, n_ions=4, levels_per_ion=12
                 line_atom=atom.astype(np.int32))
 
 
+def hydrogen_atom(nu_grid, levels, A, chi=13.6):
+    """One hydrogen-like atom: bound levels (E in eV, g) of the neutral stage
+    plus the continuum (the ionised stage, one level, g = 1), ionisation
+    potential chi eV, Einstein A_ul of every bound-bound pair in (lower,
+    upper) order, no fuzz lines.  levels = ((0, 2), (10.20, 8), (12.09, 18))
+    with A = (4.696e8, 5.572e7, 4.408e7) is the model atom of the reference's
+    three-level NLTE test (tests/3level_nlte/problem_writeup.pdf, "Test
+    problem"); the first two levels with A = (4.7e8,) the two-level atom of
+    tests/2level_nlte/run_test.py:70-72."""
+    lev_E = np.array([e for e, _ in levels] + [0.0], np.float64)
+    lev_g = np.array([g for _, g in levels] + [1], np.int32)
+    lev_ion = np.array([0] * len(levels) + [1], np.int32)
+    lev_off = np.array([0, len(lev_E)], np.int32)
+    bl = level_lines(nu_grid, lev_off, lev_E, lev_g, lev_ion, None, A=A)
+    nn = nu_grid["n"]
+    return dict(**bl, n_atoms=1, lev_off=lev_off, ion_off=np.array([0, 2], np.int32), lev_E=lev_E,
+                lev_g=lev_g, lev_ion=lev_ion, ion_chi=np.array([chi, 99999.0]),
+                atom_zero_eps=np.zeros(1, np.int32), max_levels=len(lev_E),
+                line_nu=np.zeros(0), line_gf=np.zeros(0), line_El=np.zeros(0),
+                line_ion=np.zeros(0, np.int32), line_bin=np.zeros(0, np.int32),
+                line_bin_off=np.zeros(nn + 1, np.int64), line_atom=np.zeros(0, np.int32))
+
+
 # physical_constants.h conversion factors used by the derived line data
 EV_TO_ERGS = 1.60217646e-12
 CM_TO_ANGS = 1.0e8
@@ -212,9 +235,10 @@ H_PLANCK = 6.6260755e-27
 SIGMA_TOT = 0.0265400193567
 
 
-def level_lines(nu_grid, lev_off, lev_E, lev_g, lev_ion, rng, A_min=1e4, A_max=1e8):
+def level_lines(nu_grid, lev_off, lev_E, lev_g, lev_ion, rng, A_min=1e4, A_max=1e8, A=None):
     """Level-resolved lines between every pair of levels of one ion with
-    E_u > E_l, Einstein A_ul log-uniform on [A_min, A_max] s^-1, and the
+    E_u > E_l, Einstein A_ul log-uniform on [A_min, A_max] s^-1 (or A, one
+    value per pair in (atom, lower level, upper level) order), and the
     quantities AtomicData.cpp:349-375 derives from them, in its operation
     order: nu = dE ev_to_ergs / h, B_ul = A c^2 / (2 h nu^3),
     B_lu = B_ul g_u / g_l, lam = c / nu * cm_to_angs,
@@ -229,7 +253,11 @@ def level_lines(nu_grid, lev_off, lev_E, lev_g, lev_ion, rng, A_min=1e4, A_max=1
                 if lev_ion[u] == lev_ion[l] and lev_E[u] > lev_E[l]:
                     ll.append(l); lu.append(u); at.append(a)
     ll, lu, at = np.array(ll, np.int32), np.array(lu, np.int32), np.array(at, np.int32)
-    A = 10.0 ** rng.uniform(math.log10(A_min), math.log10(A_max), len(ll))
+    if A is None:
+        A = 10.0 ** rng.uniform(math.log10(A_min), math.log10(A_max), len(ll))
+    else:
+        A = np.asarray(A, np.float64)
+        assert A.shape == ll.shape
     dE = lev_E[lu] - lev_E[ll]
     nu = dE * EV_TO_ERGS / H_PLANCK
     gl, gu = lev_g[ll].astype(np.float64), lev_g[lu].astype(np.float64)

@@ -3,17 +3,36 @@ GasState::solve_state(J_nu) under opacity_use_nlte) against the oracle:
 4 zones of the line-expansion ejecta on a grid spanning every line and
 edge, J_nu = W B_nu(T) for W = 1 and 0.1.
 
-The rate matrices of solve_nlte are ill-conditioned, so the bar is set by
-how far the reference algorithm itself moves under a change of rounding: the
-oracle built with fused multiply-adds (gcc -mfma -ffp-contract=fast) against
-the default build moves, with only atom 0 in NLTE, n_e by 4e-9, the
-populations by 1e-4 (L1, weighted by population) and the opacity and
-emissivity tables by 1.6e-5 at W = 1 and 3e-7 at W = 0.1 (relative, per
-entry above 1e-12 of the table's maximum); with
-both atoms in NLTE, n_e by 2.3e-4 at W = 1 and 1.3e-6 at W = 0.1, the
-populations by 1.3e-4, and a few bound-free-edge bins (Planck mean up to
-27%) follow the least determined populations.  The GPU (its own rounding,
-FMA contraction in the LU) must stay within a few times those shifts."""
+The rate matrices of solve_nlte are ill-conditioned.  At the oracle's final
+state (rate_conditions below, printed per case) their 2-norm condition
+numbers are 1e24-1e58 (W = 1) and 1e23-1e63 (W = 0.1) as built, and still
+2e6, 1e7, 1e12 and 7e16 for atom 0 in zones 0-3 at W = 1 (6e5, 2e7, 1e13,
+7e16 at W = 0.1; atom 1 5e5-7e12 at W = 1, 1e8-1e17 at W = 0.1) once rows
+and columns are scaled to unit maximum, so in the coolest zone a change of
+rounding alone moves the least determined populations by O(1).  Measured against that: the oracle built
+with fused multiply-adds (gcc -mfma -ffp-contract=fast) against the default
+build moves, with only atom 0 in NLTE, n_e by 4e-9, the populations by 1e-4
+(L1, weighted by population) and the opacity and emissivity tables by
+1.6e-5 at W = 1 and 3e-7 at W = 0.1 (relative, per entry above 1e-12 of
+the table's maximum); with both atoms in NLTE, n_e by 2.3e-4 at W = 1 and
+1.3e-6 at W = 0.1, the populations by 1.3e-4, and a few bound-free-edge
+bins (Planck mean up to 27%) follow the least determined populations.
+
+The GPU's own gaps (its rounding, FMA contraction in the LU) are of the
+same size: one NLTE atom, n_e 1.2e-9 / 1.8e-9, populations 1.1e-4 / 9.5e-6,
+tables 1.6e-5 / 2.3e-7 (W = 1 / 0.1); both atoms, n_e 2.2e-4 / 1.6e-7,
+populations 3.6e-4 / 2.7e-5, opacity median 3e-7 / 1.3e-8.  Each bar is
+about ten times the measured gap.  The FP32-refinement case at 49 levels
+factors in FP64 in LDS (its FP64 matrix fits), so it holds the FP64 bars.
+
+Per (zone, atom) the population gap follows the condition number where that
+dominates and n_e's shift elsewhere (printed as gap / kappa_eq eps): one
+atom at W = 1, zones 0-3: 8e-9/4e-10, 8e-10/2e-9, 1.2e-6/3e-4, 2.1e-4/14 --
+the coolest zone, with kappa_eq eps > 1, carries the gap, while zone 0
+follows the 1e-9 shift of n_e that its LTE populations are built from; with
+both atoms at W = 1 the 2e-4 shift of n_e, set through charge balance by the
+coolest zone's least determined populations, moves the warm zones' atoms by
+up to 5e-7 at kappa_eq eps ~ 1e-10."""
 import numpy as np
 import pytest
 import torch
@@ -35,13 +54,51 @@ def relerr(a, b, floor):
     return float(np.max(np.where(both, 0.0, d)))
 
 
+def rate_conditions(P, J, flags):
+    """2-norm condition numbers (as built, and after scaling rows then
+    columns to unit maximum) of the solve_nlte rate matrices of every (zone,
+    NLTE atom) at P's final n_e (the oracle's state after the build): LTE
+    populations at that n_e, radiative rates from the zone's J_nu, the
+    matrix AtomicSpecies::set_rates builds (AtomicSpecies_state.cpp:294-432)."""
+    import ctypes
+    from oracle import oracle_py
+    lib = oracle_py.load()
+    A = P.atoms_np
+    nlev, nbl = P.atoms.n_levels, max(P.atoms.n_blines, 1)
+    ne_z, T_z, elem_A = P.get("n_elec"), P.get("T"), np.asarray(P.a["elem_A"])
+    out = []
+    for z in range(NZ):
+        Jz = np.ascontiguousarray(J[z])
+        for a in np.flatnonzero(flags):
+            a = int(a)
+            Pic, Rci, lJ, lte = np.zeros(nlev), np.zeros(nlev), np.zeros(nbl), np.zeros(nlev)
+            ip, fr = np.zeros(P.atoms.n_ions), np.zeros(P.atoms.n_ions)
+            l0, i0 = int(A["lev_off"][a]), int(A["ion_off"][a])
+            lib.or_nlte_radiative_rates(ctypes.byref(P.atoms), a, ctypes.byref(P.nu), Jz.ctypes.data,
+                                        float(T_z[z]), int(elem_A[a]), float(P.op_params.line_velocity_width),
+                                        Pic.ctypes.data, Rci.ctypes.data, lJ.ctypes.data)
+            lib.or_solve_lte(ctypes.byref(P.atoms), a, float(ne_z[z]), float(T_z[z]), ip[i0:].ctypes.data,
+                             fr[i0:].ctypes.data, lte[l0:].ctypes.data)
+            n = int(A["lev_off"][a + 1]) - l0
+            R, lv = np.zeros(n * n), np.zeros(n)
+            assert lib.or_nlte_set_rates(ctypes.byref(P.atoms), a, float(ne_z[z]), float(T_z[z]), lJ.ctypes.data,
+                                         Pic.ctypes.data, Rci.ctypes.data, lte.ctypes.data, 1,
+                                         int(P.op_params.no_ground_recomb), R.ctypes.data, lv.ctypes.data) == n
+            M = R.reshape(n, n)
+            r = np.abs(M).max(1)
+            M1 = M / np.where(r > 0, r, 1)[:, None]
+            c = np.abs(M1).max(0)
+            out.append((np.linalg.cond(M), np.linalg.cond(M1 / np.where(c > 0, c, 1)[None, :])))
+    return np.array(out)
+
+
 # (W, flags, LU precision, n_e bound, population L1 bound, table bound or None)
 CASES = [
-    (1.0, (1, 0), abi.NLTE_FP64, 1e-7, 1e-3, 5e-5),
-    (0.1, (1, 0), abi.NLTE_FP64, 1e-7, 1e-3, 2e-6),
-    (0.1, (1, 0), abi.NLTE_FP32_REFINE, 1e-6, 1e-3, 2e-6),
-    (1.0, (1, 1), abi.NLTE_FP64, 2e-3, 2e-3, None),
-    (0.1, (1, 1), abi.NLTE_FP64, 2e-5, 2e-3, None),
+    (1.0, (1, 0), abi.NLTE_FP64, 1e-8, 1e-3, 5e-5),
+    (0.1, (1, 0), abi.NLTE_FP64, 2e-8, 1e-4, 2e-6),
+    (0.1, (1, 0), abi.NLTE_FP32_REFINE, 2e-8, 1e-4, 2e-6),
+    (1.0, (1, 1), abi.NLTE_FP64, 2e-3, 4e-3, None),
+    (0.1, (1, 1), abi.NLTE_FP64, 2e-6, 3e-4, None),
 ]
 
 
@@ -57,6 +114,10 @@ def test_nlte_opacity_build_matches_oracle(gpu, Wd, flags, prec, tol_ne, tol_pop
     sh = OracleBackend().opacity_build_nlte(Ph, J, np.array(flags, np.int32))
     print("status", st, sh, "evaluations", ev)
     assert st == sh == 0 and 2 < ev <= 102
+    kappa = rate_conditions(Ph, J, flags)
+    print("rate matrix condition numbers (as built / equilibrated)",
+          ", ".join("%.1e/%.1e" % tuple(k) for k in kappa))
+    assert np.all(np.isfinite(kappa)) and kappa[:, 0].min() > 1e17 and kappa[:, 1].max() > 1e16
     e = relerr(Pg.get("n_elec"), Ph.get("n_elec"), 1e-300)
     print("n_elec", e)
     assert e <= tol_ne
@@ -65,10 +126,16 @@ def test_nlte_opacity_build_matches_oracle(gpu, Wd, flags, prec, tol_ne, tol_pop
     e = max(np.abs(lg[z] - lh[z]).sum() / lh[z].sum() for z in range(NZ))
     print("lev_n L1", e)
     assert e <= tol_pop
+    # per (zone, NLTE atom): the atom's population gap against kappa eps
+    off = Ph.atoms_np["lev_off"]
+    gaps = np.array([np.abs(lg[z, off[a]:off[a + 1]] - lh[z, off[a]:off[a + 1]]).sum() /
+                     lh[z, off[a]:off[a + 1]].sum() for z in range(NZ) for a in np.flatnonzero(flags)])
+    print("per-system population gap / (kappa_eq eps):",
+          ", ".join("%.1e/%.1e" % (g, k * 2.2e-16) for g, k in zip(gaps, kappa[:, 1])))
     if tol_tab is None:
         ra = np.abs(Pg.get("abs") / np.maximum(Ph.get("abs"), 1e-300) - 1)
         print("abs median", np.median(ra))
-        assert np.median(ra) <= 1e-5
+        assert np.median(ra) <= 10 * (3e-7 if Wd == 1.0 else 1.3e-8)
         return
     for k in ("abs", "emis_cdf", "planck_mean", "rosseland_mean"):
         g, h = Pg.get(k), Ph.get(k)
