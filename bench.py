@@ -59,6 +59,8 @@ def parse():
                     help="only run a 1 GiB device copy (the HBM counter calibration) and exit")
     ap.add_argument("--nlte-only", action="store_true",
                     help="only the batched NLTE measurement (for profiling its kernel)")
+    ap.add_argument("--nlte-coupled-only", action="store_true",
+                    help="only the coupled-ion-stage NLTE batch (for profiling its kernels)")
     ap.add_argument("--mixed-only", action="store_true",
                     help="only the headline mixed-precision leg (for profiling its kernel)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -148,6 +150,9 @@ def main():
     gpu.enable_timing(True)
     if a.nlte_only:
         print(json.dumps(bench_nlte(gpu, torch, W, abi, np)))
+        return
+    if a.nlte_coupled_only:
+        print(json.dumps(bench_nlte_coupled(gpu, torch, W, abi, np)))
         return
     if not runtime.CONFIGS[a.config]["steady"]:
         print(json.dumps(bench_time_dependent(a, gpu, a.steps, a.warmup, a.packets)))
@@ -277,6 +282,8 @@ def main():
         nlte = bench_nlte(gpu, torch, W, abi, np)
         log("NLTE-coupled opacity build")
         nlte["opacity_build_nlte"] = bench_nlte_state(gpu, P, torch, abi, np)
+        log("NLTE coupled ion stages (256 per-atom systems, n up to 801)")
+        nlte["coupled_ion_stages"] = bench_nlte_coupled(gpu, torch, W, abi, np)
         out["nlte"] = nlte
     if not a.no_extra:
         log("2D time-dependent outflow (C4)")
@@ -528,6 +535,71 @@ def bench_nlte(gpu, torch, W, abi, np):
             "kernel_ms": k_ms, "kernel_fp64_ms": k64_ms,
             "lu_tflops": flops / (k_ms * 1e-3) / 1e12 if k_ms > 0 else None,
             "autotune": rep.strip().splitlines()[-1] if rep else None}
+
+
+def bench_nlte_coupled(gpu, torch, W, abi, np):
+    """The reference-shaped NLTE batch: one system per (zone, atom) with the
+    atom's four ion stages coupled (AtomicSpecies::solve_nlte over every
+    level of the atom, AtomicSpecies_state.cpp:131-196), 256 zones, stages of
+    U{20..200} levels, n up to 801 (workloads.nlte_systems_coupled): the
+    batched launch in FP64 and with FP32 + refinement (systems above the
+    201-level FP32 LDS limit solve in FP64), the per-system FP64 drop-in on a
+    sample of 32 systems, and the error against the CPU oracle on a sample
+    of 16."""
+    from oracle.oracle_py import OracleBackend
+    sy = W.nlte_systems_coupled(n_sys=256, seed=20261204)
+    d = {k: torch.from_numpy(np.ascontiguousarray(sy[k])).cuda() for k in
+         ("n", "rate_off", "vec_off", "rates", "lte")}
+    nv = len(sy["lte"])
+    ro, vo, ns = sy["rate_off"], sy["vec_off"], sy["n"]
+    out = {"systems": int(sy["n_sys"]), "n_range": [int(ns.min()), int(ns.max())],
+           "lu_gflop": float(np.sum(2.0 / 3.0 * ns.astype(np.float64) ** 3)) / 1e9}
+    bt = torch.zeros(nv, dtype=torch.float64, device="cuda:0")
+    itt = torch.zeros(sy["n_sys"], dtype=torch.int32, device="cuda:0")
+    rep = gpu.nlte_autotune(sy["n_sys"], d["n"], d["rate_off"], d["vec_off"], d["rates"], d["lte"], bt, None, itt)
+    out["autotune"] = [l for l in rep.strip().splitlines() if l.startswith("chosen")] if rep else None
+    res = {}
+    for name, prec in (("batched_fp64", abi.NLTE_FP64), ("batched_mixed", abi.NLTE_FP32_REFINE)):
+        b = torch.zeros(nv, dtype=torch.float64, device="cuda:0")
+        it = torch.zeros(sy["n_sys"], dtype=torch.int32, device="cuda:0")
+        gpu.nlte_batched(sy["n_sys"], d["n"], d["rate_off"], d["vec_off"], d["rates"], d["lte"], b, None,
+                         prec, 1e-13, it)
+        torch.cuda.synchronize()
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            gpu.nlte_batched(sy["n_sys"], d["n"], d["rate_off"], d["vec_off"], d["rates"], d["lte"], b, None,
+                             prec, 1e-13, it)
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / reps
+        out[name + "_solves_per_s"] = sy["n_sys"] / t
+        out[name + "_kernel_ms"] = gpu.last_ms("nlte")
+        res[name] = b.cpu().numpy()
+        itv = it.cpu().numpy()
+        out[name + "_fp64_solved"] = int((itv <= 0).sum())
+    # per-system drop-in on a sample
+    sample = np.arange(0, sy["n_sys"], sy["n_sys"] // 32)[:32]
+    m = int(ns.max())
+    work = torch.zeros(m * m + m, dtype=torch.float64, device="cuda:0")
+    bp = torch.zeros(nv, dtype=torch.float64, device="cuda:0")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in sample:
+        gpu.nlte_solve(int(ns[s]), d["rates"][ro[s]:], d["lte"][vo[s]:], bp[vo[s]:], None, work)
+    torch.cuda.synchronize()
+    out["plain_fp64_solves_per_s"] = len(sample) / (time.perf_counter() - t0)
+    out["batched_fp64_over_plain"] = out["batched_fp64_solves_per_s"] / out["plain_fp64_solves_per_s"]
+    ob = OracleBackend()
+    errs = {k: 0.0 for k in res}
+    for s in sample[:16]:
+        n = int(ns[s])
+        ref = np.zeros(n)
+        ob.nlte_solve(n, np.ascontiguousarray(sy["rates"][ro[s]:ro[s + 1]]),
+                      np.ascontiguousarray(sy["lte"][vo[s]:vo[s + 1]]), ref)
+        for k, x in res.items():
+            errs[k] = max(errs[k], float(np.max(np.abs(x[vo[s]:vo[s + 1]] - ref)) / np.max(np.abs(ref))))
+    out["max_rel_err_vs_oracle"] = errs
+    return out
 
 
 def bench_adjoint(gpu, torch, np):

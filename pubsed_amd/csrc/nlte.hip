@@ -162,6 +162,221 @@ __device__ void lu_solve_f64(int n, double* M, int* perm, double* x, ArgMax* red
   }
 }
 
+// Blocked form of the same elimination for systems too large for LDS, in
+// device memory: the augmented n x (n+1) matrix [M | e_{n-1}] (row stride
+// n + 1) is reduced to [U | y] panel by panel, so the right-hand side rides
+// along with the row interchanges and eliminations (forward substitution
+// done with them) and the multipliers are not kept.  Per panel of BLU_NB
+// columns: the panel's rows j0..n-1 are copied to LDS and factored there
+// column by column with the first-largest pivot (gsl_linalg_LU_decomp's
+// choice); the interchanges are applied to the columns right of the panel
+// as one composed gather; the row block U12 = L11^-1 A12 is formed one
+// thread per column; the trailing block A22 -= L21 U12 is updated with every
+// row read and written once per panel (lanes over consecutive columns,
+// wavefronts over rows, the L21 row an LDS broadcast).  Back substitution
+// runs in blocks of BLU_NB rows from the bottom: the diagonal block from
+// LDS by one thread, the rows above updated with a dot product over the
+// block.
+constexpr int BLU_NB = 16;               // panel width
+constexpr int BLU_PS = BLU_NB + 1;       // LDS row stride of the panel (doubles)
+constexpr int BLU_CW = 64;               // columns per trailing-update chunk
+constexpr int BLU_RU = 8;                // rows per thread per trailing-update pass
+
+constexpr int BLU_INTS = 5 * BLU_NB + 1;  // panel pivots, composed interchanges
+
+// dynamic LDS of lu_solve_f64_blocked: the panel and the ints (the back
+// substitution reuses the panel's space for y and one diagonal block)
+__host__ __device__ inline size_t blocked_lds_bytes(int n)
+{
+  const size_t panel = (size_t)n * BLU_PS, back = (size_t)n + BLU_NB * BLU_NB;
+  return sizeof(double) * (panel > back ? panel : back) + sizeof(int) * BLU_INTS;
+}
+
+// [M | e_{n-1}] with row stride n + 1 for lu_solve_f64_blocked
+__device__ void assemble_augmented(int n, const double* R, const double* lte, double* M)
+{
+  for (int i = threadIdx.x; i < n; i += blockDim.x) M[(size_t)i * (n + 1) + n] = (i == n - 1) ? 1.0 : 0.0;
+  assemble<double>(n, R, lte, M, n + 1);
+}
+
+// lds: blocked_lds_bytes(n) bytes of dynamic LDS
+__device__ void lu_solve_f64_blocked(int n, double* __restrict__ M, double* __restrict__ x,
+                                     double* __restrict__ lds, ArgMax* red, int* singular)
+{
+  const int tid = threadIdx.x, NT = blockDim.x;
+  const int ld = n + 1;
+  double* P = lds;                                  // panel rows, stride BLU_PS
+  int* piv = (int*)((char*)lds + blocked_lds_bytes(n) - sizeof(int) * BLU_INTS);   // BLU_INTS ints
+  for (int j0 = 0; j0 < n; j0 += BLU_NB) {
+    const int jb = min(BLU_NB, n - j0);
+    const int m = n - j0;
+    // the last, narrower panel is padded with zero columns (the trailing
+    // update runs over all BLU_NB of them)
+    for (int idx = tid; idx < m * BLU_NB; idx += NT) {
+      const int r = idx / BLU_NB, c = idx % BLU_NB;
+      P[r * BLU_PS + c] = c < jb ? M[(size_t)(j0 + r) * ld + j0 + c] : 0.0;
+    }
+    __syncthreads();
+    for (int t = 0; t < jb; ++t) {
+      ArgMax am = {-1.0, m};
+      for (int r = t + tid; r < m; r += NT) am = better(am, ArgMax{fabs(P[r * BLU_PS + t]), r});
+      const ArgMax p = block_argmax(am, red);
+      const int pr = p.i < m ? p.i : t;
+      if (tid == 0) piv[t] = pr;
+      if (pr != t)
+        for (int c = tid; c < jb; c += NT) {
+          const double a = P[t * BLU_PS + c];
+          P[t * BLU_PS + c] = P[pr * BLU_PS + c];
+          P[pr * BLU_PS + c] = a;
+        }
+      __syncthreads();
+      const double ajj = P[t * BLU_PS + t];
+      if (ajj == 0) {
+        if (tid == 0) *singular = 1;
+        continue;
+      }
+      for (int r = t + 1 + tid; r < m; r += NT) {
+        const double l = P[r * BLU_PS + t] / ajj;
+        P[r * BLU_PS + t] = l;
+        for (int c = t + 1; c < jb; ++c) P[r * BLU_PS + c] -= l * P[t * BLU_PS + c];
+      }
+      __syncthreads();
+    }
+    // the factored panel back to device memory (U11 and L21; only U11 is
+    // read again, by the back substitution)
+    for (int idx = tid; idx < m * jb; idx += NT) {
+      const int r = idx / jb, c = idx - r * jb;
+      M[(size_t)(j0 + r) * ld + j0 + c] = P[r * BLU_PS + c];
+    }
+    // the panel's interchanges on the columns to its right (the rhs
+    // included), composed into one gather: the rows they touch (at most
+    // 2 BLU_NB) and, for each, the row whose value ends there
+    int* sw_dst = piv + BLU_NB;
+    int* sw_src = sw_dst + 2 * BLU_NB;
+    int* sw_n = sw_src + 2 * BLU_NB;
+    if (tid == 0) {
+      int nt = 0;
+      for (int t = 0; t < jb; ++t) {
+        const int rr[2] = {t, piv[t]};
+        for (int e = 0; e < 2; ++e) {
+          bool seen = false;
+          for (int q = 0; q < nt; ++q) seen = seen || sw_dst[q] == rr[e];
+          if (!seen) { sw_dst[nt] = rr[e]; sw_src[nt] = rr[e]; ++nt; }
+        }
+      }
+      for (int t = 0; t < jb; ++t) {
+        int qa = 0, qb = 0;
+        for (int q = 0; q < nt; ++q) {
+          if (sw_dst[q] == t) qa = q;
+          if (sw_dst[q] == piv[t]) qb = q;
+        }
+        const int v = sw_src[qa];
+        sw_src[qa] = sw_src[qb];
+        sw_src[qb] = v;
+      }
+      *sw_n = nt;
+    }
+    __syncthreads();
+    const int nsw = *sw_n;
+    for (int k = j0 + jb + tid; k <= n; k += NT) {
+      double v[2 * BLU_NB];
+#pragma unroll
+      for (int q = 0; q < 2 * BLU_NB; ++q)
+        if (q < nsw) v[q] = M[(size_t)(j0 + sw_src[q]) * ld + k];
+#pragma unroll
+      for (int q = 0; q < 2 * BLU_NB; ++q)
+        if (q < nsw) M[(size_t)(j0 + sw_dst[q]) * ld + k] = v[q];
+    }
+    __syncthreads();
+    // U12 = L11^-1 A12, one thread per column: the column's jb entries are
+    // loaded together, substituted against L11 from LDS, stored back
+    for (int k = j0 + jb + tid; k <= n; k += NT) {
+      double u[BLU_NB];
+#pragma unroll
+      for (int t = 0; t < BLU_NB; ++t)
+        if (t < jb) u[t] = M[(size_t)(j0 + t) * ld + k];
+#pragma unroll
+      for (int t = 1; t < BLU_NB; ++t)
+        if (t < jb) {
+#pragma unroll
+          for (int q = 0; q < BLU_NB - 1; ++q)
+            if (q < t) u[t] -= P[t * BLU_PS + q] * u[q];
+        }
+#pragma unroll
+      for (int t = 0; t < BLU_NB; ++t)
+        if (t < jb) M[(size_t)(j0 + t) * ld + k] = u[t];
+    }
+    __syncthreads();
+    // A22 -= L21 U12: lanes over BLU_CW consecutive columns (coalesced
+    // rows), wavefronts over rows; each thread keeps its column of U12 in
+    // registers and updates BLU_RU rows per pass with their loads issued
+    // together; the L21 row is an LDS broadcast
+    const int rows_per = NT / BLU_CW;
+    const int col = tid % BLU_CW, rg = tid / BLU_CW;
+    const int i0 = j0 + jb;
+    for (int c0 = i0; c0 <= n; c0 += BLU_CW) {
+      const int k = c0 + col;
+      if (k > n) continue;
+      double u[BLU_NB];
+#pragma unroll
+      for (int t = 0; t < BLU_NB; ++t) u[t] = t < jb ? M[(size_t)(j0 + t) * ld + k] : 0.0;
+      int i = i0 + rg;
+      for (; i + (BLU_RU - 1) * rows_per < n; i += BLU_RU * rows_per) {
+        double v[BLU_RU];
+#pragma unroll
+        for (int e = 0; e < BLU_RU; ++e) v[e] = M[(size_t)(i + e * rows_per) * ld + k];
+#pragma unroll
+        for (int e = 0; e < BLU_RU; ++e) {
+          const double* pl = P + (size_t)(i + e * rows_per - j0) * BLU_PS;
+#pragma unroll
+          for (int t = 0; t < BLU_NB; ++t) v[e] -= pl[t] * u[t];
+        }
+#pragma unroll
+        for (int e = 0; e < BLU_RU; ++e) M[(size_t)(i + e * rows_per) * ld + k] = v[e];
+      }
+      for (; i < n; i += rows_per) {
+        const double* pl = P + (size_t)(i - j0) * BLU_PS;
+        double v = M[(size_t)i * ld + k];
+#pragma unroll
+        for (int t = 0; t < BLU_NB; ++t) v -= pl[t] * u[t];
+        M[(size_t)i * ld + k] = v;
+      }
+    }
+    __syncthreads();
+  }
+  // back substitution U x = y, y in column n; the running right-hand side in LDS
+  double* Y = lds;                                  // n doubles
+  double* D = lds + n;                              // BLU_NB x BLU_NB diagonal block
+  for (int i = tid; i < n; i += NT) Y[i] = M[(size_t)i * ld + n];
+  __syncthreads();
+  const int nblk = (n + BLU_NB - 1) / BLU_NB;
+  for (int bk = nblk - 1; bk >= 0; --bk) {
+    const int b0 = bk * BLU_NB, bs = min(BLU_NB, n - b0);
+    for (int idx = tid; idx < bs * bs; idx += NT) {
+      const int r = idx / bs, c = idx - r * bs;
+      D[r * BLU_NB + c] = M[(size_t)(b0 + r) * ld + b0 + c];
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (int r = bs - 1; r >= 0; --r) {
+        double v = Y[b0 + r];
+        for (int c = r + 1; c < bs; ++c) v -= D[r * BLU_NB + c] * Y[b0 + c];
+        Y[b0 + r] = v / D[r * BLU_NB + r];
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < b0; i += NT) {
+      const double* row = M + (size_t)i * ld + b0;
+      double v = Y[i];
+      for (int c = 0; c < bs; ++c) v -= row[c] * Y[b0 + c];
+      Y[i] = v;
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < n; i += NT) x[i] = Y[i];
+  __syncthreads();
+}
+
 __device__ void write_pops(int n, const double* x, const double* lte, double* b, double* pop)
 {
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -1126,8 +1341,13 @@ __global__ void __launch_bounds__(NLTE_MAX_THREADS) nlte_batched_mixed_kernel(Ba
     } else {
       // FP64 fallback in the workgroup's device slot
       if (tid == 0) sing_f64 = 0;
-      assemble<double>(n, R, lte, M64, n);
-      lu_solve_f64(n, M64, perm, x, red, &sing_f64);
+      if (blocked_lds_bytes(n) <= lds_bytes) {
+        assemble_augmented(n, R, lte, M64);
+        lu_solve_f64_blocked(n, M64, x, (double*)lds_raw, red, &sing_f64);
+      } else {
+        assemble<double>(n, R, lte, M64, n);
+        lu_solve_f64(n, M64, perm, x, red, &sing_f64);
+      }
       write_pops(n, x, lte, bout, pout);
       if (A.iters && tid == 0) A.iters[s] = -1;
     }
@@ -1146,7 +1366,6 @@ __global__ void __launch_bounds__(NLTE_MAX_THREADS) nlte_batched_f64_lds_kernel(
   __shared__ int sh, sing_f64, flag;
   double* slot = A.slots + (size_t)blockIdx.x * A.slot_stride;
   const int tid = threadIdx.x;
-  (void)lds_bytes;
   while (true) {
     const int s = next_system(A, &sh);
     if (s < 0) break;
@@ -1163,8 +1382,13 @@ __global__ void __launch_bounds__(NLTE_MAX_THREADS) nlte_batched_f64_lds_kernel(
       solve_f64_lds<NB>(n, R, lte, lds_d, diag, r, x, &flag);
     } else {
       if (tid == 0) sing_f64 = 0;
-      assemble<double>(n, R, lte, M64, n);
-      lu_solve_f64(n, M64, perm, x, red, &sing_f64);
+      if (blocked_lds_bytes(n) <= lds_bytes) {
+        assemble_augmented(n, R, lte, M64);
+        lu_solve_f64_blocked(n, M64, x, lds_d, red, &sing_f64);
+      } else {
+        assemble<double>(n, R, lte, M64, n);
+        lu_solve_f64(n, M64, perm, x, red, &sing_f64);
+      }
     }
     write_pops(n, x, lte, A.b + A.vec_off[s], A.pop ? A.pop + A.vec_off[s] : nullptr);
     if (A.iters && tid == 0) A.iters[s] = 0;
@@ -1232,8 +1456,9 @@ int batched_impl(sgpu_ctx* ctx, int32_t nsys, const int32_t* n, const int64_t* r
   SGPU_REQUIRE((panel == 8 || panel == 16) && wg_per_cu >= 1,
                "panel width %d (8 or 16) / workgroups per CU %d", panel, wg_per_cu);
   const unsigned nwg = (unsigned)std::min<int64_t>((int64_t)ctx->n_cu * wg_per_cu, nsys);
-  // slot: 7 vectors + perm + an FP64 matrix for the fallback / FP64 mode
-  const size_t slot = (size_t)max_n * max_n + 8 * (size_t)max_n + 64;
+  // slot: 7 vectors + perm + an FP64 matrix (augmented by its rhs) for the
+  // fallback / FP64 mode
+  const size_t slot = (size_t)max_n * (max_n + 1) + 10 * (size_t)max_n + 64;
   double* slots = (double*)scratch(ctx, "nlte_slots", sizeof(double) * slot * nwg, &st);
   if (st) return st;
   hipLaunchKernelGGL(order_kernel, dim3(1), dim3(1024), 0, ctx->stream, n, nsys, order, counter);

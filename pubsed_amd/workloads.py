@@ -313,3 +313,68 @@ def nlte_systems(n_sys=1024, n_min=20, n_max=200, T=1.0e4, seed=20261203,
         rates[rate_off[s]:rate_off[s + 1]] = R.ravel()
         lte[vec_off[s]:vec_off[s + 1]] = l
     return dict(n_sys=n_sys, n=ns, rate_off=rate_off, vec_off=vec_off, rates=rates, lte=lte)
+
+
+def nlte_systems_coupled(n_sys=256, n_ions=4, n_min=20, n_max=200, T=1.0e4, seed=20261204,
+                         chi=(7.9, 16.2, 30.7, 54.8), coll_median=1.0e2, coll_sigma_dex=2.0,
+                         A_min=1e4, A_max=1e8, bf_median=1.0e3):
+    """One rate system per atom of a zone, all its ion stages coupled, as
+    AtomicSpecies::solve_nlte solves them (AtomicSpecies_state.cpp:131-196
+    over the levels of every ion of the atom; set_rates :294-432): n_ions
+    stages of U{n_min..n_max} levels each plus the bare nucleus, n up to
+    n_ions n_max + 1.  Within a stage the bound-bound rates of
+    nlte_systems (lognormal collisions on every ordered pair, Einstein A on
+    every downward pair); between stages the bound-free pair of
+    set_rates: each level to the ground level of the next stage
+    (photoionisation, lognormal about bf_median s^-1) and back
+    (recombination, lognormal about 1e-2 bf_median).  Levels: stage k's
+    ground at the sum of the first k ionisation energies chi (eV), excited
+    levels uniform up to 12 eV above it, g on 1..9; LTE populations
+    Boltzmann within each stage times a stage fraction drawn uniform on
+    [0.1, 1] and normalised; rates multiplied by the source level's LTE
+    population (:429-431).  The stages' excitation energies span far less
+    than the chi ladder, so the LTE vector spans the dynamic range the
+    stage fractions and the 12 eV excitation give (about 1e-7)."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(n_min, n_max + 1, (n_sys, n_ions))
+    ns = (sizes.sum(1) + 1).astype(np.int32)
+    rate_off = np.zeros(n_sys + 1, np.int64)
+    vec_off = np.zeros(n_sys + 1, np.int64)
+    rate_off[1:] = np.cumsum(ns.astype(np.int64) ** 2)
+    vec_off[1:] = np.cumsum(ns.astype(np.int64))
+    rates = np.empty(rate_off[-1], np.float64)
+    lte = np.empty(vec_off[-1], np.float64)
+    for s in range(n_sys):
+        n = int(ns[s])
+        stage = np.concatenate([np.full(m, k) for k, m in enumerate(sizes[s])] + [[n_ions]])
+        ground = np.concatenate([[0], np.cumsum(sizes[s])])     # first level of each stage
+        E = np.zeros(n)
+        g = np.ones(n)
+        frac = rng.uniform(0.1, 1.0, n_ions + 1)
+        w = np.zeros(n)
+        for k in range(n_ions):
+            a, b = ground[k], ground[k + 1]
+            E[a:b] = np.concatenate([[0.0], np.sort(rng.uniform(0.0, 12.0, b - a - 1))])
+            g[a:b] = rng.integers(1, 10, b - a)
+            wk = g[a:b] * np.exp(-E[a:b] / K_EV / T)
+            w[a:b] = frac[k] * wk / wk.sum()
+            E[a:b] += sum(chi[:k])
+        E[n - 1] = sum(chi[:n_ions])
+        w[n - 1] = frac[n_ions]
+        l = np.maximum(w / w.sum(), 1e-30)
+        R = np.zeros((n, n))
+        for k in range(n_ions):
+            a, b = ground[k], ground[k + 1]
+            m = b - a
+            Rk = 10.0 ** (math.log10(coll_median) + coll_sigma_dex * rng.standard_normal((m, m)))
+            Rk += np.tril(rng.uniform(A_min, A_max, (m, m)), -1)
+            np.fill_diagonal(Rk, 0.0)
+            R[a:b, a:b] = Rk
+            c = ground[k + 1]                                    # next stage's ground level
+            R[a:b, c] = 10.0 ** (math.log10(bf_median) + coll_sigma_dex * rng.standard_normal(m))
+            R[c, a:b] = 10.0 ** (math.log10(1e-2 * bf_median) + coll_sigma_dex * rng.standard_normal(m))
+        R *= l[:, None]
+        rates[rate_off[s]:rate_off[s + 1]] = R.ravel()
+        lte[vec_off[s]:vec_off[s + 1]] = l
+    return dict(n_sys=n_sys, n=ns, rate_off=rate_off, vec_off=vec_off, rates=rates, lte=lte,
+                stage_sizes=sizes.astype(np.int32))

@@ -130,6 +130,38 @@ def test_every_launch_shape(gpu, precision, n_max, bar):
         gpu.nlte_set_tuning(precision, 256, 16, 1)
 
 
+@pytest.mark.parametrize("precision", [abi.NLTE_FP64, abi.NLTE_FP32_REFINE])
+def test_coupled_ion_stages(gpu, precision):
+    """Per-atom systems as AtomicSpecies::solve_nlte solves them
+    (AtomicSpecies_state.cpp:131-196): four ion stages of U{20..200} levels
+    coupled by bound-free rates, n up to 801, beside small systems in one
+    batch.  Above the FP32 LDS limit (201 levels) the FP32 + refinement
+    launch solves in FP64 (iters -1); every system within 1e-11 of the
+    oracle."""
+    big = W.nlte_systems_coupled(n_sys=10, seed=21)
+    small = W.nlte_systems(n_sys=6, n_min=20, n_max=120, seed=22)
+    sy = dict(n_sys=16, n=np.concatenate([big["n"], small["n"]]).astype(np.int32))
+    sy["rate_off"] = np.concatenate([[0], np.cumsum(sy["n"].astype(np.int64) ** 2)])
+    sy["vec_off"] = np.concatenate([[0], np.cumsum(sy["n"].astype(np.int64))])
+    sy["rates"] = np.concatenate([big["rates"], small["rates"]])
+    sy["lte"] = np.concatenate([big["lte"], small["lte"]])
+    assert sy["n"].max() > 600
+    ref = oracle_solutions(sy)
+    d = to_dev(sy)
+    print("case nlte coupled ion stages precision=%d n_max=%d" % (precision, sy["n"].max()), flush=True)
+    gpu.nlte_batched(sy["n_sys"], d["n"], d["rate_off"], d["vec_off"], d["rates"], d["lte"], d["b"],
+                     d["pop"], precision, 1e-13, d["iters"])
+    gpu.sync()
+    errs = normwise(sy, d["b"].cpu().numpy(), ref)
+    print("max err", errs.max())
+    assert errs.max() <= 1e-11
+    pop = d["pop"].cpu().numpy()
+    assert np.allclose(pop, np.maximum(d["b"].cpu().numpy() * sy["lte"], 1e-30), rtol=1e-15, atol=0)
+    if precision == abi.NLTE_FP32_REFINE:
+        it = d["iters"].cpu().numpy()
+        assert (it[sy["n"] > 201] == -1).all()
+
+
 def test_uneven_sizes_edge(gpu):
     """Tiny, panel-straddling and LDS-limit sizes in one batch (n = 2 ... 201,
     the largest FP32 matrix with an odd row stride in 160 KiB of LDS), plus 202

@@ -45,6 +45,36 @@ def test_lu_solve_matches_lapack():
         assert np.allclose(pop, np.maximum(b * l, 1e-30), rtol=1e-15, atol=0)
 
 
+def test_coupled_ion_stage_systems_match_lapack():
+    """The per-atom systems of AtomicSpecies::solve_nlte
+    (AtomicSpecies_state.cpp:131-196): four ion stages of U{20..200} levels
+    coupled by bound-free rates, n up to 801 (workloads.nlte_systems_coupled),
+    against LAPACK; their detailed-balance form gives b = 1."""
+    sy = W.nlte_systems_coupled(n_sys=6, seed=7)
+    assert sy["n"].max() > 400 and np.all(sy["n"] == sy["stage_sizes"].sum(1) + 1)
+    ob = OracleBackend()
+    for s in range(sy["n_sys"]):
+        n = int(sy["n"][s])
+        R = sy["rates"][sy["rate_off"][s]:sy["rate_off"][s + 1]].reshape(n, n)
+        l = sy["lte"][sy["vec_off"][s]:sy["vec_off"][s + 1]]
+        # coupling only between adjacent stages, through the next stage's ground level
+        g = np.concatenate([[0], np.cumsum(sy["stage_sizes"][s])])
+        stage = np.searchsorted(g, np.arange(n), side="right") - 1
+        i, j = np.nonzero(R)
+        cross = stage[i] != stage[j]
+        assert np.all(np.abs(stage[i] - stage[j])[cross] == 1)
+        assert np.all(np.isin(np.where(stage[i] < stage[j], j, i)[cross], g[1:]))
+        b = np.zeros(n)
+        assert ob.nlte_solve(n, np.ascontiguousarray(R), np.ascontiguousarray(l), b) == 0
+        e = np.zeros(n); e[-1] = 1
+        ref = np.linalg.solve(assemble(R, l), e)
+        assert np.max(np.abs(b - ref)) <= 1e-11 * np.max(np.abs(ref))
+        # symmetrised rates R_ij lte_i = R_ji lte_j: detailed balance, b = 1
+        Rs = np.sqrt(R * R.T)
+        assert ob.nlte_solve(n, np.ascontiguousarray(Rs), np.ascontiguousarray(l), b) == 0
+        assert np.max(np.abs(b - 1)) < 1e-8
+
+
 def test_lu_factors_reconstruct():
     lib = oracle_py.load()
     rng = np.random.default_rng(4)
