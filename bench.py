@@ -424,6 +424,15 @@ def bench_time_dependent(a, gpu, steps, warmup, packets, brief=False, graph=True
     per = {k: c[k] / max(c["n_packets"], 1) for k in ("n_events", "n_binshift", "n_redist", "n_escaped",
                                                        "n_stopped", "n_scatter")}
     spec = P.get("opt_click")
+    events_per_s = cw["n_events"] / (prop_avg * 1e-3) if prop_avg else None
+    # the per-event J_nu atomic's ceiling at this table's size (1 GB here,
+    # far beyond the L2 and MALL)
+    jbuf = torch.zeros(P.nz * P.n_nu, dtype=torch.float64, device="cuda:0")
+    adds = 1 << 30
+    gpu.atomic_rate(jbuf, adds)
+    at_ms = gpu.atomic_rate(jbuf, adds)
+    del jbuf
+    atomic_ceiling = adds / (at_ms * 1e-3)
     res = {
         "metric": "packets/s & NLTE solves/s: batched/mixed vs plain GPU path at matched MC noise",
         "value": value, "unit": "packets/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
@@ -441,6 +450,9 @@ def bench_time_dependent(a, gpu, steps, warmup, packets, brief=False, graph=True
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": PROP_KERNEL % P.grid.kind, "kernel_ms": prop_avg,
                      "bytes_per_packet": bpk,
+                     "events_per_s": events_per_s,
+                     "jnu_atomic_ceiling_per_s": atomic_ceiling,
+                     "events_over_atomic_ceiling": events_per_s / atomic_ceiling if events_per_s else None,
                      "measured_on": "the last warm-up step, run uncaptured with HIP event timing "
                                     "(%d packets)" % cw["n_packets"]},
         "opacity_build": {"kernel_ms": float(np.mean(op_ms)), "zones": P.nz,

@@ -93,19 +93,39 @@ __device__ double block_max(double v, double* red)
   return r;
 }
 
-// Assemble M (row-major, n x n) from rates R (row-major) and lte into dst.
+// Assemble M (row-major, n x n, row stride ld) from rates R (row-major) and
+// lte: M_ii = -sum_j R_ij (summed in j order), M_ij = R_ji, last row lte.
+// The transpose reads R along its rows (coalesced), eight loads in flight
+// per thread, and scatters the writes.
 template <class T>
-__device__ void assemble(int n, const double* R, const double* lte, T* M, int ld)
+__device__ __forceinline__ void assemble(int n, const double* __restrict__ R, const double* __restrict__ lte,
+                                         T* __restrict__ M, int ld)
 {
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     double Rout = 0.0;
     for (int j = 0; j < n; ++j) Rout += R[(size_t)i * n + j];
     if (i != n - 1) M[(size_t)i * ld + i] = (T)(-1 * Rout);
   }
-  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
-    int i = idx / n, j = idx - i * n;
-    if (i == n - 1) M[(size_t)i * ld + j] = (T)lte[j];
-    else if (i != j) M[(size_t)i * ld + j] = (T)R[(size_t)j * n + i];
+  const int NT = blockDim.x, nn = n * n;
+  int jj = threadIdx.x / n, ii = threadIdx.x - jj * n;   // R[jj][ii] = M[ii][jj]
+  const int dj = NT / n, di = NT - dj * n;
+  for (int base = threadIdx.x; base < nn; base += 8 * NT) {
+    double t[8];
+    int ri[8], rj[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      ri[u] = ii;
+      rj[u] = jj;
+      t[u] = base + u * NT < nn ? R[(size_t)jj * n + ii] : 0.0;
+      ii += di;
+      jj += dj;
+      if (ii >= n) { ii -= n; ++jj; }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = ri[u], j = rj[u];
+      if (base + u * NT < nn && (i == n - 1 || i != j)) M[(size_t)i * ld + j] = (T)(i == n - 1 ? lte[j] : t[u]);
+    }
   }
   __syncthreads();
 }
@@ -193,14 +213,14 @@ __host__ __device__ inline size_t blocked_lds_bytes(int n)
 }
 
 // [M | e_{n-1}] with row stride n + 1 for lu_solve_f64_blocked
-__device__ void assemble_augmented(int n, const double* R, const double* lte, double* M)
+__device__ __forceinline__ void assemble_augmented(int n, const double* R, const double* lte, double* M)
 {
   for (int i = threadIdx.x; i < n; i += blockDim.x) M[(size_t)i * (n + 1) + n] = (i == n - 1) ? 1.0 : 0.0;
   assemble<double>(n, R, lte, M, n + 1);
 }
 
 // lds: blocked_lds_bytes(n) bytes of dynamic LDS
-__device__ void lu_solve_f64_blocked(int n, double* __restrict__ M, double* __restrict__ x,
+__device__ __forceinline__ void lu_solve_f64_blocked(int n, double* __restrict__ M, double* __restrict__ x,
                                      double* __restrict__ lds, ArgMax* red, int* singular)
 {
   const int tid = threadIdx.x, NT = blockDim.x;
@@ -212,9 +232,18 @@ __device__ void lu_solve_f64_blocked(int n, double* __restrict__ M, double* __re
     const int m = n - j0;
     // the last, narrower panel is padded with zero columns (the trailing
     // update runs over all BLU_NB of them)
-    for (int idx = tid; idx < m * BLU_NB; idx += NT) {
-      const int r = idx / BLU_NB, c = idx % BLU_NB;
-      P[r * BLU_PS + c] = c < jb ? M[(size_t)(j0 + r) * ld + j0 + c] : 0.0;
+    for (int base = 0; base < m * BLU_NB; base += 4 * NT) {
+      double v[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int idx = base + e * NT + tid, r = idx / BLU_NB, c = idx % BLU_NB;
+        v[e] = (idx < m * BLU_NB && c < jb) ? M[(size_t)(j0 + r) * ld + j0 + c] : 0.0;
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int idx = base + e * NT + tid;
+        if (idx < m * BLU_NB) P[(idx / BLU_NB) * BLU_PS + idx % BLU_NB] = v[e];
+      }
     }
     __syncthreads();
     for (int t = 0; t < jb; ++t) {
@@ -254,27 +283,35 @@ __device__ void lu_solve_f64_blocked(int n, double* __restrict__ M, double* __re
     int* sw_dst = piv + BLU_NB;
     int* sw_src = sw_dst + 2 * BLU_NB;
     int* sw_n = sw_src + 2 * BLU_NB;
-    if (tid == 0) {
-      int nt = 0;
+    if (tid < WAVE) {
+      // lane q holds touched row dst (the panel's rows 0..jb-1 in lanes
+      // 0..jb-1, then each pivot row below them once) and, once the
+      // interchanges are applied in order, the row src whose value ends there
+      const int lane = tid;
+      int dst = lane < jb ? lane : -1;
+      int nt = jb;
       for (int t = 0; t < jb; ++t) {
-        const int rr[2] = {t, piv[t]};
-        for (int e = 0; e < 2; ++e) {
-          bool seen = false;
-          for (int q = 0; q < nt; ++q) seen = seen || sw_dst[q] == rr[e];
-          if (!seen) { sw_dst[nt] = rr[e]; sw_src[nt] = rr[e]; ++nt; }
+        const int r = piv[t];
+        if (r >= jb && __ballot(dst == r) == 0) {
+          if (lane == nt) dst = r;
+          ++nt;
         }
       }
+      int src = dst;
       for (int t = 0; t < jb; ++t) {
-        int qa = 0, qb = 0;
-        for (int q = 0; q < nt; ++q) {
-          if (sw_dst[q] == t) qa = q;
-          if (sw_dst[q] == piv[t]) qb = q;
+        const int r = piv[t];
+        if (r != t) {
+          const int qb = __ffsll((long long)__ballot(dst == r)) - 1;
+          const int sa = __builtin_amdgcn_readlane(src, t), sb = __builtin_amdgcn_readlane(src, qb);
+          if (lane == t) src = sb;
+          if (lane == qb) src = sa;
         }
-        const int v = sw_src[qa];
-        sw_src[qa] = sw_src[qb];
-        sw_src[qb] = v;
       }
-      *sw_n = nt;
+      if (lane < nt) {
+        sw_dst[lane] = dst;
+        sw_src[lane] = src;
+      }
+      if (lane == 0) *sw_n = nt;
     }
     __syncthreads();
     const int nsw = *sw_n;
