@@ -12,7 +12,7 @@ import math
 
 import numpy as np
 
-from . import abi, workloads as W
+from . import abi, atomdata, workloads as W
 
 DTYPES = {"f64": np.float64, "f32": np.float32, "i32": np.int32, "u32": np.uint32,
           "i64": np.int64, "u64": np.uint64}
@@ -171,7 +171,10 @@ class Problem:
         self.atoms = None
         self.n_ion = self.n_lev = 0
         if cfg["grey"] == 0:
-            if cfg.get("atom") is not None:      # one hand-built atom (hydrogen_atom's arguments)
+            if cfg.get("atom_file"):             # a Sedona atomic data file (data/*.hdf5)
+                at = atomdata.read_atomic_data(cfg["atom_file"], ej["elem_Z"], self.nu_np,
+                                               cfg.get("max_ion_stage", 9999))
+            elif cfg.get("atom") is not None:    # one hand-built atom (hydrogen_atom's arguments)
                 assert len(ej["elem_Z"]) == 1
                 at = W.hydrogen_atom(self.nu_np, **cfg["atom"])
             else:

@@ -235,24 +235,28 @@ H_PLANCK = 6.6260755e-27
 SIGMA_TOT = 0.0265400193567
 
 
-def level_lines(nu_grid, lev_off, lev_E, lev_g, lev_ion, rng, A_min=1e4, A_max=1e8, A=None):
+def level_lines(nu_grid, lev_off, lev_E, lev_g, lev_ion, rng, A_min=1e4, A_max=1e8, A=None, pairs=None):
     """Level-resolved lines between every pair of levels of one ion with
     E_u > E_l, Einstein A_ul log-uniform on [A_min, A_max] s^-1 (or A, one
-    value per pair in (atom, lower level, upper level) order), and the
+    value per pair in (atom, lower level, upper level) order) -- or the
+    lines `pairs` = (lower, upper, atom) given with their A -- and the
     quantities AtomicData.cpp:349-375 derives from them, in its operation
     order: nu = dE ev_to_ergs / h, B_ul = A c^2 / (2 h nu^3),
     B_lu = B_ul g_u / g_l, lam = c / nu * cm_to_angs,
     f_lu = (lam angs_to_cm)^2 A g_u / g_l / (8 pi sigma_tot) with pi = 3.14159,
     bin = locate_within_bounds(nu) (lines off the grid go to its first or
     last bin).  Sorted by (atom, bin) keeping each atom's line order."""
-    ll, lu, at = [], [], []
     n_atoms = len(lev_off) - 1
-    for a in range(n_atoms):
-        for l in range(lev_off[a], lev_off[a + 1]):
-            for u in range(l + 1, lev_off[a + 1]):
-                if lev_ion[u] == lev_ion[l] and lev_E[u] > lev_E[l]:
-                    ll.append(l); lu.append(u); at.append(a)
-    ll, lu, at = np.array(ll, np.int32), np.array(lu, np.int32), np.array(at, np.int32)
+    if pairs is not None:
+        ll, lu, at = (np.asarray(x, np.int32) for x in pairs)
+    else:
+        ll, lu, at = [], [], []
+        for a in range(n_atoms):
+            for l in range(lev_off[a], lev_off[a + 1]):
+                for u in range(l + 1, lev_off[a + 1]):
+                    if lev_ion[u] == lev_ion[l] and lev_E[u] > lev_E[l]:
+                        ll.append(l); lu.append(u); at.append(a)
+        ll, lu, at = np.array(ll, np.int32), np.array(lu, np.int32), np.array(at, np.int32)
     if A is None:
         A = 10.0 ** rng.uniform(math.log10(A_min), math.log10(A_max), len(ll))
     else:

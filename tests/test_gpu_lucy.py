@@ -15,9 +15,9 @@ models/make_mod.py, the run of param.lua, the criteria of run_test.py):
   luminosity against comparefiles/lucy_gr.dat, over 3 d < t < 55 d: max
   relative error 25 %, mean 10 % (run_test.py:41-47);
 * radiation temperature T_rad = (e_rad / a)^(1/4) after steps 15, 30 and
-  60 against comparefiles/plt_files/plt_000{15,30,60}.dat over
-  1e8 < v < 9.8e8 cm/s: max 50 %, mean 2 % (run_test.py:61-80; the .dat
-  twins of the .h5 files the script reads).
+  60 against comparefiles/plt_files/plt_000{15,30,60}.h5 (velr, T_rad,
+  read with pubsed_amd.hdf5) over 1e8 < v < 9.8e8 cm/s: max 50 %, mean 2 %
+  (run_test.py:61-80).
 
 The fixture files are copied unchanged under tests/golden/lucy/.  Run for
 the mixed form and the batched FP64 form; 3e4 radioactive packets per step
@@ -29,6 +29,7 @@ import numpy as np
 import pytest
 
 from pubsed_amd import abi, runtime, workloads as W
+from pubsed_amd.hdf5 import H5File
 
 pytestmark = pytest.mark.gpu
 DAY = 86400.0
@@ -104,7 +105,7 @@ def test_lucy_supernova_light_curve(gpu, precision):
     print("deposition: max %.3f mean %.3f" % (mx, mn))
     assert mx < 0.25 and mn < 0.10, (mx, mn)
     for s, Tr in trad.items():
-        ref = np.loadtxt(os.path.join(GOLD, "plt_%05d.dat" % s))
-        mx, mn = rel_errors(v, Tr, ref[:, 2], ref[:, 4], 1e8, 9.8e8)
+        ref = H5File(os.path.join(GOLD, "plt_%05d.h5" % s))     # run_test.py:64-79 reads the .h5 twins
+        mx, mn = rel_errors(v, Tr, ref["velr"].astype(np.float64), ref["T_rad"].astype(np.float64), 1e8, 9.8e8)
         print("T_rad after step %d: max %.3f mean %.3f" % (s, mx, mn))
         assert mx < 0.5 and mn < 0.02, (s, mx, mn)
