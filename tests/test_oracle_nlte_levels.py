@@ -64,12 +64,13 @@ def dilution(r):
     return 0.5 * (1 - np.sqrt(1 - (R_P / r) ** 2))
 
 
-def problem(space, spec, r):
+def problem(space, spec, r, atom_file=None):
     """A pure-hydrogen zone per radius, at rest, T = 5e4 K, rho = 1e-20, with
-    the spec's atom, grid and line width; returns (P, J_nu [nz][n_nu])."""
+    the spec's atom (or the atom of a Sedona atomic data file), grid and line
+    width; returns (P, J_nu [nz][n_nu])."""
     nz = len(r)
     P = runtime.Problem(space, "c2_line", nz=nz, elements=((1, 1),), nu=spec["nu"],
-                        nu_log_del=spec["nu_log_del"], fuzz=0, atom=spec["atom"])
+                        nu_log_del=spec["nu_log_del"], fuzz=0, atom=spec["atom"], atom_file=atom_file)
     P.set("rho", np.full(nz, 1e-20))
     P.set("T", np.full(nz, T_S))
     P.op_params.omit_composition_decay = 1
@@ -176,3 +177,24 @@ def test_two_level_closed_forms():
     for z in range(len(r)):
         assert abs(r12[z] / closed_form_two(P, J, z) - 1) <= 1e-12, z
     assert np.abs(r12 / two_level_formula(r) - 1).max() <= 5e-4
+
+
+def test_three_level_atom_file_is_the_hand_built_atom(three):
+    """tests/3level_nlte/param.lua:3 reads data/H_3lev_plusC_atomdata.hdf5
+    (copied under tests/golden/atomdata/): read through pubsed_amd.atomdata it
+    is the hand-built atom above -- the same levels, lines and derived line
+    data, the bare stage's chi 999999 against 99999 (no stage lies above it,
+    so nothing reads it) -- and the NLTE build from it returns the same
+    populations and n_e."""
+    import os
+    P0, J0, tab = three
+    path = os.path.join(os.path.dirname(__file__), "golden", "atomdata", "H_3lev_plusC_atomdata.hdf5")
+    P, J = problem(runtime.Space(None), THREE, tab[:, 0], atom_file=path)
+    a, b = P.atoms_np, P0.atoms_np
+    for k in ("lev_off", "ion_off", "lev_E", "lev_g", "lev_ion", "bline_l", "bline_u", "bline_A", "bline_nu",
+              "bline_Bul", "bline_Blu", "bline_f", "bline_bin"):
+        assert np.array_equal(a[k], b[k]), k
+    assert list(a["ion_chi"]) == [13.6, 999999.0] and list(b["ion_chi"]) == [13.6, 99999.0]
+    assert OracleBackend().opacity_build_nlte(P, J, np.ones(1, np.int32), use_collisions=0) == 0
+    for k in ("lev_n", "n_elec", "abs", "emis_cdf"):
+        assert np.array_equal(P.get(k), P0.get(k)), k
