@@ -67,6 +67,26 @@ void timer_end(sgpu_ctx* ctx, const char* phase)
   t.pending = true;
 }
 
+int fork_side(sgpu_ctx* ctx, hipStream_t* side)
+{
+  if (!ctx->side_stream) {
+    SGPU_HIP(hipStreamCreateWithFlags(&ctx->side_stream, hipStreamNonBlocking));
+    SGPU_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+    SGPU_HIP(hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming));
+  }
+  SGPU_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
+  SGPU_HIP(hipStreamWaitEvent(ctx->side_stream, ctx->fork_ev, 0));
+  *side = ctx->side_stream;
+  return SGPU_OK;
+}
+
+int join_side(sgpu_ctx* ctx)
+{
+  SGPU_HIP(hipEventRecord(ctx->join_ev, ctx->side_stream));
+  SGPU_HIP(hipStreamWaitEvent(ctx->stream, ctx->join_ev, 0));
+  return SGPU_OK;
+}
+
 }  // namespace sgpu
 
 extern "C" {
@@ -90,12 +110,16 @@ void sgpu_destroy(sgpu_ctx* ctx)
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->side_stream) (void)hipStreamSynchronize(ctx->side_stream);
   for (auto& kv : ctx->scratch) if (kv.second.ptr) (void)hipFree(kv.second.ptr);
   for (auto& kv : ctx->timers) {
     if (kv.second.start) (void)hipEventDestroy(kv.second.start);
     if (kv.second.stop) (void)hipEventDestroy(kv.second.stop);
   }
   if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+  if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+  if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
+  if (ctx->side_stream) (void)hipStreamDestroy(ctx->side_stream);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
 }

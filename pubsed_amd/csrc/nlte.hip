@@ -204,12 +204,19 @@ constexpr int BLU_RU = 8;                // rows per thread per trailing-update 
 
 constexpr int BLU_INTS = 5 * BLU_NB + 1;  // panel pivots, composed interchanges
 
-// dynamic LDS of lu_solve_f64_blocked: the panel and the ints (the back
-// substitution reuses the panel's space for y and one diagonal block)
-__host__ __device__ inline size_t blocked_lds_bytes(int n)
+constexpr int BLU_MAXW = NLTE_MAX_THREADS / WAVE;   // wavefronts of the largest workgroup
+
+// dynamic LDS of lu_solve_f64_blocked: the panel (the back substitution
+// reuses its space for y and one diagonal block), the pivot search's two
+// buffers of per-wavefront maxima, the ints
+__host__ __device__ inline size_t blocked_doubles(int n)
 {
   const size_t panel = (size_t)n * BLU_PS, back = (size_t)n + BLU_NB * BLU_NB;
-  return sizeof(double) * (panel > back ? panel : back) + sizeof(int) * BLU_INTS;
+  return panel > back ? panel : back;
+}
+__host__ __device__ inline size_t blocked_lds_bytes(int n)
+{
+  return sizeof(double) * blocked_doubles(n) + sizeof(ArgMax) * 2 * BLU_MAXW + sizeof(int) * BLU_INTS;
 }
 
 // [M | e_{n-1}] with row stride n + 1 for lu_solve_f64_blocked
@@ -219,15 +226,22 @@ __device__ __forceinline__ void assemble_augmented(int n, const double* R, const
   assemble<double>(n, R, lte, M, n + 1);
 }
 
-// lds: blocked_lds_bytes(n) bytes of dynamic LDS
-__device__ __forceinline__ void lu_solve_f64_blocked(int n, double* __restrict__ M, double* __restrict__ x,
-                                     double* __restrict__ lds, ArgMax* red, int* singular)
+// One panel step of the blocked LU of the augmented matrix M (n x (n + 1),
+// row stride n + 1) at columns j0 .. j0 + jb - 1: the panel loaded into LDS
+// and factored with partial pivoting, written back (U11 and L21), its
+// interchanges applied to the columns on its right (the rhs included), and
+// U12 = L11^-1 A12.  The panel stays in LDS for the trailing update.
+// lds: blocked_lds_bytes(n) bytes of dynamic LDS.
+__device__ __forceinline__ void blocked_panel(int n, double* __restrict__ M, int j0, double* __restrict__ lds,
+                                              ArgMax* red, int* singular)
 {
   const int tid = threadIdx.x, NT = blockDim.x;
   const int ld = n + 1;
   double* P = lds;                                  // panel rows, stride BLU_PS
+  ArgMax* wmax = (ArgMax*)(lds + blocked_doubles(n));   // [2][BLU_MAXW]
   int* piv = (int*)((char*)lds + blocked_lds_bytes(n) - sizeof(int) * BLU_INTS);   // BLU_INTS ints
-  for (int j0 = 0; j0 < n; j0 += BLU_NB) {
+  (void)red;
+  {
     const int jb = min(BLU_NB, n - j0);
     const int m = n - j0;
     // the last, narrower panel is padded with zero columns (the trailing
@@ -246,10 +260,27 @@ __device__ __forceinline__ void lu_solve_f64_blocked(int n, double* __restrict__
       }
     }
     __syncthreads();
+    // Two workgroup barriers per column: the elimination of column t also
+    // finds each thread's largest |entry| of column t + 1 among the rows it
+    // just updated; wavefront maxima go to one of two LDS buffers (column
+    // parity), so a buffer is rewritten only two barriers after its last
+    // read.  Ties go to the smaller row, as a sequential search.
+    const int nw = NT / WAVE, w = tid / WAVE, lane = tid % WAVE;
+    ArgMax am = {-1.0, m};
+    for (int r = tid; r < m; r += NT) am = better(am, ArgMax{fabs(P[r * BLU_PS]), r});
     for (int t = 0; t < jb; ++t) {
-      ArgMax am = {-1.0, m};
-      for (int r = t + tid; r < m; r += NT) am = better(am, ArgMax{fabs(P[r * BLU_PS + t]), r});
-      const ArgMax p = block_argmax(am, red);
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        ArgMax q;
+        q.v = __shfl_xor(am.v, o, WAVE);
+        q.i = __shfl_xor(am.i, o, WAVE);
+        am = better(am, q);
+      }
+      ArgMax* buf = wmax + (t & 1) * BLU_MAXW;
+      if (lane == 0) buf[w] = am;
+      __syncthreads();
+      ArgMax p = buf[0];
+      for (int k = 1; k < nw; ++k) p = better(p, buf[k]);
       const int pr = p.i < m ? p.i : t;
       if (tid == 0) piv[t] = pr;
       if (pr != t)
@@ -260,17 +291,22 @@ __device__ __forceinline__ void lu_solve_f64_blocked(int n, double* __restrict__
         }
       __syncthreads();
       const double ajj = P[t * BLU_PS + t];
+      const int tn = t + 1;
+      am = ArgMax{-1.0, m};
       if (ajj == 0) {
         if (tid == 0) *singular = 1;
+        if (tn < jb)
+          for (int r = tn + tid; r < m; r += NT) am = better(am, ArgMax{fabs(P[r * BLU_PS + tn]), r});
         continue;
       }
-      for (int r = t + 1 + tid; r < m; r += NT) {
+      for (int r = tn + tid; r < m; r += NT) {
         const double l = P[r * BLU_PS + t] / ajj;
         P[r * BLU_PS + t] = l;
-        for (int c = t + 1; c < jb; ++c) P[r * BLU_PS + c] -= l * P[t * BLU_PS + c];
+        for (int c = tn; c < jb; ++c) P[r * BLU_PS + c] -= l * P[t * BLU_PS + c];
+        if (tn < jb) am = better(am, ArgMax{fabs(P[r * BLU_PS + tn]), r});
       }
-      __syncthreads();
     }
+    __syncthreads();
     // the factored panel back to device memory (U11 and L21; only U11 is
     // read again, by the back substitution)
     for (int idx = tid; idx < m * jb; idx += NT) {
@@ -344,10 +380,22 @@ __device__ __forceinline__ void lu_solve_f64_blocked(int n, double* __restrict__
         if (t < jb) M[(size_t)(j0 + t) * ld + k] = u[t];
     }
     __syncthreads();
-    // A22 -= L21 U12: lanes over BLU_CW consecutive columns (coalesced
-    // rows), wavefronts over rows; each thread keeps its column of U12 in
-    // registers and updates BLU_RU rows per pass with their loads issued
-    // together; the L21 row is an LDS broadcast
+  }
+}
+
+// The trailing update of the panel step at j0 by one workgroup, L21 from the
+// panel in LDS: A22 -= L21 U12.  Lanes over BLU_CW consecutive columns
+// (coalesced rows), wavefronts over rows; each thread keeps its column of
+// U12 in registers and updates BLU_RU rows per pass with their loads issued
+// together; the L21 row is an LDS broadcast.
+__device__ __forceinline__ void blocked_trailing(int n, double* __restrict__ M, int j0,
+                                                 const double* __restrict__ lds)
+{
+  const int tid = threadIdx.x, NT = blockDim.x;
+  const int ld = n + 1;
+  const double* P = lds;
+  {
+    const int jb = min(BLU_NB, n - j0);
     const int rows_per = NT / BLU_CW;
     const int col = tid % BLU_CW, rg = tid / BLU_CW;
     const int i0 = j0 + jb;
@@ -381,7 +429,14 @@ __device__ __forceinline__ void lu_solve_f64_blocked(int n, double* __restrict__
     }
     __syncthreads();
   }
-  // back substitution U x = y, y in column n; the running right-hand side in LDS
+}
+
+// Back substitution U x = y of the factored augmented matrix (y in column
+// n), the running right-hand side in LDS: x is left in lds[0 .. n).
+__device__ __forceinline__ void blocked_back_subst(int n, const double* __restrict__ M, double* __restrict__ lds)
+{
+  const int tid = threadIdx.x, NT = blockDim.x;
+  const int ld = n + 1;
   double* Y = lds;                                  // n doubles
   double* D = lds + n;                              // BLU_NB x BLU_NB diagonal block
   for (int i = tid; i < n; i += NT) Y[i] = M[(size_t)i * ld + n];
@@ -410,7 +465,20 @@ __device__ __forceinline__ void lu_solve_f64_blocked(int n, double* __restrict__
     }
     __syncthreads();
   }
-  for (int i = tid; i < n; i += NT) x[i] = Y[i];
+}
+
+// Blocked FP64 LU of [M | e_{n-1}] (row stride n + 1, in device memory) by
+// one workgroup, panels in LDS, then the back substitution into x.
+// lds: blocked_lds_bytes(n) bytes of dynamic LDS.
+__device__ __forceinline__ void lu_solve_f64_blocked(int n, double* __restrict__ M, double* __restrict__ x,
+                                                     double* __restrict__ lds, ArgMax* red, int* singular)
+{
+  for (int j0 = 0; j0 < n; j0 += BLU_NB) {
+    blocked_panel(n, M, j0, lds, red, singular);
+    blocked_trailing(n, M, j0, lds);
+  }
+  blocked_back_subst(n, M, lds);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = lds[i];
   __syncthreads();
 }
 
@@ -445,27 +513,32 @@ __global__ void __launch_bounds__(NLTE_THREADS) nlte_single_kernel(int n, const 
 // ------------------------------------------------------------------
 // batched: device queue of systems, largest first
 // ------------------------------------------------------------------
-// smallest and largest system size (single workgroup): mm[0] = min, mm[1] = max
-__global__ void __launch_bounds__(1024) size_range_kernel(const int32_t* n, int nsys, int32_t* mm)
+// smallest and largest system size and the number of systems above cap
+// (single workgroup): mm[0] = min, mm[1] = max, mm[2] = count(n > cap)
+__global__ void __launch_bounds__(1024) size_range_kernel(const int32_t* n, int nsys, int cap, int32_t* mm)
 {
-  __shared__ int lo[1024 / WAVE], hi[1024 / WAVE];
-  int a = 1 << 30, b = -(1 << 30);
-  for (int s = threadIdx.x; s < nsys; s += blockDim.x) { a = min(a, n[s]); b = max(b, n[s]); }
-  for (int o = 32; o > 0; o >>= 1) { a = min(a, __shfl_xor(a, o, WAVE)); b = max(b, __shfl_xor(b, o, WAVE)); }
-  if (threadIdx.x % WAVE == 0) { lo[threadIdx.x / WAVE] = a; hi[threadIdx.x / WAVE] = b; }
+  __shared__ int lo[1024 / WAVE], hi[1024 / WAVE], cnt[1024 / WAVE];
+  int a = 1 << 30, b = -(1 << 30), c = 0;
+  for (int s = threadIdx.x; s < nsys; s += blockDim.x) { a = min(a, n[s]); b = max(b, n[s]); c += n[s] > cap; }
+  for (int o = 32; o > 0; o >>= 1) {
+    a = min(a, __shfl_xor(a, o, WAVE)); b = max(b, __shfl_xor(b, o, WAVE)); c += __shfl_xor(c, o, WAVE);
+  }
+  if (threadIdx.x % WAVE == 0) { lo[threadIdx.x / WAVE] = a; hi[threadIdx.x / WAVE] = b; cnt[threadIdx.x / WAVE] = c; }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < (int)(blockDim.x / WAVE); w++) { a = min(a, lo[w]); b = max(b, hi[w]); }
+    for (int w = 1; w < (int)(blockDim.x / WAVE); w++) { a = min(a, lo[w]); b = max(b, hi[w]); c += cnt[w]; }
     a = min(a, lo[0]); b = max(b, hi[0]);
-    mm[0] = a; mm[1] = b;
+    mm[0] = a; mm[1] = b; mm[2] = c;
   }
 }
 
 // bucket sort of system indices by size, descending (single workgroup):
 // histogram, prefix, then every thread places its systems through an
 // atomic cursor (the order within one size is arbitrary; each system's
-// result does not depend on it)
-__global__ void __launch_bounds__(1024) order_kernel(const int32_t* n, int nsys, int32_t* order, int32_t* counter)
+// result does not depend on it).  The queue cursor starts at the first
+// system of at most cap levels (the larger ones take the wide path).
+__global__ void __launch_bounds__(1024) order_kernel(const int32_t* n, int nsys, int cap, int32_t* order,
+                                                     int32_t* counter)
 {
   __shared__ int hist[1025];
   for (int k = threadIdx.x; k < 1025; k += blockDim.x) hist[k] = 0;
@@ -479,7 +552,7 @@ __global__ void __launch_bounds__(1024) order_kernel(const int32_t* n, int nsys,
   if (threadIdx.x == 0) {
     int run = 0;
     for (int k = 0; k < 1025; k++) { int c = hist[k]; hist[k] = run; run += c; }
-    *counter = 0;
+    *counter = hist[1024 - cap];
   }
   __syncthreads();
   for (int s = threadIdx.x; s < nsys; s += blockDim.x) {
@@ -536,6 +609,182 @@ __global__ void __launch_bounds__(NLTE_MAX_THREADS) nlte_batched_f64_kernel(Batc
     if (A.iters && threadIdx.x == 0) A.iters[s] = 0;
     __syncthreads();
   }
+}
+
+// ------------------------------------------------------------------
+// Wide systems: the blocked FP64 LU of every system too large for the
+// in-LDS paths, spread over the whole GPU.  Each panel step is two
+// launches: the panel (one workgroup per system, as blocked_panel) and the
+// trailing update A22 -= L21 U12 in 64 x 64 tiles of every system (a
+// workgroup per tile), so the update of a large system runs on many CUs
+// instead of one.  Wide system w is order[w] (the largest first), its
+// augmented matrix at M + w stride.
+// ------------------------------------------------------------------
+constexpr int WIDE_THREADS = 512;   // panel, assembly and back-substitution workgroups
+constexpr int WIDE_TILE = 64;       // trailing-update tile (rows and columns)
+constexpr int WIDE_UPD_THREADS = 256;
+
+struct WideArgs {
+  const int32_t* n;
+  const int32_t* order;
+  const int64_t* rate_off;
+  const int64_t* vec_off;
+  const double* rates;
+  const double* lte;
+  double* b;
+  double* pop;
+  int32_t* iters;
+  double* M;
+  size_t stride;       // doubles per system
+  int iters_value;     // reported iterations (0 FP64 mode, -1 FP64 fallback of the FP32 mode)
+};
+
+// [M | e_{n-1}] of the wide systems as assemble_augmented builds it, grid
+// (wide systems, 64-row blocks of R): the workgroup reads R's rows
+// i0 .. i0 + 63 in 64 x 64 tiles through LDS (coalesced), sums each row in
+// j order (M_ii = -sum_j R_ij, as assemble), and writes the transposed
+// tile (M_ji = R_ij) along M's rows; the last row is lte, column n the rhs.
+constexpr int WIDE_ASM_THREADS = 256;
+__global__ void __launch_bounds__(WIDE_ASM_THREADS) nlte_wide_assemble_kernel(WideArgs W)
+{
+  __shared__ double T[WIDE_TILE][WIDE_TILE + 1];
+  const int s = W.order[blockIdx.x];
+  const int n = W.n[s];
+  const int i0 = blockIdx.y * WIDE_TILE;
+  if (i0 >= n) return;
+  const double* R = W.rates + W.rate_off[s];
+  const double* lte = W.lte + W.vec_off[s];
+  double* M = W.M + (size_t)blockIdx.x * W.stride;
+  const int ld = n + 1, tid = threadIdx.x;
+  constexpr int PER = WIDE_TILE * WIDE_TILE / WIDE_ASM_THREADS;
+  double Rout = 0.0;   // row i0 + tid (threads below WIDE_TILE)
+  for (int j0 = 0; j0 < n; j0 += WIDE_TILE) {
+    double v[PER];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int idx = tid + e * WIDE_ASM_THREADS, a = idx / WIDE_TILE, b = idx % WIDE_TILE;
+      v[e] = (i0 + a < n && j0 + b < n) ? R[(size_t)(i0 + a) * n + j0 + b] : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int idx = tid + e * WIDE_ASM_THREADS;
+      T[idx / WIDE_TILE][idx % WIDE_TILE] = v[e];
+    }
+    __syncthreads();
+    if (tid < WIDE_TILE) {
+      const int jn = min(WIDE_TILE, n - j0);
+      for (int b = 0; b < jn; ++b) Rout += T[tid][b];
+    }
+    // M[j][i] = R[i][j] off the diagonal, rows j < n - 1
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int idx = tid + e * WIDE_ASM_THREADS, b = idx / WIDE_TILE, a = idx % WIDE_TILE;
+      const int i = i0 + a, j = j0 + b;
+      if (i < n && j < n - 1 && i != j) M[(size_t)j * ld + i] = T[a][b];
+    }
+    __syncthreads();
+  }
+  if (tid < WIDE_TILE) {
+    const int i = i0 + tid;
+    if (i < n) {
+      if (i != n - 1) M[(size_t)i * ld + i] = -1 * Rout;
+      M[(size_t)(n - 1) * ld + i] = lte[i];
+      M[(size_t)i * ld + n] = (i == n - 1) ? 1.0 : 0.0;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(WIDE_THREADS) nlte_wide_panel_kernel(WideArgs W, int j0)
+{
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ ArgMax red[WIDE_THREADS / WAVE];
+  __shared__ int singular;
+  const int n = W.n[W.order[blockIdx.x]];
+  if (j0 >= n) return;
+  blocked_panel(n, W.M + (size_t)blockIdx.x * W.stride, j0, (double*)lds_raw, red, &singular);
+}
+
+// grid (tiles of the largest system at this step, wide systems); tiles
+// beyond a system's own trailing matrix return at once.  Each wavefront
+// updates a 16-row strip of the 64 x 64 tile as four 16 x 16 blocks on
+// v_mfma_f64_16x16x4f64 (K = BLU_NB in four steps), operands from LDS.
+__global__ void __launch_bounds__(WIDE_UPD_THREADS) nlte_wide_update_kernel(WideArgs W, int j0)
+{
+  typedef double f64x4 __attribute__((ext_vector_type(4)));
+  __shared__ double Ls[WIDE_TILE][BLU_NB + 1];
+  __shared__ double Us[BLU_NB][WIDE_TILE + 2];
+  const int n = W.n[W.order[blockIdx.y]];
+  if (j0 >= n) return;
+  const int jb = min(BLU_NB, n - j0), i0 = j0 + jb;
+  const int mr = n - i0, mc = n + 1 - i0;
+  if (mr <= 0) return;
+  const int tc = (mc + WIDE_TILE - 1) / WIDE_TILE, tr = (mr + WIDE_TILE - 1) / WIDE_TILE;
+  if ((int)blockIdx.x >= tr * tc) return;
+  const int r0 = i0 + ((int)blockIdx.x / tc) * WIDE_TILE, c0 = i0 + ((int)blockIdx.x % tc) * WIDE_TILE;
+  double* M = W.M + (size_t)blockIdx.y * W.stride;
+  const int ld = n + 1, tid = threadIdx.x;
+  // L21 rows r0.. (negated) and U12 columns c0.. of this tile, zero-padded
+  // to BLU_NB; the loads issued together, then stored
+  constexpr int PER = WIDE_TILE * BLU_NB / WIDE_UPD_THREADS;
+  double lv[PER], uv[PER];
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const int idx = tid + e * WIDE_UPD_THREADS;
+    const int r = idx / BLU_NB, t = idx % BLU_NB;
+    lv[e] = (t < jb && r0 + r < n) ? -M[(size_t)(r0 + r) * ld + j0 + t] : 0.0;
+    const int t2 = idx / WIDE_TILE, c = idx % WIDE_TILE;
+    uv[e] = (t2 < jb && c0 + c <= n) ? M[(size_t)(j0 + t2) * ld + c0 + c] : 0.0;
+  }
+  // the tile's own entries, four 16 x 16 blocks of this wavefront's strip
+  const int wave = tid / WAVE, lane = tid % WAVE, lr = lane % 16, lk = lane / 16;
+  const int rs = r0 + wave * 16;
+  f64x4 acc[WIDE_TILE / 16];
+#pragma unroll
+  for (int cb = 0; cb < WIDE_TILE / 16; ++cb) {
+    const int col = c0 + cb * 16 + lr;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = rs + lk + 4 * v;
+      acc[cb][v] = (row < n && col <= n) ? M[(size_t)row * ld + col] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const int idx = tid + e * WIDE_UPD_THREADS;
+    Ls[idx / BLU_NB][idx % BLU_NB] = lv[e];
+    Us[idx / WIDE_TILE][idx % WIDE_TILE] = uv[e];
+  }
+  __syncthreads();
+  double av[BLU_NB / 4];
+#pragma unroll
+  for (int k4 = 0; k4 < BLU_NB / 4; ++k4) av[k4] = Ls[wave * 16 + lr][4 * k4 + lk];
+#pragma unroll
+  for (int cb = 0; cb < WIDE_TILE / 16; ++cb) {
+#pragma unroll
+    for (int k4 = 0; k4 < BLU_NB / 4; ++k4)
+      if (4 * k4 < jb)
+        acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[k4], Us[4 * k4 + lk][cb * 16 + lr], acc[cb], 0, 0, 0);
+  }
+#pragma unroll
+  for (int cb = 0; cb < WIDE_TILE / 16; ++cb) {
+    const int col = c0 + cb * 16 + lr;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = rs + lk + 4 * v;
+      if (row < n && col <= n) M[(size_t)row * ld + col] = acc[cb][v];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(WIDE_THREADS) nlte_wide_back_kernel(WideArgs W)
+{
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double* lds = (double*)lds_raw;
+  const int s = W.order[blockIdx.x];
+  const int n = W.n[s];
+  blocked_back_subst(n, W.M + (size_t)blockIdx.x * W.stride, lds);
+  write_pops(n, lds, W.lte + W.vec_off[s], W.b + W.vec_off[s], W.pop ? W.pop + W.vec_off[s] : nullptr);
+  if (W.iters && threadIdx.x == 0) W.iters[s] = W.iters_value;
 }
 
 // ------------------------------------------------------------------
@@ -1481,33 +1730,83 @@ int batched_impl(sgpu_ctx* ctx, int32_t nsys, const int32_t* n, const int64_t* r
   if (st) return st;
   int32_t* counter = order + nsys;
   int32_t* mm = order + nsys + 8;
-  hipLaunchKernelGGL(size_range_kernel, dim3(1), dim3(1024), 0, ctx->stream, n, nsys, mm);
+  // systems above cap take the wide path: above the FP64 LDS capacity in
+  // the FP64 mode, above the FP32 one in the FP32 + refinement mode
+  const int cap = precision == SGPU_NLTE_FP64 ? LdsCap<double>::n : LdsCap<float>::n;
+  hipLaunchKernelGGL(size_range_kernel, dim3(1), dim3(1024), 0, ctx->stream, n, nsys, cap, mm);
   SGPU_LAUNCH_CHECK();
-  int32_t hmm[2] = {0, 0};
+  int32_t hmm[3] = {0, 0, 0};
   SGPU_HIP(hipMemcpyAsync(hmm, mm, sizeof(hmm), hipMemcpyDeviceToHost, ctx->stream));
   SGPU_HIP(hipStreamSynchronize(ctx->stream));
   SGPU_REQUIRE(hmm[0] >= 1 && hmm[1] <= 1024, "system sizes must lie in [1, 1024] (got %d..%d)", hmm[0], hmm[1]);
-  const int max_n = hmm[1];
+  const int max_n = hmm[1], n_wide = hmm[2];
   SGPU_REQUIRE(threads >= 2 * WAVE && threads <= NLTE_MAX_THREADS && threads % WAVE == 0,
                "workgroup size %d must be a multiple of %d in [%d, %d]", threads, WAVE, 2 * WAVE, NLTE_MAX_THREADS);
   SGPU_REQUIRE((panel == 8 || panel == 16) && wg_per_cu >= 1,
                "panel width %d (8 or 16) / workgroups per CU %d", panel, wg_per_cu);
-  const unsigned nwg = (unsigned)std::min<int64_t>((int64_t)ctx->n_cu * wg_per_cu, nsys);
-  // slot: 7 vectors + perm + an FP64 matrix (augmented by its rhs) for the
-  // fallback / FP64 mode
-  const size_t slot = (size_t)max_n * (max_n + 1) + 10 * (size_t)max_n + 64;
-  double* slots = (double*)scratch(ctx, "nlte_slots", sizeof(double) * slot * nwg, &st);
-  if (st) return st;
-  hipLaunchKernelGGL(order_kernel, dim3(1), dim3(1024), 0, ctx->stream, n, nsys, order, counter);
+  SGPU_REQUIRE(n_wide >= 0 && n_wide <= nsys, "wide system count %d outside [0, %d]", n_wide, nsys);
+  hipLaunchKernelGGL(order_kernel, dim3(1), dim3(1024), 0, ctx->stream, n, nsys, cap, order, counter);
   SGPU_LAUNCH_CHECK();
-  BatchArgs A{nsys, n, rate_off, vec_off, rates, lte, b, pop, iters, order, counter, slots, slot, precision, tol};
   timer_begin(ctx, "nlte");
-  size_t lds = 0;
-  const void* fn = batched_kernel(precision, panel, max_n, &lds);
-  if (lds) SGPU_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  void* args[] = {&A, &lds};
-  SGPU_HIP(hipLaunchKernel(fn, dim3(nwg), dim3(threads), args, lds, ctx->stream));
-  SGPU_LAUNCH_CHECK();
+  if (n_wide < nsys) {
+    // the systems of at most cap levels from the device queue (its cursor
+    // starts after the wide systems); beside the wide path, on the side
+    // stream, when there is one
+    hipStream_t qs = ctx->stream;
+    if (n_wide > 0) {
+      st = fork_side(ctx, &qs);
+      if (st) return st;
+    }
+    const int narrow_max = std::min(max_n, cap);
+    const unsigned nwg = (unsigned)std::min<int64_t>((int64_t)ctx->n_cu * wg_per_cu, nsys - n_wide);
+    // slot: 7 vectors + perm + an FP64 matrix (augmented by its rhs) for the
+    // FP64 fallback of the FP32 mode
+    const size_t slot = (size_t)narrow_max * (narrow_max + 1) + 10 * (size_t)narrow_max + 64;
+    double* slots = (double*)scratch(ctx, "nlte_slots", sizeof(double) * slot * nwg, &st);
+    if (st) return st;
+    BatchArgs A{nsys, n, rate_off, vec_off, rates, lte, b, pop, iters, order, counter, slots, slot, precision, tol};
+    size_t lds = 0;
+    const void* fn = batched_kernel(precision, panel, narrow_max, &lds);
+    if (lds) SGPU_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    void* args[] = {&A, &lds};
+    SGPU_HIP(hipLaunchKernel(fn, dim3(nwg), dim3(threads), args, lds, qs));
+    SGPU_LAUNCH_CHECK();
+  }
+  if (n_wide > 0) {
+    // the wide systems are order[0 .. n_wide): one augmented matrix each
+    const size_t stride = ((size_t)max_n * (max_n + 1) + 31) / 32 * 32;
+    double* Mw = (double*)scratch(ctx, "nlte_wide", sizeof(double) * stride * n_wide, &st);
+    if (st) return st;
+    WideArgs W{n, order, rate_off, vec_off, rates, lte, b, pop, iters, Mw, stride,
+               precision == SGPU_NLTE_FP64 ? 0 : -1};
+    const size_t plds = blocked_lds_bytes(max_n);
+    SGPU_REQUIRE(plds <= NLTE_LDS_BYTES, "wide system of %d levels exceeds LDS", max_n);
+    SGPU_HIP(hipFuncSetAttribute((const void*)nlte_wide_panel_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)plds));
+    SGPU_HIP(hipFuncSetAttribute((const void*)nlte_wide_back_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)plds));
+    hipLaunchKernelGGL(nlte_wide_assemble_kernel, dim3(n_wide, (max_n + WIDE_TILE - 1) / WIDE_TILE),
+                       dim3(WIDE_ASM_THREADS), 0, ctx->stream, W);
+    SGPU_LAUNCH_CHECK();
+    for (int j0 = 0; j0 < max_n; j0 += BLU_NB) {
+      hipLaunchKernelGGL(nlte_wide_panel_kernel, dim3(n_wide), dim3(WIDE_THREADS), plds, ctx->stream, W, j0);
+      SGPU_LAUNCH_CHECK();
+      // tiles of the largest system's trailing matrix at this step
+      const int mr = max_n - j0 - std::min(BLU_NB, max_n - j0), mc = mr + 1;
+      if (mr > 0) {
+        const unsigned tiles = (unsigned)(((mr + WIDE_TILE - 1) / WIDE_TILE) * ((mc + WIDE_TILE - 1) / WIDE_TILE));
+        hipLaunchKernelGGL(nlte_wide_update_kernel, dim3(tiles, n_wide), dim3(WIDE_UPD_THREADS), 0, ctx->stream,
+                           W, j0);
+        SGPU_LAUNCH_CHECK();
+      }
+    }
+    hipLaunchKernelGGL(nlte_wide_back_kernel, dim3(n_wide), dim3(WIDE_THREADS), plds, ctx->stream, W);
+    SGPU_LAUNCH_CHECK();
+    if (n_wide < nsys) {
+      st = join_side(ctx);
+      if (st) return st;
+    }
+  }
   timer_end(ctx, "nlte");
   return SGPU_OK;
 }
@@ -1555,7 +1854,7 @@ extern "C" int sgpu_nlte_autotune(sgpu_ctx* ctx, int32_t nsys, const int32_t* n,
     int st = 0;
     int32_t* mm = (int32_t*)scratch(ctx, "nlte_mm", 64, &st);
     if (st) return st;
-    hipLaunchKernelGGL(size_range_kernel, dim3(1), dim3(1024), 0, ctx->stream, n, nsys, mm);
+    hipLaunchKernelGGL(size_range_kernel, dim3(1), dim3(1024), 0, ctx->stream, n, nsys, 1024, mm);
     SGPU_LAUNCH_CHECK();
     int32_t hmm[2];
     SGPU_HIP(hipMemcpyAsync(hmm, mm, sizeof(hmm), hipMemcpyDeviceToHost, ctx->stream));
@@ -1578,8 +1877,10 @@ extern "C" int sgpu_nlte_autotune(sgpu_ctx* ctx, int32_t nsys, const int32_t* n,
       for (int p : panels)
         for (int w : wgs) {
           // only shapes whose workgroups can share a CU (LDS and registers)
+          // (the queue kernel takes the systems up to the wide-path cap)
           size_t lds = 0;
-          const void* fn = batched_kernel(prec, p, max_n, &lds);
+          const int cap = prec == SGPU_NLTE_FP64 ? LdsCap<double>::n : LdsCap<float>::n;
+          const void* fn = batched_kernel(prec, p, std::min(max_n, cap), &lds);
           if (lds) SGPU_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
           int resident = 0;
           SGPU_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, fn, th, lds));
