@@ -197,6 +197,67 @@ __device__ void lu_solve_f64(int n, double* M, int* perm, double* x, ArgMax* red
 // runs in blocks of BLU_NB rows from the bottom: the diagonal block from
 // LDS by one thread, the rows above updated with a dot product over the
 // block.
+// ------------------------------------------------------------------
+// wavefront pivot search: DPP within rows of 16 lanes, readlane across them
+// ------------------------------------------------------------------
+template <class V> struct PivotOf { V v; int i; };
+
+template <class V>
+__device__ __forceinline__ PivotOf<V> better_p(PivotOf<V> a, PivotOf<V> b)
+{
+  // larger magnitude wins; the smaller row index wins ties (first maximum)
+  return (b.v > a.v || (b.v == a.v && b.i < a.i)) ? b : a;
+}
+
+template <int CTRL>
+__device__ __forceinline__ int dpp(int v) { return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false); }
+
+template <int CTRL>
+__device__ __forceinline__ PivotOf<float> dpp_better(PivotOf<float> m)
+{
+  PivotOf<float> o;
+  o.v = __int_as_float(dpp<CTRL>(__float_as_int(m.v)));
+  o.i = dpp<CTRL>(m.i);
+  return better_p(m, o);
+}
+
+template <int CTRL>
+__device__ __forceinline__ PivotOf<double> dpp_better(PivotOf<double> m)
+{
+  PivotOf<double> o;
+  const long long b = __double_as_longlong(m.v);
+  const int lo = dpp<CTRL>((int)(b & 0xffffffff)), hi = dpp<CTRL>((int)(b >> 32));
+  o.v = __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+  o.i = dpp<CTRL>(m.i);
+  return better_p(m, o);
+}
+
+__device__ __forceinline__ float readlane(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+__device__ __forceinline__ double readlane(double v, int l)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// the wavefront's best (largest |value|, then smallest row), uniform result:
+// xor 1 and xor 2 inside quads, half-row and row mirrors (every lane then
+// holds its 16-lane row's best), the four rows through readlane
+template <class V>
+__device__ __forceinline__ PivotOf<V> wave_argmax(PivotOf<V> m)
+{
+  m = dpp_better<0xB1>(m);    // quad_perm [1,0,3,2]
+  m = dpp_better<0x4E>(m);    // quad_perm [2,3,0,1]
+  m = dpp_better<0x141>(m);   // row_half_mirror
+  m = dpp_better<0x140>(m);   // row_mirror
+  PivotOf<V> r = {readlane(m.v, 0), __builtin_amdgcn_readlane(m.i, 0)};
+  r = better_p(r, PivotOf<V>{readlane(m.v, 16), __builtin_amdgcn_readlane(m.i, 16)});
+  r = better_p(r, PivotOf<V>{readlane(m.v, 32), __builtin_amdgcn_readlane(m.i, 32)});
+  r = better_p(r, PivotOf<V>{readlane(m.v, 48), __builtin_amdgcn_readlane(m.i, 48)});
+  return r;
+}
+
+
 constexpr int BLU_NB = 16;               // panel width
 constexpr int BLU_PS = BLU_NB + 1;       // LDS row stride of the panel (doubles)
 constexpr int BLU_CW = 64;               // columns per trailing-update chunk
@@ -266,18 +327,12 @@ __device__ __forceinline__ void blocked_panel(int n, double* __restrict__ M, int
     // parity), so a buffer is rewritten only two barriers after its last
     // read.  Ties go to the smaller row, as a sequential search.
     const int nw = NT / WAVE, w = tid / WAVE, lane = tid % WAVE;
-    ArgMax am = {-1.0, m};
-    for (int r = tid; r < m; r += NT) am = better(am, ArgMax{fabs(P[r * BLU_PS]), r});
+    PivotOf<double> am = {-1.0, m};
+    for (int r = tid; r < m; r += NT) am = better_p(am, PivotOf<double>{fabs(P[r * BLU_PS]), r});
     for (int t = 0; t < jb; ++t) {
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        ArgMax q;
-        q.v = __shfl_xor(am.v, o, WAVE);
-        q.i = __shfl_xor(am.i, o, WAVE);
-        am = better(am, q);
-      }
+      am = wave_argmax(am);
       ArgMax* buf = wmax + (t & 1) * BLU_MAXW;
-      if (lane == 0) buf[w] = am;
+      if (lane == 0) buf[w] = ArgMax{am.v, am.i};
       __syncthreads();
       ArgMax p = buf[0];
       for (int k = 1; k < nw; ++k) p = better(p, buf[k]);
@@ -292,18 +347,18 @@ __device__ __forceinline__ void blocked_panel(int n, double* __restrict__ M, int
       __syncthreads();
       const double ajj = P[t * BLU_PS + t];
       const int tn = t + 1;
-      am = ArgMax{-1.0, m};
+      am = PivotOf<double>{-1.0, m};
       if (ajj == 0) {
         if (tid == 0) *singular = 1;
         if (tn < jb)
-          for (int r = tn + tid; r < m; r += NT) am = better(am, ArgMax{fabs(P[r * BLU_PS + tn]), r});
+          for (int r = tn + tid; r < m; r += NT) am = better_p(am, PivotOf<double>{fabs(P[r * BLU_PS + tn]), r});
         continue;
       }
       for (int r = tn + tid; r < m; r += NT) {
         const double l = P[r * BLU_PS + t] / ajj;
         P[r * BLU_PS + t] = l;
         for (int c = tn; c < jb; ++c) P[r * BLU_PS + c] -= l * P[t * BLU_PS + c];
-        if (tn < jb) am = better(am, ArgMax{fabs(P[r * BLU_PS + tn]), r});
+        if (tn < jb) am = better_p(am, PivotOf<double>{fabs(P[r * BLU_PS + tn]), r});
       }
     }
     __syncthreads();
@@ -350,6 +405,9 @@ __device__ __forceinline__ void blocked_panel(int n, double* __restrict__ M, int
       if (lane == 0) *sw_n = nt;
     }
     __syncthreads();
+    // one thread per column: the touched rows gathered, then U12 = L11^-1 A12
+    // on the panel's rows (lanes 0..jb-1 of the composition hold rows
+    // 0..jb-1, so v[0..jb) is the interchanged A12 column), all stored back
     const int nsw = *sw_n;
     for (int k = j0 + jb + tid; k <= n; k += NT) {
       double v[2 * BLU_NB];
@@ -357,27 +415,15 @@ __device__ __forceinline__ void blocked_panel(int n, double* __restrict__ M, int
       for (int q = 0; q < 2 * BLU_NB; ++q)
         if (q < nsw) v[q] = M[(size_t)(j0 + sw_src[q]) * ld + k];
 #pragma unroll
-      for (int q = 0; q < 2 * BLU_NB; ++q)
-        if (q < nsw) M[(size_t)(j0 + sw_dst[q]) * ld + k] = v[q];
-    }
-    __syncthreads();
-    // U12 = L11^-1 A12, one thread per column: the column's jb entries are
-    // loaded together, substituted against L11 from LDS, stored back
-    for (int k = j0 + jb + tid; k <= n; k += NT) {
-      double u[BLU_NB];
-#pragma unroll
-      for (int t = 0; t < BLU_NB; ++t)
-        if (t < jb) u[t] = M[(size_t)(j0 + t) * ld + k];
-#pragma unroll
       for (int t = 1; t < BLU_NB; ++t)
         if (t < jb) {
 #pragma unroll
           for (int q = 0; q < BLU_NB - 1; ++q)
-            if (q < t) u[t] -= P[t * BLU_PS + q] * u[q];
+            if (q < t) v[t] -= P[t * BLU_PS + q] * v[q];
         }
 #pragma unroll
-      for (int t = 0; t < BLU_NB; ++t)
-        if (t < jb) M[(size_t)(j0 + t) * ld + k] = u[t];
+      for (int q = 0; q < 2 * BLU_NB; ++q)
+        if (q < nsw) M[(size_t)(j0 + sw_dst[q]) * ld + k] = v[q];
     }
     __syncthreads();
   }
@@ -823,63 +869,6 @@ __device__ __forceinline__ void wave_fence()
 {
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();
-}
-
-template <class V> struct PivotOf { V v; int i; };
-
-template <class V>
-__device__ __forceinline__ PivotOf<V> better_p(PivotOf<V> a, PivotOf<V> b)
-{
-  // larger magnitude wins; the smaller row index wins ties (first maximum)
-  return (b.v > a.v || (b.v == a.v && b.i < a.i)) ? b : a;
-}
-
-template <int CTRL>
-__device__ __forceinline__ int dpp(int v) { return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false); }
-
-template <int CTRL>
-__device__ __forceinline__ PivotOf<float> dpp_better(PivotOf<float> m)
-{
-  PivotOf<float> o;
-  o.v = __int_as_float(dpp<CTRL>(__float_as_int(m.v)));
-  o.i = dpp<CTRL>(m.i);
-  return better_p(m, o);
-}
-
-template <int CTRL>
-__device__ __forceinline__ PivotOf<double> dpp_better(PivotOf<double> m)
-{
-  PivotOf<double> o;
-  const long long b = __double_as_longlong(m.v);
-  const int lo = dpp<CTRL>((int)(b & 0xffffffff)), hi = dpp<CTRL>((int)(b >> 32));
-  o.v = __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-  o.i = dpp<CTRL>(m.i);
-  return better_p(m, o);
-}
-
-__device__ __forceinline__ float readlane(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
-__device__ __forceinline__ double readlane(double v, int l)
-{
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
-// the wavefront's best (largest |value|, then smallest row), uniform result:
-// xor 1 and xor 2 inside quads, half-row and row mirrors (every lane then
-// holds its 16-lane row's best), the four rows through readlane
-template <class V>
-__device__ __forceinline__ PivotOf<V> wave_argmax(PivotOf<V> m)
-{
-  m = dpp_better<0xB1>(m);    // quad_perm [1,0,3,2]
-  m = dpp_better<0x4E>(m);    // quad_perm [2,3,0,1]
-  m = dpp_better<0x141>(m);   // row_half_mirror
-  m = dpp_better<0x140>(m);   // row_mirror
-  PivotOf<V> r = {readlane(m.v, 0), __builtin_amdgcn_readlane(m.i, 0)};
-  r = better_p(r, PivotOf<V>{readlane(m.v, 16), __builtin_amdgcn_readlane(m.i, 16)});
-  r = better_p(r, PivotOf<V>{readlane(m.v, 32), __builtin_amdgcn_readlane(m.i, 32)});
-  r = better_p(r, PivotOf<V>{readlane(m.v, 48), __builtin_amdgcn_readlane(m.i, 48)});
-  return r;
 }
 
 template <class V>
