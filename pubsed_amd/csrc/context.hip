@@ -67,23 +67,23 @@ void timer_end(sgpu_ctx* ctx, const char* phase)
   t.pending = true;
 }
 
-int fork_side(sgpu_ctx* ctx, hipStream_t* side)
+int fork_side(sgpu_ctx* ctx, int i, hipStream_t* side)
 {
-  if (!ctx->side_stream) {
-    SGPU_HIP(hipStreamCreateWithFlags(&ctx->side_stream, hipStreamNonBlocking));
-    SGPU_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
-    SGPU_HIP(hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming));
+  if (!ctx->fork_ev) SGPU_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+  if (!ctx->side_stream[i]) {
+    SGPU_HIP(hipStreamCreateWithFlags(&ctx->side_stream[i], hipStreamNonBlocking));
+    SGPU_HIP(hipEventCreateWithFlags(&ctx->join_ev[i], hipEventDisableTiming));
   }
   SGPU_HIP(hipEventRecord(ctx->fork_ev, ctx->stream));
-  SGPU_HIP(hipStreamWaitEvent(ctx->side_stream, ctx->fork_ev, 0));
-  *side = ctx->side_stream;
+  SGPU_HIP(hipStreamWaitEvent(ctx->side_stream[i], ctx->fork_ev, 0));
+  *side = ctx->side_stream[i];
   return SGPU_OK;
 }
 
-int join_side(sgpu_ctx* ctx)
+int join_side(sgpu_ctx* ctx, int i)
 {
-  SGPU_HIP(hipEventRecord(ctx->join_ev, ctx->side_stream));
-  SGPU_HIP(hipStreamWaitEvent(ctx->stream, ctx->join_ev, 0));
+  SGPU_HIP(hipEventRecord(ctx->join_ev[i], ctx->side_stream[i]));
+  SGPU_HIP(hipStreamWaitEvent(ctx->stream, ctx->join_ev[i], 0));
   return SGPU_OK;
 }
 
@@ -110,7 +110,8 @@ void sgpu_destroy(sgpu_ctx* ctx)
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  if (ctx->side_stream) (void)hipStreamSynchronize(ctx->side_stream);
+  for (hipStream_t q : ctx->side_stream)
+    if (q) (void)hipStreamSynchronize(q);
   for (auto& kv : ctx->scratch) if (kv.second.ptr) (void)hipFree(kv.second.ptr);
   for (auto& kv : ctx->timers) {
     if (kv.second.start) (void)hipEventDestroy(kv.second.start);
@@ -118,8 +119,10 @@ void sgpu_destroy(sgpu_ctx* ctx)
   }
   if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
   if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
-  if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
-  if (ctx->side_stream) (void)hipStreamDestroy(ctx->side_stream);
+  for (int i = 0; i < 2; ++i) {
+    if (ctx->join_ev[i]) (void)hipEventDestroy(ctx->join_ev[i]);
+    if (ctx->side_stream[i]) (void)hipStreamDestroy(ctx->side_stream[i]);
+  }
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
 }

@@ -683,7 +683,11 @@ struct WideArgs {
   double* M;
   size_t stride;       // doubles per system
   int iters_value;     // reported iterations (0 FP64 mode, -1 FP64 fallback of the FP32 mode)
+  int gstride, goff;   // this launch's group: wide systems goff, goff + gstride, ...
 };
+
+// wide system index of the group's g-th system
+__device__ __forceinline__ int wide_index(const WideArgs& W, int g) { return g * W.gstride + W.goff; }
 
 // [M | e_{n-1}] of the wide systems as assemble_augmented builds it, grid
 // (wide systems, 64-row blocks of R): the workgroup reads R's rows
@@ -694,13 +698,14 @@ constexpr int WIDE_ASM_THREADS = 256;
 __global__ void __launch_bounds__(WIDE_ASM_THREADS) nlte_wide_assemble_kernel(WideArgs W)
 {
   __shared__ double T[WIDE_TILE][WIDE_TILE + 1];
-  const int s = W.order[blockIdx.x];
+  const int w = wide_index(W, blockIdx.x);
+  const int s = W.order[w];
   const int n = W.n[s];
   const int i0 = blockIdx.y * WIDE_TILE;
   if (i0 >= n) return;
   const double* R = W.rates + W.rate_off[s];
   const double* lte = W.lte + W.vec_off[s];
-  double* M = W.M + (size_t)blockIdx.x * W.stride;
+  double* M = W.M + (size_t)w * W.stride;
   const int ld = n + 1, tid = threadIdx.x;
   constexpr int PER = WIDE_TILE * WIDE_TILE / WIDE_ASM_THREADS;
   double Rout = 0.0;   // row i0 + tid (threads below WIDE_TILE)
@@ -745,9 +750,10 @@ __global__ void __launch_bounds__(WIDE_THREADS) nlte_wide_panel_kernel(WideArgs 
   extern __shared__ __align__(16) unsigned char lds_raw[];
   __shared__ ArgMax red[WIDE_THREADS / WAVE];
   __shared__ int singular;
-  const int n = W.n[W.order[blockIdx.x]];
+  const int w = wide_index(W, blockIdx.x);
+  const int n = W.n[W.order[w]];
   if (j0 >= n) return;
-  blocked_panel(n, W.M + (size_t)blockIdx.x * W.stride, j0, (double*)lds_raw, red, &singular);
+  blocked_panel(n, W.M + (size_t)w * W.stride, j0, (double*)lds_raw, red, &singular);
 }
 
 // grid (tiles of the largest system at this step, wide systems); tiles
@@ -759,7 +765,8 @@ __global__ void __launch_bounds__(WIDE_UPD_THREADS) nlte_wide_update_kernel(Wide
   typedef double f64x4 __attribute__((ext_vector_type(4)));
   __shared__ double Ls[WIDE_TILE][BLU_NB + 1];
   __shared__ double Us[BLU_NB][WIDE_TILE + 2];
-  const int n = W.n[W.order[blockIdx.y]];
+  const int w = wide_index(W, blockIdx.y);
+  const int n = W.n[W.order[w]];
   if (j0 >= n) return;
   const int jb = min(BLU_NB, n - j0), i0 = j0 + jb;
   const int mr = n - i0, mc = n + 1 - i0;
@@ -767,7 +774,7 @@ __global__ void __launch_bounds__(WIDE_UPD_THREADS) nlte_wide_update_kernel(Wide
   const int tc = (mc + WIDE_TILE - 1) / WIDE_TILE, tr = (mr + WIDE_TILE - 1) / WIDE_TILE;
   if ((int)blockIdx.x >= tr * tc) return;
   const int r0 = i0 + ((int)blockIdx.x / tc) * WIDE_TILE, c0 = i0 + ((int)blockIdx.x % tc) * WIDE_TILE;
-  double* M = W.M + (size_t)blockIdx.y * W.stride;
+  double* M = W.M + (size_t)w * W.stride;
   const int ld = n + 1, tid = threadIdx.x;
   // L21 rows r0.. (negated) and U12 columns c0.. of this tile, zero-padded
   // to BLU_NB; the loads issued together, then stored
@@ -826,9 +833,10 @@ __global__ void __launch_bounds__(WIDE_THREADS) nlte_wide_back_kernel(WideArgs W
 {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double* lds = (double*)lds_raw;
-  const int s = W.order[blockIdx.x];
+  const int w = wide_index(W, blockIdx.x);
+  const int s = W.order[w];
   const int n = W.n[s];
-  blocked_back_subst(n, W.M + (size_t)blockIdx.x * W.stride, lds);
+  blocked_back_subst(n, W.M + (size_t)w * W.stride, lds);
   write_pops(n, lds, W.lte + W.vec_off[s], W.b + W.vec_off[s], W.pop ? W.pop + W.vec_off[s] : nullptr);
   if (W.iters && threadIdx.x == 0) W.iters[s] = W.iters_value;
 }
@@ -1739,11 +1747,11 @@ int batched_impl(sgpu_ctx* ctx, int32_t nsys, const int32_t* n, const int64_t* r
   timer_begin(ctx, "nlte");
   if (n_wide < nsys) {
     // the systems of at most cap levels from the device queue (its cursor
-    // starts after the wide systems); beside the wide path, on the side
-    // stream, when there is one
+    // starts after the wide systems); beside the wide path, on side stream 0,
+    // when there is one
     hipStream_t qs = ctx->stream;
     if (n_wide > 0) {
-      st = fork_side(ctx, &qs);
+      st = fork_side(ctx, 0, &qs);
       if (st) return st;
     }
     const int narrow_max = std::min(max_n, cap);
@@ -1762,37 +1770,61 @@ int batched_impl(sgpu_ctx* ctx, int32_t nsys, const int32_t* n, const int64_t* r
     SGPU_LAUNCH_CHECK();
   }
   if (n_wide > 0) {
-    // the wide systems are order[0 .. n_wide): one augmented matrix each
+    // the wide systems are order[0 .. n_wide): one augmented matrix each;
+    // two interleaved groups (even and odd positions) on two streams, so
+    // that one group's panels overlap the other's trailing updates
     const size_t stride = ((size_t)max_n * (max_n + 1) + 31) / 32 * 32;
     double* Mw = (double*)scratch(ctx, "nlte_wide", sizeof(double) * stride * n_wide, &st);
     if (st) return st;
-    WideArgs W{n, order, rate_off, vec_off, rates, lte, b, pop, iters, Mw, stride,
-               precision == SGPU_NLTE_FP64 ? 0 : -1};
     const size_t plds = blocked_lds_bytes(max_n);
     SGPU_REQUIRE(plds <= NLTE_LDS_BYTES, "wide system of %d levels exceeds LDS", max_n);
     SGPU_HIP(hipFuncSetAttribute((const void*)nlte_wide_panel_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)plds));
     SGPU_HIP(hipFuncSetAttribute((const void*)nlte_wide_back_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)plds));
-    hipLaunchKernelGGL(nlte_wide_assemble_kernel, dim3(n_wide, (max_n + WIDE_TILE - 1) / WIDE_TILE),
-                       dim3(WIDE_ASM_THREADS), 0, ctx->stream, W);
-    SGPU_LAUNCH_CHECK();
-    for (int j0 = 0; j0 < max_n; j0 += BLU_NB) {
-      hipLaunchKernelGGL(nlte_wide_panel_kernel, dim3(n_wide), dim3(WIDE_THREADS), plds, ctx->stream, W, j0);
+    const int groups = n_wide > 1 ? 2 : 1;
+    hipStream_t gs[2] = {ctx->stream, ctx->stream};
+    if (groups == 2) {
+      st = fork_side(ctx, 1, &gs[1]);
+      if (st) return st;
+    }
+    for (int g = 0; g < groups; ++g) {
+      const unsigned ng = (unsigned)((n_wide - g + groups - 1) / groups);
+      WideArgs W{n, order, rate_off, vec_off, rates, lte, b, pop, iters, Mw, stride,
+                 precision == SGPU_NLTE_FP64 ? 0 : -1, groups, g};
+      hipLaunchKernelGGL(nlte_wide_assemble_kernel, dim3(ng, (max_n + WIDE_TILE - 1) / WIDE_TILE),
+                         dim3(WIDE_ASM_THREADS), 0, gs[g], W);
       SGPU_LAUNCH_CHECK();
+    }
+    for (int j0 = 0; j0 < max_n; j0 += BLU_NB) {
       // tiles of the largest system's trailing matrix at this step
       const int mr = max_n - j0 - std::min(BLU_NB, max_n - j0), mc = mr + 1;
-      if (mr > 0) {
-        const unsigned tiles = (unsigned)(((mr + WIDE_TILE - 1) / WIDE_TILE) * ((mc + WIDE_TILE - 1) / WIDE_TILE));
-        hipLaunchKernelGGL(nlte_wide_update_kernel, dim3(tiles, n_wide), dim3(WIDE_UPD_THREADS), 0, ctx->stream,
-                           W, j0);
+      const unsigned tiles = (unsigned)(((mr + WIDE_TILE - 1) / WIDE_TILE) * ((mc + WIDE_TILE - 1) / WIDE_TILE));
+      for (int g = 0; g < groups; ++g) {
+        const unsigned ng = (unsigned)((n_wide - g + groups - 1) / groups);
+        WideArgs W{n, order, rate_off, vec_off, rates, lte, b, pop, iters, Mw, stride,
+                   precision == SGPU_NLTE_FP64 ? 0 : -1, groups, g};
+        hipLaunchKernelGGL(nlte_wide_panel_kernel, dim3(ng), dim3(WIDE_THREADS), plds, gs[g], W, j0);
         SGPU_LAUNCH_CHECK();
+        if (mr > 0) {
+          hipLaunchKernelGGL(nlte_wide_update_kernel, dim3(tiles, ng), dim3(WIDE_UPD_THREADS), 0, gs[g], W, j0);
+          SGPU_LAUNCH_CHECK();
+        }
       }
     }
-    hipLaunchKernelGGL(nlte_wide_back_kernel, dim3(n_wide), dim3(WIDE_THREADS), plds, ctx->stream, W);
-    SGPU_LAUNCH_CHECK();
+    for (int g = 0; g < groups; ++g) {
+      const unsigned ng = (unsigned)((n_wide - g + groups - 1) / groups);
+      WideArgs W{n, order, rate_off, vec_off, rates, lte, b, pop, iters, Mw, stride,
+                 precision == SGPU_NLTE_FP64 ? 0 : -1, groups, g};
+      hipLaunchKernelGGL(nlte_wide_back_kernel, dim3(ng), dim3(WIDE_THREADS), plds, gs[g], W);
+      SGPU_LAUNCH_CHECK();
+    }
+    if (groups == 2) {
+      st = join_side(ctx, 1);
+      if (st) return st;
+    }
     if (n_wide < nsys) {
-      st = join_side(ctx);
+      st = join_side(ctx, 0);
       if (st) return st;
     }
   }
