@@ -829,6 +829,94 @@ __global__ void __launch_bounds__(WIDE_UPD_THREADS) nlte_wide_update_kernel(Wide
   }
 }
 
+// Back substitution U x = y (y in column n) with each thread owning rows
+// tid and tid + WIDE_THREADS (n <= 2 WIDE_THREADS): a row's 16-column
+// segment of the next block is loaded while the current block is solved,
+// the diagonal block by the first 16 lanes of wavefront 0 (row r in lane r,
+// x broadcast by readlane), the rows above updated from x in LDS.  The
+// same operations in the same order as blocked_back_subst; x is left in
+// lds[0 .. n).
+__device__ __forceinline__ void back_subst_rows(int n, const double* __restrict__ M, double* __restrict__ lds)
+{
+  constexpr int RPT = 2;
+  const int tid = threadIdx.x, lane = tid % WAVE;
+  const int ld = n + 1;
+  double* X = lds;                     // n doubles
+  double* Dl = lds + n;                // BLU_NB rows of stride BLU_PS
+  double* Yl = Dl + BLU_NB * BLU_PS;   // BLU_NB
+  double y[RPT], seg[RPT][BLU_NB], nxt[RPT][BLU_NB];
+  const int nblk = (n + BLU_NB - 1) / BLU_NB;
+  {
+    const int b0 = (nblk - 1) * BLU_NB, bs = n - b0;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int i = tid + q * WIDE_THREADS;
+      y[q] = i < n ? M[(size_t)i * ld + n] : 0.0;
+#pragma unroll
+      for (int c = 0; c < BLU_NB; ++c) seg[q][c] = (i < n && c < bs) ? M[(size_t)i * ld + b0 + c] : 0.0;
+    }
+  }
+  for (int bk = nblk - 1; bk >= 0; --bk) {
+    const int b0 = bk * BLU_NB, bs = min(BLU_NB, n - b0);
+    // the next block's segments of the rows above this block
+    if (bk > 0) {
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        const int i = tid + q * WIDE_THREADS;
+#pragma unroll
+        for (int c = 0; c < BLU_NB; ++c) nxt[q][c] = i < b0 ? M[(size_t)i * ld + b0 - BLU_NB + c] : 0.0;
+      }
+    }
+    // this block's rows hand their diagonal-block rows and y to LDS
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int i = tid + q * WIDE_THREADS;
+      if (i >= b0 && i < b0 + bs) {
+#pragma unroll
+        for (int c = 0; c < BLU_NB; ++c) Dl[(i - b0) * BLU_PS + c] = seg[q][c];
+        Yl[i - b0] = y[q];
+      }
+    }
+    __syncthreads();
+    if (tid < WAVE) {
+      double Dr[BLU_NB];
+      const int rr = lane < BLU_NB ? lane : 0;
+#pragma unroll
+      for (int c = 0; c < BLU_NB; ++c) Dr[c] = Dl[rr * BLU_PS + c];
+      const double yr = Yl[rr];
+      double xr[BLU_NB];
+#pragma unroll
+      for (int r = BLU_NB - 1; r >= 0; --r) {
+        xr[r] = 0.0;
+        if (r < bs) {
+          double v = yr;
+#pragma unroll
+          for (int c = r + 1; c < BLU_NB; ++c)
+            if (c < bs) v -= Dr[c] * xr[c];
+          xr[r] = readlane(v / Dr[r], r);
+          if (lane == r) X[b0 + r] = xr[r];
+        }
+      }
+    }
+    __syncthreads();
+    // the rows above: y_i -= U[i][b0 .. b0 + bs) x[b0 .. b0 + bs)
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int i = tid + q * WIDE_THREADS;
+      if (i < b0) {
+        double v = y[q];
+#pragma unroll
+        for (int c = 0; c < BLU_NB; ++c)
+          if (c < bs) v -= seg[q][c] * X[b0 + c];
+        y[q] = v;
+      }
+#pragma unroll
+      for (int c = 0; c < BLU_NB; ++c) seg[q][c] = nxt[q][c];
+    }
+  }
+  __syncthreads();
+}
+
 __global__ void __launch_bounds__(WIDE_THREADS) nlte_wide_back_kernel(WideArgs W)
 {
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -836,7 +924,7 @@ __global__ void __launch_bounds__(WIDE_THREADS) nlte_wide_back_kernel(WideArgs W
   const int w = wide_index(W, blockIdx.x);
   const int s = W.order[w];
   const int n = W.n[s];
-  blocked_back_subst(n, W.M + (size_t)w * W.stride, lds);
+  back_subst_rows(n, W.M + (size_t)w * W.stride, lds);
   write_pops(n, lds, W.lte + W.vec_off[s], W.b + W.vec_off[s], W.pop ? W.pop + W.vec_off[s] : nullptr);
   if (W.iters && threadIdx.x == 0) W.iters[s] = W.iters_value;
 }
@@ -1777,7 +1865,8 @@ int batched_impl(sgpu_ctx* ctx, int32_t nsys, const int32_t* n, const int64_t* r
     double* Mw = (double*)scratch(ctx, "nlte_wide", sizeof(double) * stride * n_wide, &st);
     if (st) return st;
     const size_t plds = blocked_lds_bytes(max_n);
-    SGPU_REQUIRE(plds <= NLTE_LDS_BYTES, "wide system of %d levels exceeds LDS", max_n);
+    SGPU_REQUIRE(plds <= NLTE_LDS_BYTES && max_n <= 2 * WIDE_THREADS, "wide system of %d levels exceeds LDS",
+                 max_n);
     SGPU_HIP(hipFuncSetAttribute((const void*)nlte_wide_panel_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)plds));
     SGPU_HIP(hipFuncSetAttribute((const void*)nlte_wide_back_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
