@@ -119,7 +119,7 @@ void sgpu_destroy(sgpu_ctx* ctx)
   }
   if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
   if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
-  for (int i = 0; i < 2; ++i) {
+  for (int i = 0; i < sgpu_ctx::N_SIDE; ++i) {
     if (ctx->join_ev[i]) (void)hipEventDestroy(ctx->join_ev[i]);
     if (ctx->side_stream[i]) (void)hipStreamDestroy(ctx->side_stream[i]);
   }

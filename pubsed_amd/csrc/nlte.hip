@@ -666,7 +666,14 @@ __global__ void __launch_bounds__(NLTE_MAX_THREADS) nlte_batched_f64_kernel(Batc
 // instead of one.  Wide system w is order[w] (the largest first), its
 // augmented matrix at M + w stride.
 // ------------------------------------------------------------------
-constexpr int WIDE_THREADS = 512;   // panel, assembly and back-substitution workgroups
+constexpr int WIDE_THREADS = 512;   // back-substitution workgroups
+constexpr int WIDE_PANEL_THREADS = 256;   // panel workgroups (measured: 512 took 3 % longer)
+// wide systems in interleaved groups, one stream each (the main stream and
+// side streams 1 ..), so that groups' panels overlap other groups' updates
+// (measured on the per-atom batch: 2 groups 4.3 ms, 3 groups 7.3 ms,
+// 4 groups 7.1 ms -- more streams than hardware queues serialise badly)
+constexpr int WIDE_GROUPS = 2;
+static_assert(WIDE_GROUPS <= sgpu_ctx::N_SIDE, "a side stream per extra group");
 constexpr int WIDE_TILE = 64;       // trailing-update tile (rows and columns)
 constexpr int WIDE_UPD_THREADS = 256;
 
@@ -745,10 +752,10 @@ __global__ void __launch_bounds__(WIDE_ASM_THREADS) nlte_wide_assemble_kernel(Wi
   }
 }
 
-__global__ void __launch_bounds__(WIDE_THREADS) nlte_wide_panel_kernel(WideArgs W, int j0)
+__global__ void __launch_bounds__(WIDE_PANEL_THREADS) nlte_wide_panel_kernel(WideArgs W, int j0)
 {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  __shared__ ArgMax red[WIDE_THREADS / WAVE];
+  __shared__ ArgMax red[WIDE_PANEL_THREADS / WAVE];
   __shared__ int singular;
   const int w = wide_index(W, blockIdx.x);
   const int n = W.n[W.order[w]];
@@ -1859,8 +1866,8 @@ int batched_impl(sgpu_ctx* ctx, int32_t nsys, const int32_t* n, const int64_t* r
   }
   if (n_wide > 0) {
     // the wide systems are order[0 .. n_wide): one augmented matrix each;
-    // two interleaved groups (even and odd positions) on two streams, so
-    // that one group's panels overlap the other's trailing updates
+    // WIDE_GROUPS interleaved groups on as many streams, so that one group's
+    // panels overlap the others' trailing updates
     const size_t stride = ((size_t)max_n * (max_n + 1) + 31) / 32 * 32;
     double* Mw = (double*)scratch(ctx, "nlte_wide", sizeof(double) * stride * n_wide, &st);
     if (st) return st;
@@ -1871,10 +1878,11 @@ int batched_impl(sgpu_ctx* ctx, int32_t nsys, const int32_t* n, const int64_t* r
                                  (int)plds));
     SGPU_HIP(hipFuncSetAttribute((const void*)nlte_wide_back_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)plds));
-    const int groups = n_wide > 1 ? 2 : 1;
-    hipStream_t gs[2] = {ctx->stream, ctx->stream};
-    if (groups == 2) {
-      st = fork_side(ctx, 1, &gs[1]);
+    const int groups = std::min(n_wide, WIDE_GROUPS);
+    hipStream_t gs[WIDE_GROUPS];
+    gs[0] = ctx->stream;
+    for (int g = 1; g < groups; ++g) {
+      st = fork_side(ctx, g, &gs[g]);
       if (st) return st;
     }
     for (int g = 0; g < groups; ++g) {
@@ -1893,7 +1901,7 @@ int batched_impl(sgpu_ctx* ctx, int32_t nsys, const int32_t* n, const int64_t* r
         const unsigned ng = (unsigned)((n_wide - g + groups - 1) / groups);
         WideArgs W{n, order, rate_off, vec_off, rates, lte, b, pop, iters, Mw, stride,
                    precision == SGPU_NLTE_FP64 ? 0 : -1, groups, g};
-        hipLaunchKernelGGL(nlte_wide_panel_kernel, dim3(ng), dim3(WIDE_THREADS), plds, gs[g], W, j0);
+        hipLaunchKernelGGL(nlte_wide_panel_kernel, dim3(ng), dim3(WIDE_PANEL_THREADS), plds, gs[g], W, j0);
         SGPU_LAUNCH_CHECK();
         if (mr > 0) {
           hipLaunchKernelGGL(nlte_wide_update_kernel, dim3(tiles, ng), dim3(WIDE_UPD_THREADS), 0, gs[g], W, j0);
@@ -1908,8 +1916,8 @@ int batched_impl(sgpu_ctx* ctx, int32_t nsys, const int32_t* n, const int64_t* r
       hipLaunchKernelGGL(nlte_wide_back_kernel, dim3(ng), dim3(WIDE_THREADS), plds, gs[g], W);
       SGPU_LAUNCH_CHECK();
     }
-    if (groups == 2) {
-      st = join_side(ctx, 1);
+    for (int g = 1; g < groups; ++g) {
+      st = join_side(ctx, g);
       if (st) return st;
     }
     if (n_wide < nsys) {

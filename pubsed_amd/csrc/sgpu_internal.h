@@ -34,11 +34,12 @@ struct sgpu_ctx {
   std::map<std::string, sgpu_phase_timer> timers;
   std::map<std::string, sgpu_scratch> scratch;
   nlte_tuning nlte[2];   // per sgpu_nlte_batched precision (FP64, FP32_REFINE)
-  // two streams for work that runs beside the main stream's (default
+  // streams for work that runs beside the main stream's (default
   // priority), with the events that fork them from and join them to the
   // main stream; created on first use
-  hipStream_t side_stream[2] = {nullptr, nullptr};
-  hipEvent_t fork_ev = nullptr, join_ev[2] = {nullptr, nullptr};
+  static constexpr int N_SIDE = 4;
+  hipStream_t side_stream[N_SIDE] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[N_SIDE] = {};
   // HIP graph of a captured step (sgpu_graph_begin / sgpu_graph_end)
   int capturing = 0;
   hipGraphExec_t graph_exec = nullptr;
@@ -50,7 +51,7 @@ void set_error(const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
 // grow-only named device scratch owned by the context
 void* scratch(sgpu_ctx* ctx, const char* name, size_t bytes, int* status);
-// side stream i (0 or 1) made to wait for the work queued so far on ctx->stream
+// side stream i (< sgpu_ctx::N_SIDE) made to wait for the work queued so far on ctx->stream
 int fork_side(sgpu_ctx* ctx, int i, hipStream_t* side);
 // ctx->stream made to wait for the work queued so far on side stream i
 int join_side(sgpu_ctx* ctx, int i);
