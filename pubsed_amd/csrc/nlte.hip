@@ -293,8 +293,14 @@ __device__ __forceinline__ void assemble_augmented(int n, const double* R, const
 // interchanges applied to the columns on its right (the rhs included), and
 // U12 = L11^-1 A12.  The panel stays in LDS for the trailing update.
 // lds: blocked_lds_bytes(n) bytes of dynamic LDS.
+//
+// With pend (BLU_NB x BLU_NB doubles of LDS, or null), the previous panel's
+// trailing update has been applied to this panel's columns only (a delayed
+// update): its L columns j0 - BLU_NB .. j0 - 1 take this panel's
+// interchanges too, and U12 is formed from A12 - L_prev U12_prev on this
+// panel's rows, U12_prev being the previous panel's rows of each column.
 __device__ __forceinline__ void blocked_panel(int n, double* __restrict__ M, int j0, double* __restrict__ lds,
-                                              ArgMax* red, int* singular)
+                                              ArgMax* red, int* singular, double* pend = nullptr)
 {
   const int tid = threadIdx.x, NT = blockDim.x;
   const int ld = n + 1;
@@ -409,11 +415,42 @@ __device__ __forceinline__ void blocked_panel(int n, double* __restrict__ M, int
     // on the panel's rows (lanes 0..jb-1 of the composition hold rows
     // 0..jb-1, so v[0..jb) is the interchanged A12 column), all stored back
     const int nsw = *sw_n;
+    if (pend) {
+      // the previous panel's L columns take this panel's interchanges, then
+      // its rows of this panel (BLU_NB x BLU_NB) are staged in LDS
+      for (int k = j0 - BLU_NB + tid; k < j0; k += NT) {
+        double v[2 * BLU_NB];
+#pragma unroll
+        for (int q = 0; q < 2 * BLU_NB; ++q)
+          if (q < nsw) v[q] = M[(size_t)(j0 + sw_src[q]) * ld + k];
+#pragma unroll
+        for (int q = 0; q < 2 * BLU_NB; ++q)
+          if (q < nsw) M[(size_t)(j0 + sw_dst[q]) * ld + k] = v[q];
+      }
+      __syncthreads();
+      for (int idx = tid; idx < BLU_NB * BLU_NB; idx += NT) {
+        const int t = idx / BLU_NB, q = idx % BLU_NB;
+        pend[idx] = t < jb ? M[(size_t)(j0 + t) * ld + j0 - BLU_NB + q] : 0.0;
+      }
+      __syncthreads();
+    }
     for (int k = j0 + jb + tid; k <= n; k += NT) {
       double v[2 * BLU_NB];
 #pragma unroll
       for (int q = 0; q < 2 * BLU_NB; ++q)
         if (q < nsw) v[q] = M[(size_t)(j0 + sw_src[q]) * ld + k];
+      if (pend) {
+        // the delayed update of this panel's rows: v_t -= L_prev[t] U12_prev
+        double u[BLU_NB];
+#pragma unroll
+        for (int q = 0; q < BLU_NB; ++q) u[q] = M[(size_t)(j0 - BLU_NB + q) * ld + k];
+#pragma unroll
+        for (int t = 0; t < BLU_NB; ++t)
+          if (t < jb) {
+#pragma unroll
+            for (int q = 0; q < BLU_NB; ++q) v[t] -= pend[t * BLU_NB + q] * u[q];
+          }
+      }
 #pragma unroll
       for (int t = 1; t < BLU_NB; ++t)
         if (t < jb) {
@@ -752,48 +789,57 @@ __global__ void __launch_bounds__(WIDE_ASM_THREADS) nlte_wide_assemble_kernel(Wi
   }
 }
 
-__global__ void __launch_bounds__(WIDE_PANEL_THREADS) nlte_wide_panel_kernel(WideArgs W, int j0)
+// pending: the previous panel's trailing update was applied to this panel's
+// columns only (blocked_panel's delayed update)
+__global__ void __launch_bounds__(WIDE_PANEL_THREADS) nlte_wide_panel_kernel(WideArgs W, int j0, int pending)
 {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   __shared__ ArgMax red[WIDE_PANEL_THREADS / WAVE];
+  __shared__ double pend[BLU_NB * BLU_NB];
   __shared__ int singular;
   const int w = wide_index(W, blockIdx.x);
   const int n = W.n[W.order[w]];
   if (j0 >= n) return;
-  blocked_panel(n, W.M + (size_t)w * W.stride, j0, (double*)lds_raw, red, &singular);
+  blocked_panel(n, W.M + (size_t)w * W.stride, j0, (double*)lds_raw, red, &singular, pending ? pend : nullptr);
 }
 
 // grid (tiles of the largest system at this step, wide systems); tiles
-// beyond a system's own trailing matrix return at once.  Each wavefront
-// updates a 16-row strip of the 64 x 64 tile as four 16 x 16 blocks on
-// v_mfma_f64_16x16x4f64 (K = BLU_NB in four steps), operands from LDS.
-__global__ void __launch_bounds__(WIDE_UPD_THREADS) nlte_wide_update_kernel(WideArgs W, int j0)
+// beyond a system's own trailing matrix return at once.  A22 -= L21 U12 for
+// the K columns of L from j0 (one panel, or two with a delayed update),
+// all columns (c_last < 0) or the next panel's (up to c_last).  Each wavefront updates a 16-row strip of the
+// 64 x 64 tile as four 16 x 16 blocks on v_mfma_f64_16x16x4f64 (K / 4
+// steps), operands from LDS.
+template <int K>
+__global__ void __launch_bounds__(WIDE_UPD_THREADS) nlte_wide_update_kernel(WideArgs W, int j0, int c_last)
 {
   typedef double f64x4 __attribute__((ext_vector_type(4)));
-  __shared__ double Ls[WIDE_TILE][BLU_NB + 1];
-  __shared__ double Us[BLU_NB][WIDE_TILE + 2];
+  __shared__ double Ls[WIDE_TILE][K + 1];
+  __shared__ double Us[K][WIDE_TILE + 2];
   const int w = wide_index(W, blockIdx.y);
   const int n = W.n[W.order[w]];
   if (j0 >= n) return;
-  const int jb = min(BLU_NB, n - j0), i0 = j0 + jb;
-  const int mr = n - i0, mc = n + 1 - i0;
-  if (mr <= 0) return;
+  const int jb = min(K, n - j0), i0 = j0 + jb;
+  // last column updated: every column (the rhs included) for c_last < 0,
+  // else the next panel's columns only, up to c_last
+  const int c_end = c_last < 0 ? n : min(n - 1, c_last);
+  const int mr = n - i0, mc = c_end + 1 - i0;
+  if (mr <= 0 || mc <= 0) return;
   const int tc = (mc + WIDE_TILE - 1) / WIDE_TILE, tr = (mr + WIDE_TILE - 1) / WIDE_TILE;
   if ((int)blockIdx.x >= tr * tc) return;
   const int r0 = i0 + ((int)blockIdx.x / tc) * WIDE_TILE, c0 = i0 + ((int)blockIdx.x % tc) * WIDE_TILE;
   double* M = W.M + (size_t)w * W.stride;
   const int ld = n + 1, tid = threadIdx.x;
   // L21 rows r0.. (negated) and U12 columns c0.. of this tile, zero-padded
-  // to BLU_NB; the loads issued together, then stored
-  constexpr int PER = WIDE_TILE * BLU_NB / WIDE_UPD_THREADS;
+  // to K; the loads issued together, then stored
+  constexpr int PER = WIDE_TILE * K / WIDE_UPD_THREADS;
   double lv[PER], uv[PER];
 #pragma unroll
   for (int e = 0; e < PER; ++e) {
     const int idx = tid + e * WIDE_UPD_THREADS;
-    const int r = idx / BLU_NB, t = idx % BLU_NB;
+    const int r = idx / K, t = idx % K;
     lv[e] = (t < jb && r0 + r < n) ? -M[(size_t)(r0 + r) * ld + j0 + t] : 0.0;
     const int t2 = idx / WIDE_TILE, c = idx % WIDE_TILE;
-    uv[e] = (t2 < jb && c0 + c <= n) ? M[(size_t)(j0 + t2) * ld + c0 + c] : 0.0;
+    uv[e] = (t2 < jb && c0 + c <= c_end) ? M[(size_t)(j0 + t2) * ld + c0 + c] : 0.0;
   }
   // the tile's own entries, four 16 x 16 blocks of this wavefront's strip
   const int wave = tid / WAVE, lane = tid % WAVE, lr = lane % 16, lk = lane / 16;
@@ -805,23 +851,23 @@ __global__ void __launch_bounds__(WIDE_UPD_THREADS) nlte_wide_update_kernel(Wide
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int row = rs + lk + 4 * v;
-      acc[cb][v] = (row < n && col <= n) ? M[(size_t)row * ld + col] : 0.0;
+      acc[cb][v] = (row < n && col <= c_end) ? M[(size_t)row * ld + col] : 0.0;
     }
   }
 #pragma unroll
   for (int e = 0; e < PER; ++e) {
     const int idx = tid + e * WIDE_UPD_THREADS;
-    Ls[idx / BLU_NB][idx % BLU_NB] = lv[e];
+    Ls[idx / K][idx % K] = lv[e];
     Us[idx / WIDE_TILE][idx % WIDE_TILE] = uv[e];
   }
   __syncthreads();
-  double av[BLU_NB / 4];
+  double av[K / 4];
 #pragma unroll
-  for (int k4 = 0; k4 < BLU_NB / 4; ++k4) av[k4] = Ls[wave * 16 + lr][4 * k4 + lk];
+  for (int k4 = 0; k4 < K / 4; ++k4) av[k4] = Ls[wave * 16 + lr][4 * k4 + lk];
 #pragma unroll
   for (int cb = 0; cb < WIDE_TILE / 16; ++cb) {
 #pragma unroll
-    for (int k4 = 0; k4 < BLU_NB / 4; ++k4)
+    for (int k4 = 0; k4 < K / 4; ++k4)
       if (4 * k4 < jb)
         acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[k4], Us[4 * k4 + lk][cb * 16 + lr], acc[cb], 0, 0, 0);
   }
@@ -831,7 +877,7 @@ __global__ void __launch_bounds__(WIDE_UPD_THREADS) nlte_wide_update_kernel(Wide
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int row = rs + lk + 4 * v;
-      if (row < n && col <= n) M[(size_t)row * ld + col] = acc[cb][v];
+      if (row < n && col <= c_end) M[(size_t)row * ld + col] = acc[cb][v];
     }
   }
 }
@@ -1893,18 +1939,34 @@ int batched_impl(sgpu_ctx* ctx, int32_t nsys, const int32_t* n, const int64_t* r
                          dim3(WIDE_ASM_THREADS), 0, gs[g], W);
       SGPU_LAUNCH_CHECK();
     }
-    for (int j0 = 0; j0 < max_n; j0 += BLU_NB) {
-      // tiles of the largest system's trailing matrix at this step
-      const int mr = max_n - j0 - std::min(BLU_NB, max_n - j0), mc = mr + 1;
-      const unsigned tiles = (unsigned)(((mr + WIDE_TILE - 1) / WIDE_TILE) * ((mc + WIDE_TILE - 1) / WIDE_TILE));
+    // panels in pairs (A at j0, B at j0 + BLU_NB) with a delayed update: A
+    // factored; A's update applied to B's columns only; B factored, forming
+    // U12 of its rows with A's pending update; one rank-2 BLU_NB update of
+    // the rest with both panels -- half the trailing-matrix traffic of an
+    // update per panel, panels kept narrow
+    for (int j0 = 0; j0 < max_n; j0 += 2 * BLU_NB) {
+      // rows of the largest system below A, and below A and B
+      const int mr1 = max_n - j0 - std::min(BLU_NB, max_n - j0);
+      const int mr2 = max_n - j0 - std::min(2 * BLU_NB, max_n - j0);
+      const unsigned tiles1 = (unsigned)((mr1 + WIDE_TILE - 1) / WIDE_TILE);
+      const unsigned tiles2 = (unsigned)(((mr2 + WIDE_TILE - 1) / WIDE_TILE) * ((mr2 + WIDE_TILE) / WIDE_TILE));
       for (int g = 0; g < groups; ++g) {
         const unsigned ng = (unsigned)((n_wide - g + groups - 1) / groups);
         WideArgs W{n, order, rate_off, vec_off, rates, lte, b, pop, iters, Mw, stride,
                    precision == SGPU_NLTE_FP64 ? 0 : -1, groups, g};
-        hipLaunchKernelGGL(nlte_wide_panel_kernel, dim3(ng), dim3(WIDE_PANEL_THREADS), plds, gs[g], W, j0);
+        hipLaunchKernelGGL(nlte_wide_panel_kernel, dim3(ng), dim3(WIDE_PANEL_THREADS), plds, gs[g], W, j0, 0);
         SGPU_LAUNCH_CHECK();
-        if (mr > 0) {
-          hipLaunchKernelGGL(nlte_wide_update_kernel, dim3(tiles, ng), dim3(WIDE_UPD_THREADS), 0, gs[g], W, j0);
+        if (mr1 > 0) {
+          hipLaunchKernelGGL(nlte_wide_update_kernel<BLU_NB>, dim3(tiles1, ng), dim3(WIDE_UPD_THREADS), 0, gs[g],
+                             W, j0, j0 + 2 * BLU_NB - 1);
+          SGPU_LAUNCH_CHECK();
+          hipLaunchKernelGGL(nlte_wide_panel_kernel, dim3(ng), dim3(WIDE_PANEL_THREADS), plds, gs[g], W,
+                             j0 + BLU_NB, 1);
+          SGPU_LAUNCH_CHECK();
+        }
+        if (mr2 > 0) {
+          hipLaunchKernelGGL(nlte_wide_update_kernel<2 * BLU_NB>, dim3(tiles2, ng), dim3(WIDE_UPD_THREADS), 0,
+                             gs[g], W, j0, -1);
           SGPU_LAUNCH_CHECK();
         }
       }
