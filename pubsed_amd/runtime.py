@@ -148,7 +148,9 @@ class Problem:
             self.grid = abi.Grid(abi.GRID_1D_SPHERE, nz, ej["t"], 0.0, r_loc, abi.Locate(),
                                  abi.Locate(), abi.ptr(a["vol"]))
         else:
-            ej = W.ejecta_2d(nx=cfg["nx"], nzz=cfg["nzz"], elements=cfg["elements"])
+            # a model read from a file (models.read_model_2d), or the synthetic ejecta
+            ej = cfg["model"] if cfg.get("model") is not None else \
+                W.ejecta_2d(nx=cfg["nx"], nzz=cfg["nzz"], elements=cfg["elements"])
             self.ej = ej
             nz = ej["nz"]
             a["x_out"] = S.asarray(ej["x_out"])
@@ -159,7 +161,7 @@ class Problem:
             self.grid = abi.Grid(abi.GRID_2D_CYLN, nz, ej["t"], 0.0, abi.Locate(), xl, zl,
                                  abi.ptr(a["vol"]))
         self.nz = nz
-        ne = len(cfg["elements"])
+        ne = len(ej["elem_Z"])
         a["rho"] = S.asarray(ej["rho"])
         a["T"] = S.asarray(ej["T"])
         a["X"] = S.asarray(ej["X"].ravel())

@@ -109,3 +109,57 @@ def test_lucy_supernova_light_curve(gpu, precision):
         mx, mn = rel_errors(v, Tr, ref["velr"].astype(np.float64), ref["T_rad"].astype(np.float64), 1e8, 9.8e8)
         print("T_rad after step %d: max %.3f mean %.3f" % (s, mx, mn))
         assert mx < 0.5 and mn < 0.02, (s, mx, mn)
+
+
+@pytest.mark.parametrize("precision", [abi.PREC_MIXED, abi.PREC_FP64_BATCHED])
+def test_lucy_supernova_2d_model_file(gpu, precision):
+    """The 2D Lucy supernova of tests/lucy_supernova/2D: the reference's own
+    model file models/lucy_2D.h5 (70 x 140 (varpi, z) zones, 1.37 Msun,
+    0.61 Msun of 56Ni, t = 1 d; copied under tests/golden/lucy/) read by
+    pubsed_amd.models.read_model_2d, grey kappa = 0.1, homologous, radiative
+    equilibrium, the escape spectrum in 10 mu bins (param.lua); the
+    criteria of run_test.py: every mu bin's light curve against
+    comparefiles/lucy_lc.dat within 10 % mean over 3 d < t < 55 d, and the
+    deposition against lucy_gr.dat within 25 % max / 10 % mean.  3e4
+    radioactive packets per step (the reference's test uses 1e4)."""
+    from pubsed_amd.models import read_model_2d
+    n_emit = 30000
+    ej = read_model_2d(os.path.join(GOLD, "lucy_2D.h5"))
+    P = runtime.Problem(runtime.Space("cuda:0"), "c4_2d", model=ej, nu=(1, 1, 1), grey=0.1, epsilon=1.0,
+                        escat=0, fuzz=0, radiative_eq=1, steady=0, spec_t0=-0.5 * DAY, spec_t1=100 * DAY,
+                        spec_dt=0.5 * DAY, spec_n_mu=10)
+    assert P.nz == 70 * 140 and abs(P.t_now - DAY) < 1e-6
+    P.tparams.precision = precision
+    cap = 150 * n_emit
+    P.alloc_packets(cap)
+    scr = P.alloc_packets(cap, name="scr")
+    t, k = 1.0 * DAY, 0
+    dep_t, dep_L = [], []
+    print("case lucy 2D precision=%d" % precision, flush=True)
+    while t < 58 * DAY:
+        dt = min(0.5 * DAY, 0.05 * t)
+        got, alive, esc = runtime.time_step(gpu, P, dt, n_emit, scr, k)
+        assert got == n_emit and alive <= cap - n_emit
+        k += 1
+        t += dt
+        dep_t.append((t - 0.5 * dt) / DAY)
+        dep_L.append(float((P.get("L_radio_dep") * P.get("vol")).sum()))
+    gpu.sync()
+    print("steps", k, "packets on the grid", P.n_packets, flush=True)
+    tg = P.spec_grids["time"]
+    edges = np.concatenate([[tg["min"]], tg["x"]])
+    tc = 0.5 * (edges[1:] + edges[:-1]) / DAY
+    n_mu = P.spec_shape[2]
+    flux = P.get("opt_flux").reshape(P.spec_shape)
+    tl, Ll = np.loadtxt(os.path.join(GOLD, "lucy_lc.dat"), unpack=True)
+    # each mu bin's isotropic-equivalent luminosity: flux / (dt / n_mu), as
+    # spectrum_array::print normalises it (spectrum_array.cpp:269-284)
+    for i in range(n_mu):
+        L = flux[:, 0, i, 0] * n_mu / np.diff(edges)
+        mx, mn = rel_errors(tc, L, tl, Ll, 3, 55)
+        print("mu bin %d: max %.3f mean %.3f" % (i, mx, mn))
+        assert mn < 0.10, (i, mx, mn)
+    tg2, Lg = np.loadtxt(os.path.join(GOLD, "lucy_gr.dat"), unpack=True)
+    mx, mn = rel_errors(np.array(dep_t), np.array(dep_L), tg2, Lg, 3, 55)
+    print("deposition: max %.3f mean %.3f" % (mx, mn))
+    assert mx < 0.25 and mn < 0.10, (mx, mn)
