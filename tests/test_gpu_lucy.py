@@ -66,19 +66,25 @@ def rel_errors(x, y, xc, yc, lo, hi):
     return float(e.max()), float(e.mean())
 
 
-@pytest.mark.parametrize("precision", [abi.PREC_MIXED, abi.PREC_FP64_BATCHED])
-def test_lucy_supernova_light_curve(gpu, precision):
+# (precision, transport_use_ddmc): the mixed and batched FP64 forms of
+# tests/lucy_supernova/1D, and tests/lucy_supernova/1D_ddmc (the same problem
+# and criteria with transport_use_ddmc = 2, transport_ddmc_tau_threshold = 3,
+# in the FP64 drop-in, the form that carries DDMC)
+@pytest.mark.parametrize("precision,ddmc", [(abi.PREC_MIXED, 0), (abi.PREC_FP64_BATCHED, 0), (abi.PREC_FP64, 2)])
+def test_lucy_supernova_light_curve(gpu, precision, ddmc):
     n_emit = 30000
     P = runtime.Problem(runtime.Space("cuda:0"), "c1_grey", nu=(1, 1, 1), grey=0.1, epsilon=1.0, steady=0,
                         radiative_eq=1, spec_t0=-0.5 * DAY, spec_t1=100 * DAY, spec_dt=0.5 * DAY)
     v = lucy_model(P, 1.0 * DAY)
     P.tparams.precision = precision
+    P.tparams.use_ddmc = ddmc
+    P.tparams.ddmc_tau_threshold = 3.0
     cap = 150 * n_emit
     P.alloc_packets(cap)
     scr = P.alloc_packets(cap, name="scr")
     t, k = 1.0 * DAY, 0
     dep_t, dep_L, trad = [], [], {}
-    print("case lucy precision=%d" % precision, flush=True)
+    print("case lucy precision=%d ddmc=%d" % (precision, ddmc), flush=True)
     while t < 58 * DAY:
         dt = min(0.5 * DAY, 0.05 * t)
         got, alive, esc = runtime.time_step(gpu, P, dt, n_emit, scr, k)
